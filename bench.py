@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the 2D fan-beam projection pair on one MI355X.
+
+Workload (one "step" = one forward projection + one voxel-driven backprojection):
+  image 1024 x 1024 FP32 (10 random ellipses, seed 0), pixel spacing 1; flat detector of 1449
+  cells, spacing 1; 720 angles over [0, 2 pi); source-to-isocentre 2048, source-to-detector
+  3072. Inputs are resident in HBM before timing starts.
+
+Prints one JSON line: value = forward+backward pairs per second (wall clock over K steps,
+synchronised on both sides), per-op device times from HIP events on the stream the kernels
+run on, the HBM-roofline figure of the dominant kernel, and a CPU baseline that times the
+oracle's FP32 port of the reference arithmetic on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+WORKLOAD = dict(n=1024, D=1449, A=720, DSO=2048.0, DSD=3072.0)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_case(torch, cm, phantom, np):
+    w = WORKLOAD
+    angles = np.linspace(0, 2 * np.pi, w["A"], endpoint=False)
+    geom = cm.FanGeometry(DSD=w["DSD"], DSO=w["DSO"], angles=angles,
+                          vol_shape=(w["n"], w["n"]), det_shape=w["D"])
+    img = torch.tensor(phantom.ellipses_2d(w["n"], seed=0), dtype=torch.float32,
+                       device="cuda:0")
+    sino = torch.empty(w["A"], w["D"], dtype=torch.float32, device="cuda:0")
+    vol = torch.zeros(w["n"], w["n"], dtype=torch.float32, device="cuda:0")
+    return geom, img, sino, vol
+
+
+def algorithmic_bytes(info, w, s=4):
+    """Bytes a texture-less gather moves if every interpolation tap came from HBM:
+    forward  sum_rays N * 4 taps * s + A * D * s
+    backward A * n^2 * 2 taps * s + 2 * n^2 * s (read-modify-write of the image)."""
+    n2 = w["n"] * w["n"]
+    fwd = info["total_samples"] * 4 * s + w["A"] * w["D"] * s
+    bwd = w["A"] * n2 * 2 * s + 2 * n2 * s
+    return fwd, bwd
+
+
+def cpu_baseline(np, geom_rec):
+    """Times the oracle's FP32 port (OpenMP) on a bounded sample: forward on 144 of 720
+    angles and backprojection on 256 of 1024 image rows, scaled to whole operators."""
+    from oracle import oracle
+    from curadon_amd import phantom
+    w = WORKLOAD
+    img = phantom.ellipses_2d(w["n"], seed=0).astype(np.float32)
+    fa = 144
+    fr = 256
+    oracle.fwd2d_f32(geom_rec, img, angles=(0, 8))  # warm-up (thread pool, pages)
+    t0 = time.perf_counter()
+    sino = oracle.fwd2d_f32(geom_rec, img, angles=(0, fa))
+    t1 = time.perf_counter()
+    oracle.bp2d_f32(geom_rec, sino, rows=(0, fr))
+    t2 = time.perf_counter()
+    t_fwd = (t1 - t0) * w["A"] / fa
+    t_bwd = (t2 - t1) * w["n"] / fr
+    return {
+        "value": 1.0 / (t_fwd + t_bwd), "unit": "fwd+bwd pairs/s",
+        "cores": oracle.num_threads(), "kind": "port",
+        "sample": f"FP32 port of the reference arithmetic, forward on {fa}/{w['A']} angles "
+                  f"and backprojection on {fr}/{w['n']} rows, scaled to full operators",
+        "fwd_s": t_fwd, "bwd_s": t_bwd, "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the projectors run on a single MI355X here and "
+              "no multi-GPU figure is produced (run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import curadon_amd as cm
+    from curadon_amd import phantom
+
+    log("bench: building plan and inputs")
+    geom, img, sino, vol = build_case(torch, cm, phantom, np)
+    info = geom.plan.info()
+    fwd_bytes, bwd_bytes = algorithmic_bytes(info, WORKLOAD)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        cm.forward(img, sino, geom)
+        cm.backward(sino, vol, geom)
+    torch.cuda.synchronize()
+    log(f"bench: warm-up done ({args.warmup} steps), timing {args.steps} steps")
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        cm.forward(img, sino, geom)
+        ev[k][1].record(stream)
+        cm.backward(sino, vol, geom)
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = np.array([e[0].elapsed_time(e[1]) for e in ev])
+    bwd_ms = np.array([e[1].elapsed_time(e[2]) for e in ev])
+    step_ms = fwd_ms + bwd_ms
+    log(f"bench: timed region {wall:.3f} s; fwd {fwd_ms.mean():.3f} ms, bwd {bwd_ms.mean():.3f} ms")
+
+    fwd_gbs = fwd_bytes / (fwd_ms.mean() * 1e-3) / 1e9
+    bwd_gbs = bwd_bytes / (bwd_ms.mean() * 1e-3) / 1e9
+    dominant = "fwd2d" if fwd_ms.mean() >= bwd_ms.mean() else "bp2d"
+    dom_gbs = fwd_gbs if dominant == "fwd2d" else bwd_gbs
+    dom_bytes = fwd_bytes if dominant == "fwd2d" else bwd_bytes
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle FP32 port)")
+        import types
+        w = WORKLOAD
+        rec = types.SimpleNamespace(
+            vol_shape=np.array([w["n"], w["n"]]), vol_spacing=np.ones(2), vol_offset=np.zeros(2),
+            det_shape=w["D"], det_spacing=1.0, det_offset=0.0,
+            angles=np.linspace(0, 2 * np.pi, w["A"], endpoint=False),
+            DSO=np.full(w["A"], w["DSO"]), DSD=np.full(w["A"], w["DSD"]))
+        cpu = cpu_baseline(np, rec)
+
+    props = torch.cuda.get_device_properties(0)
+    pair_ms = wall / args.steps * 1e3
+    out = {
+        "metric": "fwd+bwd projections/s (2D fan, FP32)",
+        "value": args.steps / wall,
+        "unit": "fwd+bwd pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": pair_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-ellipse phantom)",
+        "config": {
+            "workload": "fan2d 1024x1024 f32, 1449 cells, 720 angles over 2pi, DSO 2048, "
+                        "DSD 3072; step = forward + voxel-driven backprojection",
+            "batch": 1,
+        },
+        "ops": {
+            "fwd2d": {"ms_mean": float(fwd_ms.mean()), "ms_p50": float(np.percentile(fwd_ms, 50)),
+                      "ms_p99": float(np.percentile(fwd_ms, 99)), "op_per_s": 1e3 / fwd_ms.mean(),
+                      "alg_GB_per_s": fwd_gbs, "alg_bytes": fwd_bytes},
+            "bp2d": {"ms_mean": float(bwd_ms.mean()), "ms_p50": float(np.percentile(bwd_ms, 50)),
+                     "ms_p99": float(np.percentile(bwd_ms, 99)), "op_per_s": 1e3 / bwd_ms.mean(),
+                     "alg_GB_per_s": bwd_gbs, "alg_bytes": bwd_bytes},
+            "pair": {"ms_p50": float(np.percentile(step_ms, 50)),
+                     "ms_p99": float(np.percentile(step_ms, 99))},
+            "total_samples": info["total_samples"],
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dominant, "achieved": dom_gbs, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None,
+            "alg_bytes_per_launch": dom_bytes,
+            "note": "achieved = algorithmic bytes (every interpolation tap counted as an HBM "
+                    "read) / mean op time from HIP events on the launch stream",
+        },
+        "cpu_baseline": cpu,
+        "device": {"name": props.name, "cus": props.multi_processor_count,
+                   "gcn_arch": getattr(props, "gcnArchName", "")},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,19 @@
+"""curadon_amd -- MI355X (gfx950) projection operators with curadon's Python API.
+
+Drop-in names (python/standalone/src/curadon/__init__.py:1-15): FanGeometry, ConeGeometry,
+forward, backward, TorchForward, TorchBackprojection, plan_2d, forward_2d, backward_2d,
+forward_3d, backward_3d. Additions: adjoint / adjoint_2d / TorchAdjoint (matched transpose).
+Usage: ``import curadon_amd as curadon``.
+"""
+from .geometry import ConeGeometry, FanGeometry
+from . import _lib
+from ._lib import CuradonError, adjoint_2d, backward_2d, forward_2d, plan_2d
+from .forward import forward
+from .backward import adjoint, backward
+from .diffable import TorchAdjoint, TorchBackprojection, TorchForward
+
+__all__ = [
+    "FanGeometry", "ConeGeometry", "forward", "backward", "adjoint", "TorchForward",
+    "TorchBackprojection", "TorchAdjoint", "plan_2d", "forward_2d", "backward_2d",
+    "adjoint_2d", "CuradonError",
+]

@@ -1,0 +1,327 @@
+"""ctypes binding of libcuradon_mi.so, the replacement for curadon's nanobind module
+``curadon_ext`` (python/standalone/src/bind.cpp:58-66).
+
+Names, argument order and error types follow the reference binding:
+``plan_2d(device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec, det_count,
+det_spacing, det_offset, DSO, DSD, angles, det_pitch, COR)`` with r/w ``forward_block_x`` /
+``forward_block_y`` (python/standalone/src/plan.cu:30-59), ``forward_2d(volume, sinogram,
+plan)`` (forward.cu:158-210), ``backward_2d(volume, sinogram, plan)`` (backward.cu:299-351),
+``forward_3d`` / ``backward_3d`` with 15 arguments (forward.cu:89-141, backward.cu:230-282).
+
+Arrays are torch tensors on a ROCm device; work is enqueued on the tensor device's current
+torch stream. There is no CPU fallback: without the native library, importing this module
+raises ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import torch  # loads the HIP runtime that libcuradon_mi.so binds to (same soname)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libcuradon_mi.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"curadon_amd: native library not found at {LIB_PATH}; build it with "
+        "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C curadon_amd/csrc`"
+    )
+
+_lib = C.CDLL(LIB_PATH)
+
+# ----------------------------------------------------------------------------- enums ----
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_DTYPE = 2
+ERR_DEVICE = 3
+ERR_HIP = 4
+ERR_OUT_OF_MEMORY = 5
+ERR_NOT_SUPPORTED = 6
+
+F16, F32, BF16 = 16, 32, 1016
+BP3D_REFERENCE, BP3D_MATCHED = 0, 1
+
+_DTYPE_CODE = {torch.float32: F32, torch.float16: F16, torch.bfloat16: BF16}
+_CODE_DTYPE = {v: k for k, v in _DTYPE_CODE.items()}
+_CODE_NAME = {F32: "f32", F16: "f16", BF16: "bf16"}
+
+
+class CuradonError(RuntimeError):
+    """A failure reported by the native library (HIP runtime error, out of memory...)."""
+
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+class CuradonOutOfMemory(CuradonError, MemoryError):
+    pass
+
+
+_u64 = C.c_uint64
+_i32 = C.c_int
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class Plan2DInfo(C.Structure):
+    _fields_ = [
+        ("device", C.c_int), ("vol_prec", C.c_int), ("det_prec", C.c_int),
+        ("nx", _u64), ("ny", _u64), ("det_count", _u64), ("nangles", _u64),
+        ("total_samples", _u64), ("table_bytes", _u64),
+    ]
+
+
+class Cone3DDesc(C.Structure):
+    _fields_ = [
+        ("vol_shape", _u64 * 3), ("vol_spacing", C.c_double * 3), ("vol_offset", C.c_double * 3),
+        ("det_shape", _u64 * 2), ("det_spacing", C.c_double * 2), ("det_offset", C.c_double * 2),
+        ("det_rotation", C.c_double * 3), ("DSO", C.c_double), ("DSD", C.c_double),
+        ("COR", C.c_double), ("nangles", _u64), ("phi", _dp), ("theta", _dp), ("psi", _dp),
+    ]
+
+
+class Plan3DInfo(C.Structure):
+    _fields_ = [
+        ("device", C.c_int), ("vol_prec", C.c_int), ("det_prec", C.c_int),
+        ("nx", _u64), ("ny", _u64), ("nz", _u64), ("nu", _u64), ("nv", _u64),
+        ("nangles", _u64), ("total_samples", _u64), ("table_bytes", _u64),
+    ]
+
+
+# Every symbol the public header declares, with its ctypes signature.
+SIGNATURES = {
+    "cmi_last_error": (C.c_char_p, []),
+    "cmi_version": (C.c_int, []),
+    "cmi_plan2d_create": (_i32, [_i32, _i32, _u64p, _dp, _dp, _i32, _u64, C.c_double, C.c_double,
+                                 _dp, _dp, _dp, _u64, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "cmi_plan2d_destroy": (_i32, [_vp]),
+    "cmi_plan2d_set_block": (_i32, [_vp, C.c_uint32, C.c_uint32]),
+    "cmi_plan2d_get_block": (_i32, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "cmi_plan2d_get_info": (_i32, [_vp, C.POINTER(Plan2DInfo)]),
+    "cmi_forward_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_backward_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_adjoint_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_plan3d_create": (_i32, [_i32, _i32, _i32, C.POINTER(Cone3DDesc), C.POINTER(_vp)]),
+    "cmi_plan3d_destroy": (_i32, [_vp]),
+    "cmi_forward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_backward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
+    "cmi_adjoint_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+}
+
+for _name, (_res, _args) in SIGNATURES.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+def last_error() -> str:
+    return _lib.cmi_last_error().decode("utf-8", "replace")
+
+
+def version() -> int:
+    return _lib.cmi_version()
+
+
+def check(status: int):
+    """Raise the Python error the reference binding would raise for this failure class."""
+    if status == OK:
+        return
+    msg = last_error()
+    if status == ERR_DEVICE:
+        raise AttributeError(msg)        # nb::attribute_error, forward.cu:161-171
+    if status == ERR_DTYPE:
+        raise TypeError(msg)
+    if status == ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if status == ERR_OUT_OF_MEMORY:
+        raise CuradonOutOfMemory(status, msg)
+    if status == ERR_NOT_SUPPORTED:
+        raise NotImplementedError(msg)
+    raise CuradonError(status, msg)
+
+
+def precision_code(prec) -> int:
+    """Map a precision argument to the library's dtype code.
+
+    Accepts the reference's ints (32 -> f32, 16 -> f16; plan.cu:20-28), torch dtypes and the
+    strings 'f32', 'f16', 'bf16'."""
+    if isinstance(prec, torch.dtype):
+        if prec in _DTYPE_CODE:
+            return _DTYPE_CODE[prec]
+    elif isinstance(prec, str):
+        m = {"f32": F32, "float32": F32, "f16": F16, "float16": F16, "bf16": BF16,
+             "bfloat16": BF16}
+        if prec.lower() in m:
+            return m[prec.lower()]
+    else:
+        try:
+            v = int(prec)
+        except (TypeError, ValueError):
+            v = None
+        if v in (32, 16):
+            return F32 if v == 32 else F16
+        if v == BF16:
+            return BF16
+    raise AttributeError("Precision must either be f32, f16 or bf16")  # plan.cu:27
+
+
+def dtype_of_code(code: int) -> torch.dtype:
+    return _CODE_DTYPE[code]
+
+
+def tensor_code(t: torch.Tensor) -> int:
+    if t.dtype not in _DTYPE_CODE:
+        raise TypeError("Only float32, float16 and bfloat16 are supported")  # forward.cu:200
+    return _DTYPE_CODE[t.dtype]
+
+
+def _require_device_tensor(t, what):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise TypeError(f"{what} must be a ROCm device tensor (got {t.device})")
+    if not t.is_contiguous():
+        raise TypeError(f"{what} must be C-contiguous")
+
+
+def stream_of(t: torch.Tensor) -> int:
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+    if n is not None and a.size != n:
+        raise AttributeError(f"expected {n} values, got {a.size}")
+    return a
+
+
+# ----------------------------------------------------------------------------- 2D -------
+
+class plan_2d:
+    """Per-geometry state of the 2D fan-beam operators (replaces curad::plan_2d).
+
+    Construction builds the per-angle geometry in FP64 on the host and the per-ray table on
+    ``device``; the plan is immutable afterwards apart from the launch-shape tunables."""
+
+    def __init__(self, device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec,
+                 det_count, det_spacing, det_offset, DSO, DSD, angles, det_pitch=0.0, COR=0.0):
+        self._h = None
+        angles = _f64(angles)
+        A = angles.size
+        DSO = _f64(DSO, A)
+        DSD = _f64(DSD, A)
+        vs = np.ascontiguousarray(np.asarray(vol_shape, dtype=np.uint64).reshape(-1))
+        if vs.size != 2:
+            raise AttributeError("vol_shape must be two values")
+        sp = _f64(vol_spacing, 2)
+        of = _f64(vol_offset, 2)
+        self.device = int(device)
+        self.vol_prec = precision_code(vol_prec)
+        self.det_prec = precision_code(det_prec)
+        self.vol_shape = (int(vs[0]), int(vs[1]))
+        self.det_count = int(det_count)
+        self.nangles = A
+        h = _vp()
+        check(_lib.cmi_plan2d_create(
+            self.device, self.vol_prec, vs.ctypes.data_as(_u64p), sp.ctypes.data_as(_dp),
+            of.ctypes.data_as(_dp), self.det_prec, self.det_count, float(det_spacing),
+            float(det_offset), DSO.ctypes.data_as(_dp), DSD.ctypes.data_as(_dp),
+            angles.ctypes.data_as(_dp), A, float(det_pitch), float(COR), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _block(self):
+        bx, by = C.c_uint32(), C.c_uint32()
+        check(_lib.cmi_plan2d_get_block(self._h, C.byref(bx), C.byref(by)))
+        return bx.value, by.value
+
+    @property
+    def forward_block_x(self):
+        return self._block()[0]
+
+    @forward_block_x.setter
+    def forward_block_x(self, v):
+        check(_lib.cmi_plan2d_set_block(self._h, int(v), self._block()[1]))
+
+    @property
+    def forward_block_y(self):
+        return self._block()[1]
+
+    @forward_block_y.setter
+    def forward_block_y(self, v):
+        check(_lib.cmi_plan2d_set_block(self._h, self._block()[0], int(v)))
+
+    def info(self) -> dict:
+        i = Plan2DInfo()
+        check(_lib.cmi_plan2d_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in Plan2DInfo._fields_}
+
+    def sinogram_shape(self):
+        return (self.nangles, self.det_count)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h is not None and _lib is not None:
+            _lib.cmi_plan2d_destroy(h)
+
+
+def _check_pair_2d(volume, sinogram, plan, which):
+    _require_device_tensor(volume, "volume")
+    _require_device_tensor(sinogram, "sinogram")
+    if volume.device != sinogram.device:
+        raise AttributeError("Volume and sinogram must be on the same device")   # forward.cu:161
+    if volume.device.index != plan.device:
+        raise AttributeError("Volume and plan are not on the same device")         # :165
+    vc, sc = tensor_code(volume), tensor_code(sinogram)
+    if vc != plan.vol_prec:
+        raise AttributeError(
+            f"Plan precision does not match volume precision ({_CODE_NAME[vc]})")  # :174-176
+    if sc != plan.det_prec:
+        raise AttributeError(
+            f"Plan precision does not match sinogram precision ({_CODE_NAME[sc]})")
+    ny, nx = plan.vol_shape
+    A, D = plan.nangles, plan.det_count
+    if volume.dim() < 2 or tuple(volume.shape[-2:]) != (ny, nx):
+        raise ValueError(f"{which}: volume shape {tuple(volume.shape)} does not end in {(ny, nx)}")
+    if sinogram.dim() < 2 or tuple(sinogram.shape[-2:]) != (A, D):
+        raise ValueError(f"{which}: sinogram shape {tuple(sinogram.shape)} does not end in {(A, D)}")
+    bv = volume.numel() // (ny * nx)
+    bs = sinogram.numel() // (A * D)
+    if bv != bs or tuple(volume.shape[:-2]) != tuple(sinogram.shape[:-2]):
+        raise ValueError(f"{which}: batch shapes differ: volume {tuple(volume.shape)}, "
+                         f"sinogram {tuple(sinogram.shape)}")
+    return vc, sc, bv
+
+
+def forward_2d(volume, sinogram, plan):
+    """sinogram := A volume (overwrites). Batched over leading dimensions."""
+    vc, sc, b = _check_pair_2d(volume, sinogram, plan, "forward_2d")
+    if b == 0:
+        return
+    check(_lib.cmi_forward_2d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc, b,
+                              stream_of(volume)))
+
+
+def backward_2d(volume, sinogram, plan):
+    """volume += B sinogram, the reference's voxel-driven backprojection."""
+    vc, sc, b = _check_pair_2d(volume, sinogram, plan, "backward_2d")
+    if b == 0:
+        return
+    check(_lib.cmi_backward_2d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc, b,
+                               stream_of(volume)))
+
+
+def adjoint_2d(volume, sinogram, plan):
+    """volume += A^T sinogram, the exact transpose of forward_2d."""
+    vc, sc, b = _check_pair_2d(volume, sinogram, plan, "adjoint_2d")
+    if b == 0:
+        return
+    check(_lib.cmi_adjoint_2d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc, b,
+                              stream_of(volume)))

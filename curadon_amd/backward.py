@@ -1,0 +1,53 @@
+"""``backward`` (python/standalone/src/curadon/backward.py:7-29): voxel-driven
+backprojection, accumulated (+=) into a caller-provided volume, which is returned.
+
+Also accepted: ``backward(sinogram, geom)`` (the volume is then allocated with zeros) and
+``backward(sinogram, geom, volume=...)``. For 3D, ``mode`` selects the reference's projection
+('reference', the default) or the forward model's geometry ('matched').
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib as _C
+from .forward import _split
+from .geometry import ConeGeometry, FanGeometry
+
+
+def backward(sinogram, volume=None, geom=None, *, volume_out=None, mode="reference"):
+    volume, geom = _split(volume, geom, None, volume_out, "volume")
+    if isinstance(geom, FanGeometry):
+        if volume is None:
+            volume = torch.zeros(tuple(sinogram.shape[:-2]) + tuple(int(v) for v in geom.vol_shape),
+                                 dtype=_C.dtype_of_code(geom.plan.vol_prec),
+                                 device=sinogram.device)
+        _C.backward_2d(volume, sinogram, geom.plan)
+        return volume
+    if isinstance(geom, ConeGeometry):
+        if volume is None:
+            volume = torch.zeros(tuple(sinogram.shape[:-3]) + tuple(int(v) for v in geom.vol_shape),
+                                 dtype=_C.dtype_of_code(geom.plan.vol_prec),
+                                 device=sinogram.device)
+        _C.backward_3d_plan(volume, sinogram, geom.plan, mode=mode)
+        return volume
+    raise ValueError("Unknown geometry type")
+
+
+def adjoint(sinogram, volume=None, geom=None, *, volume_out=None):
+    """Matched adjoint A^T of ``forward`` (exact transpose), accumulated into ``volume``."""
+    volume, geom = _split(volume, geom, None, volume_out, "volume")
+    if isinstance(geom, FanGeometry):
+        if volume is None:
+            volume = torch.zeros(tuple(sinogram.shape[:-2]) + tuple(int(v) for v in geom.vol_shape),
+                                 dtype=_C.dtype_of_code(geom.plan.vol_prec),
+                                 device=sinogram.device)
+        _C.adjoint_2d(volume, sinogram, geom.plan)
+        return volume
+    if isinstance(geom, ConeGeometry):
+        if volume is None:
+            volume = torch.zeros(tuple(sinogram.shape[:-3]) + tuple(int(v) for v in geom.vol_shape),
+                                 dtype=_C.dtype_of_code(geom.plan.vol_prec),
+                                 device=sinogram.device)
+        _C.adjoint_3d_plan(volume, sinogram, geom.plan)
+        return volume
+    raise ValueError("Unknown geometry type")

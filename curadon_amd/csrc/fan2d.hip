@@ -1,0 +1,573 @@
+// 2D fan-beam projectors for gfx950: plan, per-ray table, forward ray march, voxel-driven
+// backprojection and the matched (transposed) ray march.
+//
+// Data layout in HBM
+//   plan tables  geo  [A] AngleGeom (FP64)       source / cell-0 vector / cell step
+//                meta [A] float4                 (-sin, cos, DSD, DSO) for the backprojector
+//                ageo [A] float[6]               FP32 copy of geo for the adjoint's ray search
+//                ev   [A*D] float4               (ex + 1.5, ey + 1.5, vx, vy) per ray: the
+//                                                entry point shifted into padded-copy
+//                                                coordinates (texel centre at +0.5, border 2)
+//                ns   [A*D] float2               (world step length, N as int bits) per ray
+//   per call     for each image b of the batch, one block of 2 * (nx+4) * (ny+4) elements:
+//                pad  [ny+4][nx+4]               image with a 2-texel zero border
+//                padT [nx+4][ny+4]               the same, transposed
+// The forward kernel reads the copy whose rows run along each ray's dominant axis, so the 64
+// rays of a wavefront (adjacent detector cells of one angle) gather from a few cache lines per
+// step whichever way the fan points, and the zero border replaces per-tap range checks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "cmi_internal.hpp"
+#include "geometry2d.hpp"
+
+using namespace cmi;
+using cmi::fan2d::AngleGeom;
+
+struct cmi_plan2d {
+    int device = 0;
+    int vol_prec = CMI_F32, det_prec = CMI_F32;
+    fan2d::Scalars g{};
+    int64_t A = 0;
+    double pitch = 0.0, cor = 0.0;
+    uint32_t block_x = 64, block_y = 4;
+    uint64_t total_samples = 0;
+    uint64_t table_bytes = 0;
+    AngleGeom *d_geo = nullptr;
+    float4 *d_meta = nullptr;
+    float *d_ageo = nullptr;
+    float4 *d_ev = nullptr;
+    float2 *d_ns = nullptr;
+};
+
+namespace {
+
+constexpr int kPad = 2; // zero border of the staged image copies, in texels
+
+// ---------------------------------------------------------------- ray table ----------------
+
+__global__ __launch_bounds__(256) void k_ray_table(const AngleGeom *__restrict__ geo, int A,
+                                                   int D, int64_t nx, int64_t ny, double sx,
+                                                   double sy, float4 *__restrict__ ev,
+                                                   float2 *__restrict__ ns,
+                                                   unsigned long long *__restrict__ total) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    unsigned long long n = 0;
+    if (k < D && a < A) {
+        const fan2d::Ray r = fan2d::setup_ray(geo[a], nx, ny, sx, sy, k);
+        const size_t i = (size_t)a * D + k;
+        const double shift = kPad - 0.5; // texel-centre offset and zero border
+        ev[i] = make_float4((float)(r.e[0] + shift), (float)(r.e[1] + shift), (float)r.v[0],
+                            (float)r.v[1]);
+        ns[i] = make_float2((float)r.scale, __int_as_float((int)r.n));
+        n = (unsigned long long)r.n;
+    }
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(total, n);
+}
+
+// ---------------------------------------------------------------- staging ------------------
+
+// pad[py][px] = img[py-2][px-2] (0 outside) and padT[px][py] = pad[py][px], tile by tile.
+template <class S>
+__global__ __launch_bounds__(256) void k_stage2d(const typename S::raw *__restrict__ img,
+                                                 typename S::raw *__restrict__ scratch, int nx,
+                                                 int ny, int Wp, int Hp) {
+    using raw = typename S::raw;
+    __shared__ raw tile[32][33];
+    const size_t b = blockIdx.z;
+    img += b * (size_t)nx * ny;
+    raw *pad = scratch + b * 2 * (size_t)Wp * Hp;
+    raw *padT = pad + (size_t)Wp * Hp;
+    const int px0 = blockIdx.x * 32, py0 = blockIdx.y * 32;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int py = py0 + r, px = px0 + threadIdx.x;
+        const int x = px - kPad, y = py - kPad;
+        raw v = (raw)0.0f;
+        if (x >= 0 && x < nx && y >= 0 && y < ny) v = img[(size_t)y * nx + x];
+        if (px < Wp && py < Hp) pad[(size_t)py * Wp + px] = v;
+        tile[r][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int px = px0 + r, py = py0 + threadIdx.x;
+        if (px < Wp && py < Hp) padT[(size_t)px * Hp + py] = tile[threadIdx.x][r];
+    }
+}
+
+// ---------------------------------------------------------------- forward ------------------
+
+// March of one ray through a padded copy whose rows run along the ray's dominant axis.
+// (x, y) are padded-copy texel coordinates minus 0.5, so floor() gives the lower tap and
+// fract() its weight, as the linear texture filter does. `tbase` is the byte offset of the
+// copy inside the image's block and `rstride` the copy's row length in elements.
+template <class SI, bool YDOM>
+__device__ __forceinline__ float march2d(rsrc_t R, float4 e, int n, int rstride, int tbase) {
+    const int soff1 = tbase + rstride * (int)sizeof(typename SI::raw);
+    float acc = 0.0f, jf = 0.0f;
+    for (int j = 0; j < n; ++j, jf += 1.0f) {
+        const float x = fmaf(jf, e.z, e.x);
+        const float y = fmaf(jf, e.w, e.y);
+        const int ix = (int)floorf(x), iy = (int)floorf(y);
+        const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
+        const int o = YDOM ? __mul24(iy, rstride) + ix : __mul24(ix, rstride) + iy;
+        const Pair p0 = buf_pair<SI>(R, o, tbase); // YDOM: (ix, iy),(ix+1, iy)  else (ix, iy),(ix, iy+1)
+        const Pair p1 = buf_pair<SI>(R, o, soff1); // YDOM: row iy+1             else column ix+1
+        const float fa = YDOM ? fx : fy, fb = YDOM ? fy : fx;
+        const float u0 = fmaf(fa, p0.hi - p0.lo, p0.lo);
+        const float u1 = fmaf(fa, p1.hi - p1.lo, p1.lo);
+        acc += fmaf(fb, u1 - u0, u0);
+    }
+    return acc;
+}
+
+// One thread per ray (a, k); a wavefront holds 64 adjacent cells of one angle.
+// sino[b][a][k] = scale * sum_{j<N} bilinear(img, e + j v)   (forward_2d.hpp:63-78)
+template <class SI, class SS>
+__global__ __launch_bounds__(256) void k_fwd2d(const typename SI::raw *__restrict__ scratch,
+                                               int Wp, int Hp, const float4 *__restrict__ ev,
+                                               const float2 *__restrict__ ns,
+                                               typename SS::raw *__restrict__ sino, int D,
+                                               int A) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y * blockDim.y + threadIdx.y;
+    if (k >= D || a >= A) return;
+    const size_t b = blockIdx.z;
+    const size_t r = (size_t)a * D + k;
+    const float4 e = ev[r];
+    const float2 q = ns[r];
+    const int n = __float_as_int(q.y);
+    const uint32_t one = (uint32_t)Wp * Hp;
+    const rsrc_t R = make_rsrc(scratch + b * 2 * one, bytes_of<SI>(2 * (uint64_t)one));
+    float acc;
+    if (fabsf(e.w) == 1.0f) // y is the dominant axis: row-major copy
+        acc = march2d<SI, true>(R, e, n, Wp, 0);
+    else                    // x dominant: transposed copy, second half of the block
+        acc = march2d<SI, false>(R, e, n, Hp, (int)(one * sizeof(typename SI::raw)));
+    sino[b * (size_t)A * D + r] = from_f32<SS>(acc * q.x);
+}
+
+// ---------------------------------------------------------------- backprojection ---------
+
+// One thread per pixel, all angles in one pass (backward_2d.hpp:31-99):
+// vol[b][y][x] += (1/du) sum_a w_a lerp(sino[b][a], w_a eta_a + D/2),  w_a = DSD/(xi_a + DSO).
+template <class SV, class SS>
+__global__ __launch_bounds__(256) void k_bp2d(typename SV::raw *__restrict__ vol,
+                                              const typename SS::raw *__restrict__ sino,
+                                              const float4 *__restrict__ meta, int nx, int ny,
+                                              int D, int A, float sx, float sy, float ox,
+                                              float oy, float inv_du, float half_d) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= nx || iy >= ny) return;
+    const size_t b = blockIdx.z;
+    const uint32_t sino_elems = (uint32_t)A * D;
+    const rsrc_t R = make_rsrc(sino + b * sino_elems, bytes_of<SS>(sino_elems));
+    const float x = ((float)ix - (float)nx / 2.0f) * sx + ox + 0.5f;
+    const float y = ((float)iy - (float)ny / 2.0f) * sy + oy + 0.5f;
+    const float px = x * inv_du, py = y * inv_du;
+    const float dmax = (float)(D - 1);
+    float acc = 0.0f;
+    int row = 0;
+#pragma unroll 4
+    for (int a = 0; a < A; ++a, row += D) {
+        const float4 m = meta[a]; // wave-uniform: scalar loads
+        const float xi = m.x * x - m.y * y;
+        const float eta = m.y * px + m.x * py;
+        const float w = m.z * __builtin_amdgcn_rcpf(xi + m.w);
+        const float u = fmaf(w, eta, half_d) - 0.5f;
+        const float fl = floorf(u);
+        const float al = u - fl;
+        const float flc = fminf(fmaxf(fl, -1.0f), dmax);
+        const int o = row + (int)flc;
+        float t0 = buf_load<SS>(R, o);
+        float t1 = buf_load<SS>(R, o + 1);
+        t0 = (fl >= 0.0f && fl <= dmax) ? t0 : 0.0f;
+        t1 = (fl >= -1.0f && fl < dmax) ? t1 : 0.0f;
+        acc = fmaf(fmaf(al, t1 - t0, t0), w, acc);
+    }
+    const size_t i = b * (size_t)nx * ny + (size_t)iy * nx + ix;
+    vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc * inv_du);
+}
+
+// ---------------------------------------------------------------- adjoint ------------------
+
+// Exact transpose of k_fwd2d in gather form: texel (tx, ty) collects, for every angle, the
+// samples of the rays whose bilinear footprint contains it. Candidate rays come from projecting
+// the footprint box [tx-.5, tx+1.5) x [ty-.5, ty+1.5) onto the detector; candidate samples
+// from the ray's dominant axis; each candidate is then re-evaluated with the forward kernel's
+// own FP32 expressions, so the weights are the forward's.
+template <class SV, class SS>
+__global__ __launch_bounds__(256) void k_adj2d(typename SV::raw *__restrict__ vol,
+                                               const typename SS::raw *__restrict__ sino,
+                                               const float *__restrict__ ageo,
+                                               const float4 *__restrict__ ev,
+                                               const float2 *__restrict__ ns, int nx, int ny,
+                                               int D, int A) {
+    const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ty = blockIdx.y * blockDim.y + threadIdx.y;
+    if (tx >= nx || ty >= ny) return;
+    const size_t b = blockIdx.z;
+    const uint32_t sino_elems = (uint32_t)A * D;
+    const rsrc_t R = make_rsrc(sino + b * sino_elems, bytes_of<SS>(sino_elems));
+    // footprint box in the plan's texel-centre frame
+    const float qx[2] = {(float)tx - 0.5f, (float)tx + 1.5f};
+    const float qy[2] = {(float)ty - 0.5f, (float)ty + 1.5f};
+    // the texel's lower-tap index in the ray table's padded frame
+    const int px = tx + kPad, py = ty + kPad;
+    const float fpx = (float)px, fpy = (float)py;
+    float acc = 0.0f;
+    for (int a = 0; a < A; ++a) {
+        const float *gp = ageo + 6 * a; // wave-uniform
+        const float sxw = gp[0], syw = gp[1], ox = gp[2], oy = gp[3], dx = gp[4], dy = gp[5];
+        float kmin = INFINITY, kmax = -INFINITY;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float wx = qx[c & 1] - sxw, wy = qy[c >> 1] - syw;
+            const float num = wx * oy - wy * ox;
+            const float den = wx * dy - wy * dx;
+            const float kk = -num / den;
+            kmin = fminf(kmin, kk);
+            kmax = fmaxf(kmax, kk);
+        }
+        const int k0 = max((int)floorf(kmin - 0.01f), 0);
+        const int k1 = min((int)ceilf(kmax + 0.01f), D - 1);
+        for (int k = k0; k <= k1; ++k) {
+            const size_t r = (size_t)a * D + k;
+            const float2 q = ns[r];
+            const int n = __float_as_int(q.y);
+            if (n == 0) continue;
+            const float4 e = ev[r];
+            const bool ydom = fabsf(e.w) == 1.0f;
+            // samples whose dominant coordinate can floor to p-1 or p
+            const float ed = ydom ? e.y : e.x, vd = ydom ? e.w : e.z;
+            const float td = ydom ? fpy : fpx;
+            const float lo = (td - ed) * vd - 1.0f;
+            const int j0 = max((int)floorf(lo - 1e-3f), 0);
+            const int j1 = min((int)floorf(lo + 2.0f + 1e-3f), n - 1);
+            if (j0 > j1) continue;
+            const float yv = buf_load<SS>(R, (int)r) * q.x;
+            float part = 0.0f;
+            for (int j = j0; j <= j1; ++j) {
+                // the forward march's own expressions (march2d)
+                const float jf = (float)j;
+                const float x = fmaf(jf, e.z, e.x);
+                const float y = fmaf(jf, e.w, e.y);
+                const int ix = (int)floorf(x), iy = (int)floorf(y);
+                const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
+                const float wx = ix == px ? 1.0f - fx : (ix == px - 1 ? fx : 0.0f);
+                const float wy = iy == py ? 1.0f - fy : (iy == py - 1 ? fy : 0.0f);
+                part = fmaf(wx, wy, part);
+            }
+            acc = fmaf(part, yv, acc);
+        }
+    }
+    const size_t i = b * (size_t)nx * ny + (size_t)ty * nx + tx;
+    vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc);
+}
+
+// ---------------------------------------------------------------- dispatch -----------------
+
+template <class F>
+cmi_status dispatch2(int dt_a, int dt_b, F &&f) {
+    auto inner = [&](auto ta) -> cmi_status {
+        switch (dt_b) {
+        case CMI_F32: return f(ta, F32{});
+        case CMI_F16: return f(ta, F16{});
+        case CMI_BF16: return f(ta, BF16{});
+        default: return fail(CMI_ERR_DTYPE, "unsupported dtype %d", dt_b);
+        }
+    };
+    switch (dt_a) {
+    case CMI_F32: return inner(F32{});
+    case CMI_F16: return inner(F16{});
+    case CMI_BF16: return inner(BF16{});
+    default: return fail(CMI_ERR_DTYPE, "unsupported dtype %d", dt_a);
+    }
+}
+
+cmi_status check_call(const cmi_plan2d *p, const void *vol, int vol_dtype, const void *sino,
+                      int sino_dtype, uint64_t batch, const char *fn) {
+    if (!p) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: plan is null", fn);
+    if (!vol) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: volume pointer is null", fn);
+    if (!sino) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: sinogram pointer is null", fn);
+    if (batch == 0) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: batch must be >= 1", fn);
+    if (!dtype_ok(vol_dtype) || !dtype_ok(sino_dtype))
+        return fail(CMI_ERR_DTYPE, "%s: unsupported dtype (volume %d, sinogram %d)", fn,
+                    vol_dtype, sino_dtype);
+    if (vol_dtype != p->vol_prec)
+        return fail(CMI_ERR_DTYPE, "%s: plan precision does not match volume precision (%s)",
+                    fn, dtype_name(vol_dtype));
+    if (sino_dtype != p->det_prec)
+        return fail(CMI_ERR_DTYPE,
+                    "%s: plan precision does not match sinogram precision (%s)", fn,
+                    dtype_name(sino_dtype));
+    const uint64_t img = (uint64_t)p->g.nx * p->g.ny, sn = (uint64_t)p->A * p->g.D;
+    const uint64_t pad = 2 * (uint64_t)(p->g.nx + 2 * kPad) * (p->g.ny + 2 * kPad);
+    if (std::max({img, sn, pad}) * 4 >= (1ull << 31))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "%s: one image or sinogram exceeds 2 GiB", fn);
+    if (batch > 65535) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: batch exceeds 65535", fn);
+    CMI_TRY(check_device_ptr(vol, p->device, "volume"));
+    CMI_TRY(check_device_ptr(sino, p->device, "sinogram"));
+    return CMI_OK;
+}
+
+} // namespace
+
+// ===================================================================== C-ABI ===============
+
+extern "C" {
+
+CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vol_shape[2],
+                                     const double vol_spacing[2], const double vol_offset[2],
+                                     int det_prec, uint64_t det_count, double det_spacing,
+                                     double det_offset, const double *DSO, const double *DSD,
+                                     const double *angles, uint64_t nangles, double det_pitch,
+                                     double COR, cmi_plan2d **plan) {
+    clear_error();
+    if (!plan) return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create: output pointer is null");
+    *plan = nullptr;
+    if (!vol_shape || !vol_spacing || !vol_offset || !DSO || !DSD || !angles)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create: null geometry array");
+    if (!dtype_ok(vol_prec) || !dtype_ok(det_prec))
+        return fail(CMI_ERR_DTYPE, "plan2d_create: precision must be f32, f16 or bf16");
+    if (vol_shape[0] == 0 || vol_shape[1] == 0 || det_count == 0 || nangles == 0)
+        return fail(CMI_ERR_INVALID_ARGUMENT,
+                    "plan2d_create: empty shape (vol %llu x %llu, det %llu, angles %llu)",
+                    (unsigned long long)vol_shape[0], (unsigned long long)vol_shape[1],
+                    (unsigned long long)det_count, (unsigned long long)nangles);
+    if (vol_shape[0] > (1u << 20) || vol_shape[1] > (1u << 20) || det_count > (1u << 24) ||
+        nangles > (1u << 24) || det_count * nangles >= (1ull << 31))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create: shape too large");
+    if (!(vol_spacing[0] > 0) || !(vol_spacing[1] > 0) || !(det_spacing > 0))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create: spacings must be positive");
+    for (uint64_t a = 0; a < nangles; ++a)
+        if (!std::isfinite(angles[a]) || !std::isfinite(DSO[a]) || !std::isfinite(DSD[a]))
+            return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create: non-finite geometry at angle %llu",
+                        (unsigned long long)a);
+
+    auto p = std::make_unique<cmi_plan2d>();
+    p->device = device;
+    p->vol_prec = vol_prec;
+    p->det_prec = det_prec;
+    // Python order (y, x) -> (x, y), as plan.cu:48-50
+    p->g.nx = (int64_t)vol_shape[1];
+    p->g.ny = (int64_t)vol_shape[0];
+    p->g.sx = vol_spacing[1];
+    p->g.sy = vol_spacing[0];
+    p->g.ox = vol_offset[1];
+    p->g.oy = vol_offset[0];
+    p->g.D = (int64_t)det_count;
+    p->g.du = det_spacing;
+    p->g.det_offset = det_offset;
+    p->A = (int64_t)nangles;
+    p->pitch = det_pitch;
+    p->cor = COR;
+
+    const int64_t A = p->A, D = p->g.D;
+    std::vector<AngleGeom> geo(A);
+    std::vector<float4> meta(A);
+    std::vector<float> ageo(6 * A);
+    for (int64_t a = 0; a < A; ++a) {
+        geo[a] = fan2d::plan_angle(p->g, angles[a], DSO[a], DSD[a]);
+        meta[a] = make_float4((float)-std::sin(angles[a]), (float)std::cos(angles[a]),
+                              (float)DSD[a], (float)DSO[a]);
+        for (int d = 0; d < 2; ++d) {
+            ageo[6 * a + d] = (float)geo[a].src[d];
+            ageo[6 * a + 2 + d] = (float)geo[a].org[d];
+            ageo[6 * a + 4 + d] = (float)geo[a].del[d];
+        }
+    }
+
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(device));
+    struct Cleanup {
+        cmi_plan2d *p;
+        bool armed = true;
+        ~Cleanup() {
+            if (!armed) return;
+            (void)hipFree(p->d_geo);
+            (void)hipFree(p->d_meta);
+            (void)hipFree(p->d_ageo);
+            (void)hipFree(p->d_ev);
+            (void)hipFree(p->d_ns);
+        }
+    } cleanup{p.get()};
+    const size_t rays = (size_t)A * D;
+    CMI_HIP_TRY(hipMalloc(&p->d_geo, sizeof(AngleGeom) * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_meta, sizeof(float4) * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_ageo, sizeof(float) * 6 * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_ev, sizeof(float4) * rays));
+    CMI_HIP_TRY(hipMalloc(&p->d_ns, sizeof(float2) * rays));
+    p->table_bytes = sizeof(AngleGeom) * A + sizeof(float4) * A + sizeof(float) * 6 * A +
+                     (sizeof(float4) + sizeof(float2)) * rays;
+    unsigned long long *d_total = nullptr;
+    CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
+    std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
+        d_total, [](unsigned long long *q) { (void)hipFree(q); });
+    hipStream_t s;
+    CMI_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<ihipStream_t, void (*)(hipStream_t)> stream_guard(
+        s, [](hipStream_t q) { (void)hipStreamDestroy(q); });
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_geo, geo.data(), sizeof(AngleGeom) * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_meta, meta.data(), sizeof(float4) * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_ageo, ageo.data(), sizeof(float) * 6 * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(unsigned long long), s));
+    dim3 blk(256), grd((unsigned)((D + 255) / 256), (unsigned)A);
+    hipLaunchKernelGGL(k_ray_table, grd, blk, 0, s, p->d_geo, (int)A, (int)D, p->g.nx, p->g.ny,
+                       p->g.sx, p->g.sy, p->d_ev, p->d_ns, d_total);
+    CMI_HIP_TRY(hipGetLastError());
+    unsigned long long total = 0;
+    CMI_HIP_TRY(hipMemcpyAsync(&total, d_total, sizeof total, hipMemcpyDeviceToHost, s));
+    CMI_HIP_TRY(hipStreamSynchronize(s));
+    p->total_samples = total;
+    cleanup.armed = false;
+    *plan = p.release();
+    return CMI_OK;
+}
+
+CMI_API cmi_status cmi_plan2d_destroy(cmi_plan2d *plan) {
+    clear_error();
+    if (!plan) return CMI_OK;
+    DeviceGuard guard;
+    const cmi_status st = guard.enter(plan->device);
+    if (st == CMI_OK) {
+        // Work enqueued with this plan may still be running on any stream.
+        (void)hipDeviceSynchronize();
+        (void)hipFree(plan->d_geo);
+        (void)hipFree(plan->d_meta);
+        (void)hipFree(plan->d_ageo);
+        (void)hipFree(plan->d_ev);
+        (void)hipFree(plan->d_ns);
+    }
+    delete plan;
+    return st;
+}
+
+CMI_API cmi_status cmi_plan2d_set_block(cmi_plan2d *plan, uint32_t block_x, uint32_t block_y) {
+    clear_error();
+    if (!plan) return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_set_block: plan is null");
+    if (block_x == 0 || block_x % 64 != 0 || block_y == 0 || block_x * block_y > 1024)
+        return fail(CMI_ERR_INVALID_ARGUMENT,
+                    "plan2d_set_block: need block_x a multiple of 64 and block_x*block_y <= "
+                    "1024 (got %u x %u)",
+                    block_x, block_y);
+    plan->block_x = block_x;
+    plan->block_y = block_y;
+    return CMI_OK;
+}
+
+CMI_API cmi_status cmi_plan2d_get_block(const cmi_plan2d *plan, uint32_t *block_x,
+                                        uint32_t *block_y) {
+    clear_error();
+    if (!plan || !block_x || !block_y)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_get_block: null argument");
+    *block_x = plan->block_x;
+    *block_y = plan->block_y;
+    return CMI_OK;
+}
+
+CMI_API cmi_status cmi_plan2d_get_info(const cmi_plan2d *plan, cmi_plan2d_info *info) {
+    clear_error();
+    if (!plan || !info) return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_get_info: null argument");
+    info->device = plan->device;
+    info->vol_prec = plan->vol_prec;
+    info->det_prec = plan->det_prec;
+    info->nx = (uint64_t)plan->g.nx;
+    info->ny = (uint64_t)plan->g.ny;
+    info->det_count = (uint64_t)plan->g.D;
+    info->nangles = (uint64_t)plan->A;
+    info->total_samples = plan->total_samples;
+    info->table_bytes = plan->table_bytes;
+    return CMI_OK;
+}
+
+CMI_API cmi_status cmi_forward_2d(const cmi_plan2d *plan, const void *volume, int vol_dtype,
+                                  void *sinogram, int sino_dtype, uint64_t batch,
+                                  void *stream) {
+    clear_error();
+    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "forward_2d"));
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(plan->device));
+    const hipStream_t s = (hipStream_t)stream;
+    const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad;
+    const size_t esz = dtype_size(vol_dtype);
+    const size_t one = (size_t)Wp * Hp * esz; // bytes of one padded copy
+    void *scratch = nullptr;
+    CMI_HIP_TRY(hipMallocAsync(&scratch, 2 * one * batch, s));
+    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto si, auto ss) -> cmi_status {
+        using SI = decltype(si);
+        using SS = decltype(ss);
+        using RI = typename SI::raw;
+        dim3 sb(32, 8), sg((unsigned)((Wp + 31) / 32), (unsigned)((Hp + 31) / 32),
+                           (unsigned)batch);
+        hipLaunchKernelGGL((k_stage2d<SI>), sg, sb, 0, s, (const RI *)volume, (RI *)scratch, nx,
+                           ny, Wp, Hp);
+        dim3 fb(plan->block_x, plan->block_y),
+            fg((unsigned)((D + plan->block_x - 1) / plan->block_x),
+               (unsigned)((A + plan->block_y - 1) / plan->block_y), (unsigned)batch);
+        hipLaunchKernelGGL((k_fwd2d<SI, SS>), fg, fb, 0, s, (const RI *)scratch, Wp, Hp,
+                           plan->d_ev, plan->d_ns, (typename SS::raw *)sinogram, D, A);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+    const hipError_t fe = hipFreeAsync(scratch, s);
+    if (st != CMI_OK) return st;
+    CMI_HIP_TRY(fe);
+    return CMI_OK;
+}
+
+CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,
+                                   const void *sinogram, int sino_dtype, uint64_t batch,
+                                   void *stream) {
+    clear_error();
+    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "backward_2d"));
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(plan->device));
+    const hipStream_t s = (hipStream_t)stream;
+    const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
+    return dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
+        using SV = decltype(sv);
+        using SS = decltype(ss);
+        dim3 b(64, 4), g((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)batch);
+        hipLaunchKernelGGL((k_bp2d<SV, SS>), g, b, 0, s, (typename SV::raw *)volume,
+                           (const typename SS::raw *)sinogram, plan->d_meta, nx, ny, D, A,
+                           (float)plan->g.sx, (float)plan->g.sy, (float)plan->g.ox,
+                           (float)plan->g.oy, (float)(1.0 / plan->g.du), (float)D / 2.0f);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+}
+
+CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,
+                                  const void *sinogram, int sino_dtype, uint64_t batch,
+                                  void *stream) {
+    clear_error();
+    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "adjoint_2d"));
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(plan->device));
+    const hipStream_t s = (hipStream_t)stream;
+    const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
+    return dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
+        using SV = decltype(sv);
+        using SS = decltype(ss);
+        dim3 b(64, 4), g((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)batch);
+        hipLaunchKernelGGL((k_adj2d<SV, SS>), g, b, 0, s, (typename SV::raw *)volume,
+                           (const typename SS::raw *)sinogram, plan->d_ageo, plan->d_ev,
+                           plan->d_ns, nx, ny, D, A);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+}
+
+} // extern "C"

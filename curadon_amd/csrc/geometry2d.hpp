@@ -1,0 +1,103 @@
+// FP64 2D fan-beam geometry: per-angle plan and per-ray march set-up.
+//
+// The per-ray set-up decides discrete quantities (miss or hit, sample count N) that must be
+// identical for the forward projector and its adjoint, so it is evaluated once per plan in
+// FP64 and stored in the ray table; the kernels only march. Contraction into FMA is turned
+// off so the FP64 sequence is the plain IEEE one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#pragma clang fp contract(off)
+
+namespace cmi::fan2d {
+
+// Per-angle geometry in texel coordinates of the image (texel i centred at i + 0.5):
+// src = source position, org = vector from the source to detector cell 0's centre,
+// del = step from cell k to cell k + 1.
+struct AngleGeom {
+    double src[2];
+    double org[2];
+    double del[2];
+};
+
+struct Scalars {
+    int64_t nx, ny, D;
+    double sx, sy, ox, oy;
+    double du, det_offset;
+};
+
+// Clockwise rotation by the angle with cosine c and sine s, then translation by -t
+// (include/curadon/detail/rotation.hpp:51-61).
+__host__ __device__ inline void rot_cw(const double v[2], double c, double s, const double t[2],
+                                       double out[2]) {
+    out[0] = v[0] * c + v[1] * s - t[0];
+    out[1] = -v[0] * s + v[1] * c - t[1];
+}
+
+// plan_2d::precompute (include/curadon/detail/plan/plan_2d.hpp:17-80, 189-241) in FP64.
+inline AngleGeom plan_angle(const Scalars &g, double angle, double dso, double dsd) {
+    AngleGeom r{};
+    const double c = std::cos(angle), s = std::sin(angle);
+    const double det_extent = (double)g.D * g.du;
+    const double ext[2] = {(double)g.nx * g.sx, (double)g.ny * g.sy};
+    const double sp[2] = {g.sx, g.sy}, off[2] = {g.ox, g.oy}, zero[2] = {0.0, 0.0};
+    const double dod = dsd - dso;
+    const double s0[2] = {g.det_offset, dso};
+    double sw[2];
+    rot_cw(s0, c, s, zero, sw);
+    const double p0[2] = {-det_extent / 2.0 + g.du * 0.5, -dod};
+    const double p1[2] = {-det_extent / 2.0 + g.du * 1.5, -dod};
+    double r0[2], r1[2];
+    rot_cw(p0, c, s, sw, r0);
+    rot_cw(p1, c, s, sw, r1);
+    for (int d = 0; d < 2; ++d) {
+        const double o = r0[d] / sp[d];
+        r.org[d] = o;
+        r.del[d] = r1[d] / sp[d] - o;
+        r.src[d] = (sw[d] - off[d] + 0.5 * ext[d]) / sp[d];
+    }
+    return r;
+}
+
+// One ray after clipping, in the form the kernels consume.
+struct Ray {
+    double e[2];    // entry point, texel coordinates
+    double v[2];    // step; the dominant component is exactly +-1
+    double scale;   // world length of one step
+    int64_t n;      // number of samples; 0 for a missed ray
+};
+
+// Slab clip against [-1, n] per axis (include/curadon/detail/intersection.h:96-112), the miss
+// test tmin > tmax - 1e-6 (forward_2d.hpp:58) and the march set-up (forward_2d.hpp:63-77).
+__host__ __device__ inline Ray setup_ray(const AngleGeom &ag, int64_t nx, int64_t ny, double sx,
+                                         double sy, int64_t k) {
+    Ray r{};
+    const double rd[2] = {ag.org[0] + (double)k * ag.del[0], ag.org[1] + (double)k * ag.del[1]};
+    const double hi[2] = {(double)nx, (double)ny};
+    double tmin = FLT_MIN, tmax = FLT_MAX;
+    for (int d = 0; d < 2; ++d) {
+        const double t1 = (-1.0 - ag.src[d]) / rd[d];
+        const double t2 = (hi[d] - ag.src[d]) / rd[d];
+        tmin = fmin(fmax(t1, tmin), fmax(t2, tmin));
+        tmax = fmax(fmin(t1, tmax), fmin(t2, tmax));
+    }
+    if (tmin > tmax - 1e-6) return r; // n = 0
+    const double q[2] = {rd[0] * (tmax - tmin), rd[1] * (tmax - tmin)};
+    const double m = fmax(fabs(q[0]), fabs(q[1]));
+    r.e[0] = ag.src[0] + tmin * rd[0];
+    r.e[1] = ag.src[1] + tmin * rd[1];
+    r.v[0] = q[0] / m;
+    r.v[1] = q[1] / m;
+    r.scale = hypot(r.v[0] * sx, r.v[1] * sy);
+    r.n = (int64_t)rint(m); // round half to even (forward_2d.hpp:66)
+    return r;
+}
+
+} // namespace cmi::fan2d
+
+#pragma clang fp contract(on)

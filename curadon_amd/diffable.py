@@ -1,0 +1,55 @@
+"""Autograd functions (python/standalone/src/curadon/diffable.py:14-38).
+
+``TorchForward.apply(volume, geom)`` returns ``A volume``; its gradient is the matched
+adjoint ``A^T grad`` (exact transpose of the ray march), not the voxel-driven backprojector
+the reference pairs it with (diffable.py:22-24).
+
+``TorchBackprojection.apply(sinogram, geom)`` returns the voxel-driven backprojection
+``B sinogram``. Its gradient is ``A grad``, as the reference intends (diffable.py:35-38).
+``B`` is not the transpose of ``A`` (different weighting and interpolation), so this gradient
+is an approximation; use ``TorchAdjoint`` for an exact pair.
+
+Each function returns exactly one gradient per input.
+"""
+from __future__ import annotations
+
+import torch
+from torch.autograd import Function
+
+from .backward import adjoint, backward
+from .forward import forward
+
+
+class TorchForward(Function):
+    @staticmethod
+    def forward(ctx, volume, geom):
+        ctx.geom = geom
+        return forward(volume.contiguous(), geom)
+
+    @staticmethod
+    def backward(ctx, grad_sino):
+        return adjoint(grad_sino.contiguous(), ctx.geom), None
+
+
+class TorchAdjoint(Function):
+    """``A^T sinogram`` with gradient ``A grad`` (exact pair with TorchForward)."""
+
+    @staticmethod
+    def forward(ctx, sino, geom):
+        ctx.geom = geom
+        return adjoint(sino.contiguous(), geom)
+
+    @staticmethod
+    def backward(ctx, grad_vol):
+        return forward(grad_vol.contiguous(), ctx.geom), None
+
+
+class TorchBackprojection(Function):
+    @staticmethod
+    def forward(ctx, sino, geom):
+        ctx.geom = geom
+        return backward(sino.contiguous(), geom)
+
+    @staticmethod
+    def backward(ctx, grad_vol):
+        return forward(grad_vol.contiguous(), ctx.geom), None

@@ -1,0 +1,269 @@
+"""Geometry descriptors with the constructor signatures, attributes and validation errors of
+curadon's ``FanGeometry`` / ``ConeGeometry`` (python/standalone/src/curadon/geometry.py:53-336).
+
+Differences from the reference, all deliberate:
+  * a scalar ``DSO`` is broadcast from ``DSO`` (the reference broadcasts ``DSD`` into it,
+    geometry.py:165, which puts the detector through the isocentre);
+  * the native plan is built on first use (``geom.plan``) instead of in ``__init__``, so a
+    geometry can be described without a GPU;
+  * ``vol_prec`` / ``sino_prec`` also accept torch dtypes and 'bf16';
+  * ``ConeGeometry`` owns a cached native 3D plan (the reference rebuilds its geometry tables
+    on every call).
+"""
+from __future__ import annotations
+
+from typing import Any, List, Optional, Tuple, Union
+
+import numpy as np
+
+
+def is_convertible_to_float(val: Any):
+    try:
+        float(val)
+        return True
+    except ValueError:
+        return False
+    except TypeError:
+        return False
+
+
+def geom_to_str(geom):
+    p = ["-----", "Geometry parameters",
+         f"Distance from source to detector (DSD) = {geom.DSD}",
+         f"Distance from source to origin (DSO) = {geom.DSO}"]
+    if isinstance(geom, FanGeometry):
+        p.append(f"Projections angles = {np.rad2deg(geom.angles)}")
+    p += ["-----", "Detector parameters",
+          f"Number of pixels (det_shape) = {geom.det_shape}",
+          f"Size of each pixel (det_spacing) = {geom.det_spacing}",
+          f"Total size of the detector (det_size) = {np.asarray(geom.det_shape) * geom.det_spacing}",
+          f"Detector rotation (det_rotation) = {geom.det_rotation}",
+          f"Offset of detector (det_offset) = {geom.det_offset}",
+          "-----", "Image parameters",
+          f"Number of voxels (vol_shape) = {geom.vol_shape}",
+          f"Size of each voxel (vol_spacing) = {geom.vol_spacing}",
+          f"Total size of the image (vol_size) = {geom.vol_shape * geom.vol_spacing}",
+          f"Offset of image from origin (vol_offset) = {geom.vol_offset}"]
+    return "\n".join(p)
+
+
+class FanGeometry:
+    """2D fan-beam geometry with a flat detector (geometry.py:136-235)."""
+
+    def to_vec(self):
+        """ASTRA-like vectors ``[src.x, src.y, det.x, det.y, u.x, u.y]`` per angle
+        (geometry.py:54-79): source at (0, DSO), detector centre at (0, -(DSD - DSO)) and unit
+        detector axis (1, 0), rotated by ``[[c, s], [-s, c]]``."""
+        out = []
+        for alpha, dsd, dso in zip(self.angles, self.DSD, self.DSO):
+            c, s = np.cos(alpha), np.sin(alpha)
+            R = np.array([[c, s], [-s, c]])
+            src = R @ np.array([0.0, dso])
+            det = R @ np.array([0.0, -(dsd - dso)])
+            u = R @ np.array([1.0, 0.0])
+            out.append([*src.tolist(), *det.tolist(), *u.tolist()])
+        return out
+
+    @staticmethod
+    def from_vec(vol_shape: Tuple[int, int], det_shape: int, vecs: List[List[float]], **kwargs):
+        """Build a FanGeometry from ASTRA-like vectors (geometry.py:84-133): DSD = |src - det|,
+        DSO = |src|, angle = atan2(src2det) mod 2 pi, detector spacing = |u| (equal for all
+        views, else AttributeError)."""
+        vecs = np.asarray(vecs, dtype=np.float64)
+        det_spacing = float(np.linalg.norm(vecs[0][-2:]))
+        dsd, dso, angles = [], [], []
+        for v in vecs:
+            sx, sy, dx, dy, ux, uy = v
+            src = np.array([sx, sy])
+            s2d = src - np.array([dx, dy])
+            dsd.append(float(np.linalg.norm(s2d)))
+            dso.append(float(np.linalg.norm(src)))
+            angles.append(float(np.arctan2(s2d[0], s2d[1]) % (2 * np.pi)))
+            if not np.isclose(det_spacing, np.linalg.norm([ux, uy])):
+                raise AttributeError("Detector must have equal spacing for all projections")
+        return FanGeometry(DSD=dsd, DSO=dso, angles=angles, vol_shape=vol_shape,
+                           det_shape=det_shape, det_spacing=det_spacing, **kwargs)
+
+    def __init__(self, DSD: Union[float, List[float]], DSO: Union[float, List[float]],
+                 angles: List[float], vol_shape: Tuple[int, int], det_shape: int,
+                 det_spacing: Optional[float] = None, det_offset: Optional[float] = None,
+                 det_rotation: Optional[float] = None,
+                 vol_spacing: Optional[Tuple[float, float]] = None,
+                 vol_offset: Optional[Tuple[float, float]] = None, device: int = 0,
+                 vol_prec=32, sino_prec=32):
+        self.angles = np.asarray(angles, dtype=np.float64)
+        if self.angles.ndim != 1:
+            raise AttributeError(f"Angles must be 1D array, got {str(self.angles.shape)}")
+        self.nangles = self.angles.shape[0]
+
+        if is_convertible_to_float(DSD) is True:
+            self.DSD = np.full(self.nangles, float(DSD))
+        else:
+            self.DSD = np.asarray(DSD, dtype=np.float64)
+            if self.DSD.ndim != 1 or self.DSD.size != self.nangles:
+                raise AttributeError(
+                    f"DSD must be a single float, or a list of floats nangles entries "
+                    f"(is {self.DSD.size}, expected {self.nangles})")
+
+        if is_convertible_to_float(DSO) is True:
+            self.DSO = np.full(self.nangles, float(DSO))
+        else:
+            self.DSO = np.asarray(DSO, dtype=np.float64)
+            if self.DSO.ndim != 1 or self.DSO.size != self.nangles:
+                raise AttributeError(
+                    f"DSO must be a single float, or a list of floats nangles entries "
+                    f"(is {self.DSO.size}, expected {self.nangles})")
+
+        self.vol_shape = np.asarray(vol_shape)
+        if self.vol_shape.shape != (2,):
+            raise AttributeError(f"vol_shape must be two values, got {str(self.vol_shape.shape)}")
+
+        if is_convertible_to_float(det_shape) is False:
+            raise AttributeError(f"det_shape must to be float, got {type(det_shape)}")
+        self.det_shape = int(det_shape)
+
+        if det_spacing is not None:
+            if is_convertible_to_float(det_spacing) is False:
+                raise TypeError(f"Expected det_spacing to be float, got {type(det_spacing)}")
+            self.det_spacing = float(det_spacing)
+        else:
+            self.det_spacing = 1.0
+
+        if det_offset is not None:
+            if is_convertible_to_float(det_offset) is False:
+                raise TypeError(f"Expected det_offset to be float, got {type(det_offset)}")
+            self.det_offset = float(det_offset)
+        else:
+            self.det_offset = 0.0
+
+        if det_rotation is not None:
+            if is_convertible_to_float(det_rotation) is False:
+                raise TypeError(f"Expected det_rotation to be float, got {type(det_rotation)}")
+            self.det_rotation = float(det_rotation)
+        else:
+            self.det_rotation = 0.0
+
+        if vol_spacing is not None:
+            self.vol_spacing = np.asarray(vol_spacing, dtype=np.float64)
+            if self.vol_spacing.shape != (2,):
+                raise AttributeError(
+                    f"vol_spacing must be two values, got {str(self.vol_spacing.shape)}")
+        else:
+            self.vol_spacing = np.ones(self.vol_shape.shape)
+
+        if vol_offset is not None:
+            self.vol_offset = np.asarray(vol_offset, dtype=np.float64)
+            if self.vol_offset.shape != (2,):
+                raise AttributeError(
+                    f"vol_offset must be two values, got {str(self.vol_offset.shape)}")
+        else:
+            self.vol_offset = np.zeros(self.vol_shape.shape)
+
+        self.device = int(device)
+        self.vol_prec = vol_prec
+        self.sino_prec = sino_prec
+        self._plan = None
+
+    @property
+    def plan(self):
+        """The native plan (``curadon_amd._lib.plan_2d``), built on first access. COR is 0, as
+        the reference passes it (geometry.py:226-228)."""
+        if self._plan is None:
+            from . import _lib
+            self._plan = _lib.plan_2d(self.device, self.vol_prec, self.vol_shape,
+                                      self.vol_spacing, self.vol_offset, self.sino_prec,
+                                      self.det_shape, self.det_spacing, self.det_offset,
+                                      self.DSO, self.DSD, self.angles, self.det_rotation, 0.0)
+        return self._plan
+
+    def sinogram_shape(self):
+        return (self.nangles, self.det_shape)
+
+    def __str__(self):
+        return geom_to_str(self)
+
+
+class ConeGeometry:
+    """3D cone-beam geometry with a flat panel (geometry.py:237-336). Axis order as the
+    reference's Python side: vol_shape (z, y, x), det_shape (v, u); angles (A,) or (3, A) as
+    (phi, theta, psi); det_rotation (pitch, roll, yaw)."""
+
+    def __init__(self, DSD: float, DSO: float, angles, vol_shape: Tuple[int, int, int],
+                 det_shape: Tuple[int, int], det_spacing=None, det_offset=None,
+                 det_rotation=None, vol_spacing=None, vol_offset=None, COR=None, device: int = 0,
+                 vol_prec=32, sino_prec=32):
+        if is_convertible_to_float(DSD) is False:
+            raise TypeError(f"Expected DSD to be float, got {type(DSD)}")
+        self.DSD = float(DSD)
+        if is_convertible_to_float(DSO) is False:
+            raise TypeError(f"Expected DSO to be float, got {type(DSO)}")
+        self.DSO = float(DSO)
+
+        self.det_shape = np.asarray(det_shape)
+        if self.det_shape.shape != (2,):
+            raise AttributeError(
+                f"Detector shape must be two values, got {str(self.det_shape.shape)}")
+        self.vol_shape = np.asarray(vol_shape)
+        if self.vol_shape.shape != (3,):
+            raise AttributeError(
+                f"Volume shape must be three values, got {str(self.vol_shape.shape)}")
+
+        self.angles = np.asarray(angles, dtype=np.float64)
+        if self.angles.ndim == 1:
+            self.nangles = self.angles.shape[0]
+            z = np.zeros((self.nangles,), dtype=np.float64)
+            self.angles = np.vstack((self.angles, z, z))
+        elif self.angles.ndim == 2:
+            if self.angles.shape[0] != 3:
+                raise AttributeError(
+                    f"Angles must either be a 1D array, or a 2D array of shape (3, nangles), "
+                    f"got {str(self.angles.shape)}")
+            self.nangles = self.angles.shape[1]
+        else:
+            raise AttributeError(
+                f"Angles must either be a 1D array, or a 2D array of shape (3, nangles), "
+                f"got {str(self.angles.shape)}")
+
+        def vec(val, n, name, default):
+            if val is None:
+                return default
+            a = np.asarray(val, dtype=np.float64)
+            if a.shape != (n,):
+                raise AttributeError(f"{name} must be {'two' if n == 2 else 'three'} values, "
+                                     f"got {str(a.shape)}")
+            return a
+
+        self.det_spacing = vec(det_spacing, 2, "det_spacing", np.ones(2))
+        self.det_offset = vec(det_offset, 2, "det_offset", np.zeros(2))
+        self.det_rotation = vec(det_rotation, 3, "det_rotation", np.zeros(3))
+        if COR is not None:
+            if is_convertible_to_float(COR) is False:
+                raise TypeError(f"Expected COR to be float, got {type(COR)}")
+            self.COR = float(COR)
+        else:
+            self.COR = 0.0
+        self.vol_spacing = vec(vol_spacing, 3, "vol_spacing", np.ones(3))
+        self.vol_size = self.vol_shape * self.vol_spacing
+        self.vol_offset = vec(vol_offset, 3, "vol_offset", np.zeros(3))
+        self.device = int(device)
+        self.vol_prec = vol_prec
+        self.sino_prec = sino_prec
+        self._plan = None
+
+    @property
+    def plan(self):
+        """The native 3D plan, built on first access and reused by every call."""
+        if self._plan is None:
+            from . import _lib
+            self._plan = _lib.plan_3d(self.device, self.vol_prec, self.sino_prec,
+                                      self.vol_shape, self.vol_spacing, self.vol_offset,
+                                      self.angles[0], self.angles[1], self.angles[2],
+                                      self.det_shape, self.det_spacing, self.det_offset,
+                                      self.det_rotation, self.DSO, self.DSD, self.COR)
+        return self._plan
+
+    def sinogram_shape(self):
+        return (self.nangles, *[int(v) for v in self.det_shape])
+
+    def __str__(self):
+        return geom_to_str(self)

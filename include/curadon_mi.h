@@ -1,0 +1,201 @@
+/*
+ * curadon_mi.h -- C-ABI of the MI355X (gfx950) projection operators.
+ *
+ * This is the drop-in boundary for curadon's native extension `curadon_ext`
+ * (python/standalone/src/bind.cpp:58-66). Every entry point takes plain pointers, sizes and
+ * an explicit HIP stream (passed as void*), returns a status code, and never calls exit().
+ * Device arrays are caller-owned; plans own their device tables and are immutable after
+ * creation except for the launch-shape tunables, so one plan may be used from several host
+ * threads and streams at once.
+ *
+ * Layouts are the reference's (python/standalone/src/forward.cu:143-156, 113-138):
+ *   2D image     (ny, nx)            x fastest          -- batch of B images: (B, ny, nx)
+ *   2D sinogram  (A, D)              detector fastest   -- batch: (B, A, D)
+ *   3D volume    (nz, ny, nx)        x fastest          -- batch: (B, nz, ny, nx)
+ *   3D sinogram  (A, V, U)           u fastest          -- batch: (B, A, V, U)
+ * Geometry arguments use the Python axis order of the reference ((y, x), (z, y, x), (v, u)).
+ *
+ * Semantics kept from the reference:
+ *   forward  overwrites the sinogram (forward_2d.hpp:58-61,78);
+ *   backward accumulates into the volume, += (backward_2d.hpp:99, backward_3d.hpp:100-107).
+ * Differences, all stated where the entry point is declared: no host synchronisation (work
+ * is enqueued on the given stream), 3D missed rays write 0, and no per-call copies into
+ * texture arrays.
+ */
+#ifndef CURADON_MI_H
+#define CURADON_MI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define CMI_API __attribute__((visibility("default")))
+#else
+#define CMI_API
+#endif
+
+/* Status codes. The message of the last failure on the calling host thread is returned by
+ * cmi_last_error(). Replaces gpuErrchk -> exit() (include/curadon/detail/error.h:6-18) and
+ * the nanobind attribute_error / type_error raised by the binding
+ * (python/standalone/src/forward.cu:106-108, 161-201; plan.cu:20-28). */
+typedef enum cmi_status {
+    CMI_OK = 0,
+    CMI_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad shape, size or geometry value */
+    CMI_ERR_DTYPE = 2,            /* unsupported dtype, or dtype differs from the plan's */
+    CMI_ERR_DEVICE = 3,           /* array or plan on another device than expected */
+    CMI_ERR_HIP = 4,              /* HIP runtime failure; message holds the HIP error text */
+    CMI_ERR_OUT_OF_MEMORY = 5,
+    CMI_ERR_NOT_SUPPORTED = 6
+} cmi_status;
+
+/* Storage precisions. F32 and F16 keep the numeric values of curad::precision
+ * (include/curadon/detail/texture.hpp:18-21); BF16 is an addition. Arithmetic is FP32. */
+typedef enum cmi_dtype { CMI_F16 = 16, CMI_F32 = 32, CMI_BF16 = 1016 } cmi_dtype;
+
+/* Backprojector variants for 3D (see cmi_backward_3d). */
+typedef enum cmi_bp3d_mode { CMI_BP3D_REFERENCE = 0, CMI_BP3D_MATCHED = 1 } cmi_bp3d_mode;
+
+/* Message of the last failed call on this host thread ("" if none). */
+CMI_API const char *cmi_last_error(void);
+
+/* Library version, major * 10000 + minor * 100 + patch. */
+CMI_API int cmi_version(void);
+
+/* ====================================================================== 2D fan beam == */
+
+typedef struct cmi_plan2d cmi_plan2d;
+
+/*
+ * Replaces the plan_2d constructor bound at python/standalone/src/plan.cu:30-59
+ * (curad::plan_2d, include/curadon/detail/plan/plan_2d.hpp:103-126). Same argument order:
+ * device, volume precision, vol_shape (y, x), vol_spacing (y, x), vol_offset (y, x),
+ * detector precision, det_count, det_spacing, det_offset, per-angle DSO / DSD / angles,
+ * det_pitch, COR. The reference takes FP32 geometry arrays; these take FP64 (FP32 values
+ * widen exactly) because the per-angle and per-ray geometry is computed in FP64.
+ * det_pitch and COR are stored and unused, as in the reference (plan_2d.hpp:121-122).
+ * Builds the per-angle geometry on the host and the per-ray table on `device`.
+ */
+CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vol_shape[2],
+                                     const double vol_spacing[2], const double vol_offset[2],
+                                     int det_prec, uint64_t det_count, double det_spacing,
+                                     double det_offset, const double *DSO, const double *DSD,
+                                     const double *angles, uint64_t nangles, double det_pitch,
+                                     double COR, cmi_plan2d **plan);
+
+CMI_API cmi_status cmi_plan2d_destroy(cmi_plan2d *plan);
+
+/* Launch-shape tunables, the analogue of the r/w forward_block_x / forward_block_y
+ * properties (plan.cu:57-58, plan_2d.hpp:184-186). block_x = detector cells per
+ * workgroup (multiple of 64, <= 1024), block_y = angles per workgroup, with
+ * block_x * block_y <= 1024. */
+CMI_API cmi_status cmi_plan2d_set_block(cmi_plan2d *plan, uint32_t block_x, uint32_t block_y);
+CMI_API cmi_status cmi_plan2d_get_block(const cmi_plan2d *plan, uint32_t *block_x,
+                                        uint32_t *block_y);
+
+/* Shape queries. total_samples = sum over rays of the forward sample count N. */
+typedef struct cmi_plan2d_info {
+    int device;
+    int vol_prec, det_prec;
+    uint64_t nx, ny, det_count, nangles;
+    uint64_t total_samples;
+    uint64_t table_bytes;
+} cmi_plan2d_info;
+CMI_API cmi_status cmi_plan2d_get_info(const cmi_plan2d *plan, cmi_plan2d_info *info);
+
+/*
+ * Ray-driven forward projection, sinogram := A volume, for `batch` images.
+ * Replaces curadon_ext.forward_2d (python/standalone/src/forward.cu:158-202 ->
+ * curad::fp::forward_2d, include/curadon/detail/forward_2d.hpp:82-139).
+ * vol_dtype must equal the plan's volume precision (forward.cu:173-189); sino_dtype must
+ * equal the plan's detector precision (not checked by the reference). Enqueued on `stream`;
+ * does not block. Scratch: the call stages each image into zero-padded row-major and
+ * transposed copies from a stream-ordered allocation that is released on the same stream.
+ */
+CMI_API cmi_status cmi_forward_2d(const cmi_plan2d *plan, const void *volume, int vol_dtype,
+                                  void *sinogram, int sino_dtype, uint64_t batch, void *stream);
+
+/*
+ * Voxel-driven backprojection with the reference's weighting, volume += B sinogram.
+ * Replaces curadon_ext.backward_2d (python/standalone/src/backward.cu:299-343 ->
+ * curad::bp::backproject_2d, include/curadon/detail/backward_2d.hpp:103-166). All angles
+ * are processed in one launch (no 1024-angle chunking). Enqueued on `stream`.
+ */
+CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,
+                                   const void *sinogram, int sino_dtype, uint64_t batch,
+                                   void *stream);
+
+/*
+ * Matched adjoint of cmi_forward_2d, volume += A^T sinogram: the exact transpose of the
+ * ray-march above (same sample positions and bilinear weights), in deterministic gather form.
+ * No counterpart in the reference, whose autograd pairs the forward with the voxel-driven
+ * backprojector (python/standalone/src/curadon/diffable.py:22-24).
+ */
+CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,
+                                  const void *sinogram, int sino_dtype, uint64_t batch,
+                                  void *stream);
+
+/* ===================================================================== 3D cone beam == */
+
+/*
+ * Cone-beam description, in the argument order and axis order of curadon_ext.forward_3d /
+ * backward_3d (python/standalone/src/forward.cu:89-104, backward.cu:230-245): vol_shape /
+ * vol_spacing / vol_offset in (z, y, x), det_shape / det_spacing / det_offset in (v, u),
+ * det_rotation as (pitch, roll, yaw), per-angle phi / theta / psi, scalar DSO, DSD, COR.
+ */
+typedef struct cmi_cone3d_desc {
+    uint64_t vol_shape[3];
+    double vol_spacing[3];
+    double vol_offset[3];
+    uint64_t det_shape[2];
+    double det_spacing[2];
+    double det_offset[2];
+    double det_rotation[3];
+    double DSO, DSD, COR;
+    uint64_t nangles;
+    const double *phi;
+    const double *theta;
+    const double *psi;
+} cmi_cone3d_desc;
+
+typedef struct cmi_plan3d cmi_plan3d;
+
+/* The reference has no 3D plan: forward_3d / backproject_3d rebuild their geometry, allocate
+ * a cudaArray and upload __constant__ tables on every call (forward_3d.hpp:214-274,
+ * backward_3d.hpp:181-253). This plan holds the per-angle tables on `device` instead. */
+CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
+                                     const cmi_cone3d_desc *desc, cmi_plan3d **plan);
+CMI_API cmi_status cmi_plan3d_destroy(cmi_plan3d *plan);
+
+/*
+ * Cone-beam forward projection, sinogram := A volume (unscaled sum of trilinear samples, as
+ * forward_3d.hpp:87-93). Replaces curadon_ext.forward_3d (forward.cu:89-141 ->
+ * curad::fp::forward_3d, forward_3d.hpp:214-274). Rays that miss the volume write 0 (the
+ * reference leaves them unwritten, forward_3d.hpp:74-76).
+ */
+CMI_API cmi_status cmi_forward_3d(const cmi_plan3d *plan, const void *volume, int vol_dtype,
+                                  void *sinogram, int sino_dtype, uint64_t batch, void *stream);
+
+/*
+ * Voxel-driven backprojection, volume += B sinogram. Replaces curadon_ext.backward_3d
+ * (backward.cu:230-282 -> curad::bp::backproject_3d, backward_3d.hpp:181-253).
+ * mode CMI_BP3D_REFERENCE reproduces the reference's projection (detector plane at
+ * z = DSO - DSD, integer detector centre, detector spacing ignored; backward_3d.hpp:79-94).
+ * mode CMI_BP3D_MATCHED projects each voxel through the forward model's geometry instead.
+ */
+CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol_dtype,
+                                   const void *sinogram, int sino_dtype, uint64_t batch,
+                                   int mode, void *stream);
+
+/* Matched adjoint of cmi_forward_3d, volume += A^T sinogram (exact transpose, deterministic
+ * gather form). No counterpart in the reference. */
+CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_dtype,
+                                  const void *sinogram, int sino_dtype, uint64_t batch,
+                                  void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,44 @@
+"""Geometry recipes shared by the tests (the benchmark workloads, scaled versions and tiny
+cases). Plain records: the oracle reads the same attributes as curadon_amd.FanGeometry."""
+from types import SimpleNamespace
+
+import numpy as np
+
+
+def fan(n, D, A, dso, dsd, du=1.0, det_offset=0.0, spacing=(1.0, 1.0), offset=(0.0, 0.0),
+        arc=2 * np.pi, angles=None, ny=None):
+    ny = n if ny is None else ny
+    ang = np.linspace(0, arc, A, endpoint=False) if angles is None else np.asarray(angles, float)
+    return SimpleNamespace(
+        vol_shape=np.array([ny, n]), vol_spacing=np.array(spacing, float),
+        vol_offset=np.array(offset, float), det_shape=int(D), det_spacing=float(du),
+        det_offset=float(det_offset), angles=ang, DSO=np.full(ang.size, float(dso)),
+        DSD=np.full(ang.size, float(dsd)), nangles=ang.size)
+
+
+def c1():
+    """64^2, 91 cells, 360 angles, DSO 128, DSD 192."""
+    return fan(64, 91, 360, 128.0, 192.0)
+
+
+def c2():
+    """1024^2, 1449 cells, 720 angles, DSO 2048, DSD 3072."""
+    return fan(1024, 1449, 720, 2048.0, 3072.0)
+
+
+def c4():
+    """512^2, 724 cells, 512 angles; the workload leaves DSO/DSD open: DSO = 2n, DSD = 3n."""
+    return fan(512, 724, 512, 1024.0, 1536.0)
+
+
+def tiny():
+    """8x8 image, 7 cells, 12 angles: the golden-fixture case."""
+    return fan(8, 7, 12, 16.0, 24.0)
+
+
+def as_fan_geometry(rec, **kw):
+    import curadon_amd as cm
+    return cm.FanGeometry(DSD=rec.DSD, DSO=rec.DSO, angles=rec.angles,
+                          vol_shape=rec.vol_shape, det_shape=rec.det_shape,
+                          det_spacing=rec.det_spacing, det_offset=rec.det_offset,
+                          vol_spacing=rec.vol_spacing, vol_offset=rec.vol_offset, **kw)

@@ -1,0 +1,37 @@
+"""Writes the golden fixtures under tests/golden/ from the FP64 oracle.
+
+The reference ships no numeric fixtures for its projectors (its tests compare perceptual
+hashes against pyelsa / phantominator output, python/tests/test_forward_2d.py:91-138), so these
+fixtures are outputs of the oracle, kept to catch regressions of the oracle itself; the
+closed-form and library checks in tests/test_oracle_*.py are what tie the oracle to the
+reference's arithmetic.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from geoms import tiny  # noqa: E402
+from oracle import oracle  # noqa: E402
+
+
+def main():
+    oracle.build()
+    g = tiny()
+    r = np.random.default_rng(20240801)
+    image = r.random((8, 8))
+    sino_in = r.random((g.nangles, g.det_shape))
+    np.savez(os.path.join(HERE, "fan2d_tiny.npz"), angles=g.angles, image=image,
+             sino=oracle.fwd2d(g, image), sino_in=sino_in, bp=oracle.bp2d(g, sino_in),
+             adj=oracle.adj2d(g, sino_in))
+    print("wrote fan2d_tiny.npz")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,232 @@
+"""GPU parity of the 2D fan-beam HIP kernels against the FP64 oracle, through the C-ABI.
+
+Tolerances (FP32 storage and arithmetic vs FP64):
+  forward   rel-L2 <= 1e-5, max-abs <= 2e-5 * max|ref|
+  backward  rel-L2 <= 1e-5, max-abs <= 2e-5 * max|ref|
+  adjoint   rel-L2 <= 1e-5, max-abs <= 2e-5 * max|ref|
+  dot test  |<Ax,y> - <x,A^T y>| / |<Ax,y>| <= 1e-5 (both sides on the GPU in FP32)
+At the 1024^2 workload the oracle runs on row / angle subsets (the operators are per-ray and
+per-pixel, so a subset is an exact restriction).
+"""
+import numpy as np
+import pytest
+import torch
+
+import curadon_amd as cm
+from curadon_amd import phantom
+from geoms import as_fan_geometry, c1, c2, c4, fan, tiny
+
+pytestmark = pytest.mark.gpu
+
+RTOL_L2 = 1e-5
+MAXABS = 2e-5
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def check_close(got, ref, what, rtol=RTOL_L2, maxabs=MAXABS):
+    r = rel_l2(got, ref)
+    m = float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
+    print(f"{what}: rel-L2 {r:.3e}  max-abs/max {m:.3e}", flush=True)
+    assert r <= rtol, (what, r)
+    assert m <= maxabs, (what, m)
+
+
+def dev(x, gpu, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(x)).to(device=gpu, dtype=dtype)
+
+
+def host(t):
+    return t.double().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def g1(gpu):
+    rec = c1()
+    return rec, as_fan_geometry(rec)
+
+
+def test_c1_forward(oracle, g1, gpu):
+    rec, geom = g1
+    img = phantom.ellipses_2d(64, seed=0)
+    ref = oracle.fwd2d(rec, img)
+    out = cm.forward(dev(img, gpu), geom)
+    torch.cuda.synchronize()
+    check_close(host(out), ref, "c1 forward")
+
+
+def test_c1_backward(oracle, g1, gpu):
+    rec, geom = g1
+    sino = np.random.default_rng(1).random((360, 91))
+    ref = oracle.bp2d(rec, sino)
+    vol = torch.zeros(64, 64, device=gpu)
+    cm.backward(dev(sino, gpu), vol, geom)
+    check_close(host(vol), ref, "c1 backward")
+
+
+def test_c1_backward_accumulates(g1, gpu):
+    rec, geom = g1
+    s = dev(np.random.default_rng(2).random((360, 91)), gpu)
+    base = torch.rand(64, 64, device=gpu)
+    once = cm.backward(s, geom)
+    acc = cm.backward(s, base.clone(), geom)
+    torch.testing.assert_close(acc, base + once, rtol=1e-6, atol=1e-5)
+
+
+def test_c1_adjoint(oracle, g1, gpu):
+    rec, geom = g1
+    sino = np.random.default_rng(3).random((360, 91))
+    ref = oracle.adj2d(rec, sino)
+    out = cm.adjoint(dev(sino, gpu), geom)
+    check_close(host(out), ref, "c1 adjoint")
+
+
+def test_c1_dot_product(g1, gpu):
+    rec, geom = g1
+    g = torch.Generator(device="cpu").manual_seed(4)
+    x = torch.rand(64, 64, generator=g).to(gpu)
+    y = torch.rand(360, 91, generator=g).to(gpu)
+    lhs = torch.vdot(cm.forward(x, geom).double().ravel(), y.double().ravel())
+    rhs = torch.vdot(x.double().ravel(), cm.adjoint(y, geom).double().ravel())
+    r = abs(float(lhs - rhs)) / abs(float(lhs))
+    print(f"c1 dot test: {r:.3e}")
+    assert r <= 1e-5
+
+
+def test_tiny_golden(g1, gpu):
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "fan2d_tiny.npz"))
+    geom = as_fan_geometry(tiny())
+    check_close(host(cm.forward(dev(d["image"], gpu), geom)), d["sino"], "tiny forward")
+    check_close(host(cm.backward(dev(d["sino_in"], gpu), geom)), d["bp"], "tiny backward")
+    check_close(host(cm.adjoint(dev(d["sino_in"], gpu), geom)), d["adj"], "tiny adjoint")
+
+
+@pytest.fixture(scope="module")
+def g2(gpu):
+    rec = c2()
+    geom = as_fan_geometry(rec)
+    img = phantom.ellipses_2d(1024, seed=0)
+    return rec, geom, img
+
+
+ANGLE_BLOCKS = [(0, 6), (87, 93), (178, 184), (357, 363), (534, 540), (714, 720)]
+ROW_BLOCKS = [(0, 4), (300, 304), (510, 514), (1020, 1024)]
+
+
+def test_c2_forward_subsets(oracle, g2, gpu):
+    rec, geom, img = g2
+    out = host(cm.forward(dev(img, gpu), geom))
+    got, ref = [], []
+    for a0, a1 in ANGLE_BLOCKS:
+        r = oracle.fwd2d(rec, img, angles=(a0, a1))
+        got.append(out[a0:a1])
+        ref.append(r[a0:a1])
+    check_close(np.concatenate(got), np.concatenate(ref), "c2 forward (36 angles)")
+
+
+def test_c2_backward_subsets(oracle, g2, gpu):
+    rec, geom, img = g2
+    sino = host(cm.forward(dev(img, gpu), geom))
+    vol = host(cm.backward(dev(sino, gpu), geom))
+    got, ref = [], []
+    for y0, y1 in ROW_BLOCKS:
+        r = oracle.bp2d(rec, sino, rows=(y0, y1))
+        got.append(vol[y0:y1])
+        ref.append(r[y0:y1])
+    check_close(np.concatenate(got), np.concatenate(ref), "c2 backward (16 rows)")
+
+
+def test_c2_adjoint_angle_subset(oracle, g2, gpu):
+    rec, geom, _ = g2
+    y = np.zeros((720, 1449))
+    a0, a1 = 87, 93
+    y[a0:a1] = np.random.default_rng(5).random((a1 - a0, 1449))
+    ref = oracle.adj2d(rec, y, angles=(a0, a1))
+    got = host(cm.adjoint(dev(y, gpu), geom))
+    check_close(got, ref, "c2 adjoint (6 angles)")
+
+
+def test_c2_dot_product(g2, gpu):
+    rec, geom, img = g2
+    g = torch.Generator(device="cpu").manual_seed(6)
+    x = torch.rand(1024, 1024, generator=g).to(gpu)
+    y = torch.rand(720, 1449, generator=g).to(gpu)
+    lhs = torch.vdot(cm.forward(x, geom).double().ravel(), y.double().ravel())
+    rhs = torch.vdot(x.double().ravel(), cm.adjoint(y, geom).double().ravel())
+    r = abs(float(lhs - rhs)) / abs(float(lhs))
+    print(f"c2 dot test: {r:.3e}")
+    assert r <= 1e-5
+
+
+def test_total_samples_match_oracle(oracle, g2):
+    rec, geom, img = g2
+    _, n = oracle.fwd2d(rec, np.zeros((1024, 1024)), angles=(0, 720), count=True)
+    assert geom.plan.info()["total_samples"] == n
+
+
+def test_forward_deterministic_and_block_independent(g1, gpu):
+    rec, geom = g1
+    x = torch.rand(64, 64, device=gpu)
+    a = cm.forward(x, geom)
+    b = cm.forward(x, geom)
+    plan = geom.plan
+    old = (plan.forward_block_x, plan.forward_block_y)
+    try:
+        plan.forward_block_x, plan.forward_block_y = 128, 2
+        c = cm.forward(x, geom)
+    finally:
+        plan.forward_block_x, plan.forward_block_y = old
+    assert torch.equal(a, b) and torch.equal(a, c)
+    y = torch.rand(360, 91, device=gpu)
+    assert torch.equal(cm.adjoint(y, geom), cm.adjoint(y, geom))
+
+
+def test_batched_equals_single(gpu):
+    rec = fan(96, 136, 64, 192.0, 288.0)
+    geom = as_fan_geometry(rec)
+    xs = torch.rand(3, 96, 96, device=gpu)
+    sb = cm.forward(xs, geom)
+    for i in range(3):
+        assert torch.equal(sb[i], cm.forward(xs[i].contiguous(), geom))
+    ys = torch.rand(3, 64, 136, device=gpu)
+    vb = cm.backward(ys, geom)
+    ab = cm.adjoint(ys, geom)
+    for i in range(3):
+        assert torch.equal(vb[i], cm.backward(ys[i].contiguous(), geom))
+        assert torch.equal(ab[i], cm.adjoint(ys[i].contiguous(), geom))
+
+
+@pytest.mark.parametrize("prec,tol", [(torch.float16, 2e-3), (torch.bfloat16, 1.5e-2)])
+def test_half_storage_c1(oracle, gpu, prec, tol):
+    rec = c1()
+    geom = as_fan_geometry(rec, vol_prec=prec, sino_prec=prec)
+    img = phantom.ellipses_2d(64, seed=0)
+    ref = oracle.fwd2d(rec, img)
+    out = host(cm.forward(dev(img, gpu, prec), geom))
+    r = rel_l2(out, ref)
+    print(f"c1 forward {prec}: rel-L2 {r:.3e}")
+    assert r <= tol
+    sino = np.random.default_rng(7).random((360, 91))
+    refb = oracle.bp2d(rec, sino)
+    outb = host(cm.backward(dev(sino, gpu, prec), geom))
+    rb = rel_l2(outb, refb)
+    print(f"c1 backward {prec}: rel-L2 {rb:.3e}")
+    assert rb <= tol
+
+
+def test_errors_like_reference(g1, gpu):
+    rec, geom = g1
+    x = torch.rand(64, 64, device=gpu)
+    with pytest.raises(AttributeError, match="precision"):
+        cm.forward(x.half(), geom)
+    with pytest.raises(TypeError):
+        cm.forward(x.double(), geom)
+    with pytest.raises(ValueError):
+        cm.forward(torch.rand(63, 64, device=gpu), geom)
+    with pytest.raises(TypeError):
+        cm.forward(x.t(), geom)  # not contiguous
+    with pytest.raises(ValueError, match="Unknown geometry"):
+        cm.forward(x, torch.zeros(1), None)
