@@ -142,6 +142,20 @@ def main():
     dom_gbs = fwd_gbs if dominant == "fwd2d" else bwd_gbs
     dom_bytes = fwd_bytes if dominant == "fwd2d" else bwd_bytes
 
+    # matched adjoint (autograd's backward of the forward), timed separately
+    adj_vol = torch.zeros_like(vol)
+    cm.adjoint(sino, adj_vol, geom)
+    torch.cuda.synchronize()
+    ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n_adj = max(1, min(5, args.steps))
+    ea.record(stream)
+    for _ in range(n_adj):
+        cm.adjoint(sino, adj_vol, geom)
+    eb.record(stream)
+    torch.cuda.synchronize()
+    adj_ms = ea.elapsed_time(eb) / n_adj
+    log(f"bench: adjoint {adj_ms:.3f} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle FP32 port)")
@@ -181,6 +195,8 @@ def main():
             "bp2d": {"ms_mean": float(bwd_ms.mean()), "ms_p50": float(np.percentile(bwd_ms, 50)),
                      "ms_p99": float(np.percentile(bwd_ms, 99)), "op_per_s": 1e3 / bwd_ms.mean(),
                      "alg_GB_per_s": bwd_gbs, "alg_bytes": bwd_bytes},
+            "adj2d": {"ms_mean": adj_ms, "op_per_s": 1e3 / adj_ms,
+                      "alg_GB_per_s": fwd_bytes / (adj_ms * 1e-3) / 1e9},
             "pair": {"ms_p50": float(np.percentile(step_ms, 50)),
                      "ms_p99": float(np.percentile(step_ms, 99))},
             "total_samples": info["total_samples"],

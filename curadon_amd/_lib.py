@@ -242,6 +242,10 @@ class plan_2d:
         check(_lib.cmi_plan2d_get_block(self._h, C.byref(bx), C.byref(by)))
         return bx.value, by.value
 
+    def set_block(self, block_x, block_y):
+        """Set both launch-shape tunables at once (block_x * block_y <= 256)."""
+        check(_lib.cmi_plan2d_set_block(self._h, int(block_x), int(block_y)))
+
     @property
     def forward_block_x(self):
         return self._block()[0]

@@ -87,7 +87,10 @@ __device__ __forceinline__ typename S::raw from_f32(float v) {
 
 // ------------------------------------------------------------------ buffer loads -------
 // Raw buffer resources give 32-bit offsets from a scalar base and hardware range checking:
-// a load whose offset lies beyond num_records bytes returns 0 instead of touching memory.
+// a load that lies wholly beyond num_records bytes returns 0 instead of touching memory.
+// The kernels use that only as a memory-safety net and keep every load they rely on fully
+// inside the resource: a multi-dword load that straddles the limit (the compiler merges
+// adjacent dword loads into one) comes back as zeros in all of its dwords.
 
 using rsrc_t = __amdgpu_buffer_rsrc_t;
 

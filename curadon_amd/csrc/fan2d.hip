@@ -5,13 +5,19 @@
 //   plan tables  geo  [A] AngleGeom (FP64)       source / cell-0 vector / cell step
 //                meta [A] float4                 (-sin, cos, DSD, DSO) for the backprojector
 //                ageo [A] float[6]               FP32 copy of geo for the adjoint's ray search
-//                ev   [A*D] float4               (ex + 1.5, ey + 1.5, vx, vy) per ray: the
-//                                                entry point shifted into padded-copy
-//                                                coordinates (texel centre at +0.5, border 2)
-//                ns   [A*D] float2               (world step length, N as int bits) per ray
+//                ev   [A*D] float4               (mx, my, vx, vy) per ray: m is the sample
+//                                                j = N/2 in centred padded coordinates (see
+//                                                below), v the step
+//                ns   [A*D] float4               (world step length, N as int bits, -N/2, 0)
+// Sample j of a ray sits at m + (j - N/2) v. Coordinates are "centred padded": padded-copy
+// texel coordinate minus 0.5 (so floor() is the lower tap) minus the integer centre c0 of the
+// padded copy. Anchoring at the ray's midpoint near the image centre keeps |m| and |x| small,
+// so the FP32 rounding of the anchor and of each position stays far below what the same
+// values would carry at texel coordinates up to n, and it is not shared by all samples.
 //   per call     for each image b of the batch, one block of 2 * (nx+4) * (ny+4) elements:
 //                pad  [ny+4][nx+4]               image with a 2-texel zero border
 //                padT [nx+4][ny+4]               the same, transposed
+//                (backprojection) sinogram with a 2-cell zero border: [B][A][D+4]
 // The forward kernel reads the copy whose rows run along each ray's dominant axis, so the 64
 // rays of a wavefront (adjacent detector cells of one angle) gather from a few cache lines per
 // step whichever way the fan points, and the zero border replaces per-tap range checks.
@@ -42,19 +48,20 @@ struct cmi_plan2d {
     float4 *d_meta = nullptr;
     float *d_ageo = nullptr;
     float4 *d_ev = nullptr;
-    float2 *d_ns = nullptr;
+    float4 *d_ns = nullptr;
 };
 
 namespace {
 
-constexpr int kPad = 2; // zero border of the staged image copies, in texels
+constexpr int kPad = 2;        // zero border of the staged image copies, in texels
+constexpr int kMaxBlock = 256; // threads per workgroup every kernel here is compiled for
 
 // ---------------------------------------------------------------- ray table ----------------
 
-__global__ __launch_bounds__(256) void k_ray_table(const AngleGeom *__restrict__ geo, int A,
+__global__ __launch_bounds__(kMaxBlock) void k_ray_table(const AngleGeom *__restrict__ geo, int A,
                                                    int D, int64_t nx, int64_t ny, double sx,
                                                    double sy, float4 *__restrict__ ev,
-                                                   float2 *__restrict__ ns,
+                                                   float4 *__restrict__ ns,
                                                    unsigned long long *__restrict__ total) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int a = blockIdx.y;
@@ -62,10 +69,15 @@ __global__ __launch_bounds__(256) void k_ray_table(const AngleGeom *__restrict__
     if (k < D && a < A) {
         const fan2d::Ray r = fan2d::setup_ray(geo[a], nx, ny, sx, sy, k);
         const size_t i = (size_t)a * D + k;
-        const double shift = kPad - 0.5; // texel-centre offset and zero border
-        ev[i] = make_float4((float)(r.e[0] + shift), (float)(r.e[1] + shift), (float)r.v[0],
-                            (float)r.v[1]);
-        ns[i] = make_float2((float)r.scale, __int_as_float((int)r.n));
+        const int64_t jm = r.n / 2;
+        // texel coordinate -> centred padded coordinate
+        const double cx = (double)(kPad - 0.5) - (double)((nx + 2 * kPad) / 2);
+        const double cy = (double)(kPad - 0.5) - (double)((ny + 2 * kPad) / 2);
+        const double mx = r.e[0] + (double)jm * r.v[0] + cx;
+        const double my = r.e[1] + (double)jm * r.v[1] + cy;
+        ev[i] = r.n ? make_float4((float)mx, (float)my, (float)r.v[0], (float)r.v[1])
+                    : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ns[i] = make_float4((float)r.scale, __int_as_float((int)r.n), (float)(-jm), 0.0f);
         n = (unsigned long long)r.n;
     }
     for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
@@ -76,7 +88,7 @@ __global__ __launch_bounds__(256) void k_ray_table(const AngleGeom *__restrict__
 
 // pad[py][px] = img[py-2][px-2] (0 outside) and padT[px][py] = pad[py][px], tile by tile.
 template <class S>
-__global__ __launch_bounds__(256) void k_stage2d(const typename S::raw *__restrict__ img,
+__global__ __launch_bounds__(kMaxBlock) void k_stage2d(const typename S::raw *__restrict__ img,
                                                  typename S::raw *__restrict__ scratch, int nx,
                                                  int ny, int Wp, int Hp) {
     using raw = typename S::raw;
@@ -101,28 +113,69 @@ __global__ __launch_bounds__(256) void k_stage2d(const typename S::raw *__restri
     }
 }
 
+// spad[b][a][c] = sino[b][a][c-2] for c in [2, D+2), 0 in the two border cells each side.
+template <class S>
+__global__ __launch_bounds__(kMaxBlock) void k_stage_sino2d(const typename S::raw *__restrict__ sino,
+                                                      typename S::raw *__restrict__ spad, int D,
+                                                      int A) {
+    using raw = typename S::raw;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Dp = D + 2 * kPad;
+    if (c >= Dp) return;
+    const size_t row = (size_t)blockIdx.z * A + blockIdx.y;
+    const int k = c - kPad;
+    spad[row * Dp + c] = (k >= 0 && k < D) ? sino[row * D + k] : (raw)0.0f;
+}
+
 // ---------------------------------------------------------------- forward ------------------
 
-// March of one ray through a padded copy whose rows run along the ray's dominant axis.
-// (x, y) are padded-copy texel coordinates minus 0.5, so floor() gives the lower tap and
-// fract() its weight, as the linear texture filter does. `tbase` is the byte offset of the
-// copy inside the image's block and `rstride` the copy's row length in elements.
+// Bilinear sample at centred padded coordinate (x, y) from a copy whose rows run along the
+// dominant axis: floor() gives the lower tap relative to the copy's centre, fract() its
+// weight, as the linear texture filter does. `cofs` is the element offset of the copy's centre
+// inside the image's block, `rstride` the copy's row length in elements.
 template <class SI, bool YDOM>
-__device__ __forceinline__ float march2d(rsrc_t R, float4 e, int n, int rstride, int tbase) {
-    const int soff1 = tbase + rstride * (int)sizeof(typename SI::raw);
-    float acc = 0.0f, jf = 0.0f;
-    for (int j = 0; j < n; ++j, jf += 1.0f) {
-        const float x = fmaf(jf, e.z, e.x);
-        const float y = fmaf(jf, e.w, e.y);
-        const int ix = (int)floorf(x), iy = (int)floorf(y);
-        const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
-        const int o = YDOM ? __mul24(iy, rstride) + ix : __mul24(ix, rstride) + iy;
-        const Pair p0 = buf_pair<SI>(R, o, tbase); // YDOM: (ix, iy),(ix+1, iy)  else (ix, iy),(ix, iy+1)
-        const Pair p1 = buf_pair<SI>(R, o, soff1); // YDOM: row iy+1             else column ix+1
-        const float fa = YDOM ? fx : fy, fb = YDOM ? fy : fx;
-        const float u0 = fmaf(fa, p0.hi - p0.lo, p0.lo);
-        const float u1 = fmaf(fa, p1.hi - p1.lo, p1.lo);
-        acc += fmaf(fb, u1 - u0, u0);
+__device__ __forceinline__ float sample2d(rsrc_t R, float x, float y, int rstride, int soff1,
+                                         int cofs) {
+    const int ix = (int)floorf(x), iy = (int)floorf(y);
+    const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
+    const int o = (YDOM ? __mul24(iy, rstride) + ix : __mul24(ix, rstride) + iy) + cofs;
+    const Pair p0 = buf_pair<SI>(R, o, 0);     // YDOM: (ix, iy),(ix+1, iy)  else (ix, iy),(ix, iy+1)
+    const Pair p1 = buf_pair<SI>(R, o, soff1); // YDOM: row iy+1             else column ix+1
+    const float fa = YDOM ? fx : fy, fb = YDOM ? fy : fx;
+    const float u0 = fmaf(fa, p0.hi - p0.lo, p0.lo);
+    const float u1 = fmaf(fa, p1.hi - p1.lo, p1.lo);
+    return fmaf(fb, u1 - u0, u0);
+}
+
+// March of one ray: sample j sits at m + (j - N/2) v. Samples are summed in blocks of
+// kSumBlock into partial sums that are then added to the total, which keeps the FP32 rounding
+// of a ~1000-sample sum near that of a 32-term sum; inside a block two partials alternate so
+// that each lane has two samples' loads in flight.
+constexpr int kSumBlock = 32;
+
+template <class SI, bool YDOM>
+__device__ __forceinline__ float march2d(rsrc_t R, float4 e, float4 q, int rstride, int cofs) {
+    const int n = __float_as_int(q.y);
+    const int soff1 = rstride * (int)sizeof(typename SI::raw);
+    float acc = 0.0f, jf = q.z;
+    int j = 0;
+    while (j < n) {
+        const int jend = min(j + kSumBlock, n);
+        float s0 = 0.0f, s1 = 0.0f;
+        for (; j + 1 < jend; j += 2, jf += 2.0f) {
+            const float jg = jf + 1.0f;
+            s0 += sample2d<SI, YDOM>(R, fmaf(jf, e.z, e.x), fmaf(jf, e.w, e.y), rstride, soff1,
+                                     cofs);
+            s1 += sample2d<SI, YDOM>(R, fmaf(jg, e.z, e.x), fmaf(jg, e.w, e.y), rstride, soff1,
+                                     cofs);
+        }
+        if (j < jend) {
+            s0 += sample2d<SI, YDOM>(R, fmaf(jf, e.z, e.x), fmaf(jf, e.w, e.y), rstride, soff1,
+                                     cofs);
+            ++j;
+            jf += 1.0f;
+        }
+        acc += s0 + s1;
     }
     return acc;
 }
@@ -130,9 +183,9 @@ __device__ __forceinline__ float march2d(rsrc_t R, float4 e, int n, int rstride,
 // One thread per ray (a, k); a wavefront holds 64 adjacent cells of one angle.
 // sino[b][a][k] = scale * sum_{j<N} bilinear(img, e + j v)   (forward_2d.hpp:63-78)
 template <class SI, class SS>
-__global__ __launch_bounds__(256) void k_fwd2d(const typename SI::raw *__restrict__ scratch,
+__global__ __launch_bounds__(kMaxBlock) void k_fwd2d(const typename SI::raw *__restrict__ scratch,
                                                int Wp, int Hp, const float4 *__restrict__ ev,
-                                               const float2 *__restrict__ ns,
+                                               const float4 *__restrict__ ns,
                                                typename SS::raw *__restrict__ sino, int D,
                                                int A) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -141,15 +194,15 @@ __global__ __launch_bounds__(256) void k_fwd2d(const typename SI::raw *__restric
     const size_t b = blockIdx.z;
     const size_t r = (size_t)a * D + k;
     const float4 e = ev[r];
-    const float2 q = ns[r];
-    const int n = __float_as_int(q.y);
-    const uint32_t one = (uint32_t)Wp * Hp;
-    const rsrc_t R = make_rsrc(scratch + b * 2 * one, bytes_of<SI>(2 * (uint64_t)one));
+    const float4 q = ns[r];
+    const int one = Wp * Hp;
+    const rsrc_t R = make_rsrc(scratch + b * 2 * (size_t)one, bytes_of<SI>(2 * (uint64_t)one));
+    const int c0x = Wp / 2, c0y = Hp / 2;
     float acc;
     if (fabsf(e.w) == 1.0f) // y is the dominant axis: row-major copy
-        acc = march2d<SI, true>(R, e, n, Wp, 0);
+        acc = march2d<SI, true>(R, e, q, Wp, c0y * Wp + c0x);
     else                    // x dominant: transposed copy, second half of the block
-        acc = march2d<SI, false>(R, e, n, Hp, (int)(one * sizeof(typename SI::raw)));
+        acc = march2d<SI, false>(R, e, q, Hp, one + c0x * Hp + c0y);
     sino[b * (size_t)A * D + r] = from_f32<SS>(acc * q.x);
 }
 
@@ -158,8 +211,8 @@ __global__ __launch_bounds__(256) void k_fwd2d(const typename SI::raw *__restric
 // One thread per pixel, all angles in one pass (backward_2d.hpp:31-99):
 // vol[b][y][x] += (1/du) sum_a w_a lerp(sino[b][a], w_a eta_a + D/2),  w_a = DSD/(xi_a + DSO).
 template <class SV, class SS>
-__global__ __launch_bounds__(256) void k_bp2d(typename SV::raw *__restrict__ vol,
-                                              const typename SS::raw *__restrict__ sino,
+__global__ __launch_bounds__(kMaxBlock) void k_bp2d(typename SV::raw *__restrict__ vol,
+                                              const typename SS::raw *__restrict__ spad,
                                               const float4 *__restrict__ meta, int nx, int ny,
                                               int D, int A, float sx, float sy, float ox,
                                               float oy, float inv_du, float half_d) {
@@ -167,30 +220,27 @@ __global__ __launch_bounds__(256) void k_bp2d(typename SV::raw *__restrict__ vol
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
     if (ix >= nx || iy >= ny) return;
     const size_t b = blockIdx.z;
-    const uint32_t sino_elems = (uint32_t)A * D;
-    const rsrc_t R = make_rsrc(sino + b * sino_elems, bytes_of<SS>(sino_elems));
+    const int Dp = D + 2 * kPad;
+    const uint32_t sino_elems = (uint32_t)A * Dp;
+    const rsrc_t R = make_rsrc(spad + b * sino_elems, bytes_of<SS>(sino_elems));
     const float x = ((float)ix - (float)nx / 2.0f) * sx + ox + 0.5f;
     const float y = ((float)iy - (float)ny / 2.0f) * sy + oy + 0.5f;
     const float px = x * inv_du, py = y * inv_du;
-    const float dmax = (float)(D - 1);
+    // Lower-tap coordinate u - 0.5, clamped into [-2, D]: outside the detector both taps then
+    // fall in the zero border, so no per-tap test is needed.
+    const float hi = (float)D;
     float acc = 0.0f;
-    int row = 0;
+    int row = kPad;
 #pragma unroll 4
-    for (int a = 0; a < A; ++a, row += D) {
+    for (int a = 0; a < A; ++a, row += Dp) {
         const float4 m = meta[a]; // wave-uniform: scalar loads
         const float xi = m.x * x - m.y * y;
         const float eta = m.y * px + m.x * py;
         const float w = m.z * __builtin_amdgcn_rcpf(xi + m.w);
-        const float u = fmaf(w, eta, half_d) - 0.5f;
-        const float fl = floorf(u);
-        const float al = u - fl;
-        const float flc = fminf(fmaxf(fl, -1.0f), dmax);
-        const int o = row + (int)flc;
-        float t0 = buf_load<SS>(R, o);
-        float t1 = buf_load<SS>(R, o + 1);
-        t0 = (fl >= 0.0f && fl <= dmax) ? t0 : 0.0f;
-        t1 = (fl >= -1.0f && fl < dmax) ? t1 : 0.0f;
-        acc = fmaf(fmaf(al, t1 - t0, t0), w, acc);
+        const float u = __builtin_amdgcn_fmed3f(fmaf(w, eta, half_d) - 0.5f, -2.0f, hi);
+        const float al = __builtin_amdgcn_fractf(u);
+        const Pair p = buf_pair<SS>(R, row + (int)floorf(u), 0);
+        acc = fmaf(fmaf(al, p.hi - p.lo, p.lo), w, acc);
     }
     const size_t i = b * (size_t)nx * ny + (size_t)iy * nx + ix;
     vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc * inv_du);
@@ -204,11 +254,11 @@ __global__ __launch_bounds__(256) void k_bp2d(typename SV::raw *__restrict__ vol
 // from the ray's dominant axis; each candidate is then re-evaluated with the forward kernel's
 // own FP32 expressions, so the weights are the forward's.
 template <class SV, class SS>
-__global__ __launch_bounds__(256) void k_adj2d(typename SV::raw *__restrict__ vol,
+__global__ __launch_bounds__(kMaxBlock) void k_adj2d(typename SV::raw *__restrict__ vol,
                                                const typename SS::raw *__restrict__ sino,
                                                const float *__restrict__ ageo,
                                                const float4 *__restrict__ ev,
-                                               const float2 *__restrict__ ns, int nx, int ny,
+                                               const float4 *__restrict__ ns, int nx, int ny,
                                                int D, int A) {
     const int tx = blockIdx.x * blockDim.x + threadIdx.x;
     const int ty = blockIdx.y * blockDim.y + threadIdx.y;
@@ -219,8 +269,8 @@ __global__ __launch_bounds__(256) void k_adj2d(typename SV::raw *__restrict__ vo
     // footprint box in the plan's texel-centre frame
     const float qx[2] = {(float)tx - 0.5f, (float)tx + 1.5f};
     const float qy[2] = {(float)ty - 0.5f, (float)ty + 1.5f};
-    // the texel's lower-tap index in the ray table's padded frame
-    const int px = tx + kPad, py = ty + kPad;
+    // the texel's lower-tap index in the ray table's centred padded frame
+    const int px = tx + kPad - (nx + 2 * kPad) / 2, py = ty + kPad - (ny + 2 * kPad) / 2;
     const float fpx = (float)px, fpy = (float)py;
     float acc = 0.0f;
     for (int a = 0; a < A; ++a) {
@@ -240,23 +290,22 @@ __global__ __launch_bounds__(256) void k_adj2d(typename SV::raw *__restrict__ vo
         const int k1 = min((int)ceilf(kmax + 0.01f), D - 1);
         for (int k = k0; k <= k1; ++k) {
             const size_t r = (size_t)a * D + k;
-            const float2 q = ns[r];
+            const float4 q = ns[r];
             const int n = __float_as_int(q.y);
             if (n == 0) continue;
             const float4 e = ev[r];
             const bool ydom = fabsf(e.w) == 1.0f;
-            // samples whose dominant coordinate can floor to p-1 or p
-            const float ed = ydom ? e.y : e.x, vd = ydom ? e.w : e.z;
+            // samples whose dominant coordinate can floor to p-1 or p; jf = j - N/2
+            const float md = ydom ? e.y : e.x, vd = ydom ? e.w : e.z;
             const float td = ydom ? fpy : fpx;
-            const float lo = (td - ed) * vd - 1.0f;
-            const int j0 = max((int)floorf(lo - 1e-3f), 0);
-            const int j1 = min((int)floorf(lo + 2.0f + 1e-3f), n - 1);
-            if (j0 > j1) continue;
+            const float lo = (td - md) * vd - 1.0f;
+            const float jlo = fmaxf(floorf(lo - 1e-3f), q.z);
+            const float jhi = fminf(floorf(lo + 2.0f + 1e-3f), q.z + (float)(n - 1));
+            if (jlo > jhi) continue;
             const float yv = buf_load<SS>(R, (int)r) * q.x;
             float part = 0.0f;
-            for (int j = j0; j <= j1; ++j) {
+            for (float jf = jlo; jf <= jhi; jf += 1.0f) {
                 // the forward march's own expressions (march2d)
-                const float jf = (float)j;
                 const float x = fmaf(jf, e.z, e.x);
                 const float y = fmaf(jf, e.w, e.y);
                 const int ix = (int)floorf(x), iy = (int)floorf(y);
@@ -308,7 +357,7 @@ cmi_status check_call(const cmi_plan2d *p, const void *vol, int vol_dtype, const
         return fail(CMI_ERR_DTYPE,
                     "%s: plan precision does not match sinogram precision (%s)", fn,
                     dtype_name(sino_dtype));
-    const uint64_t img = (uint64_t)p->g.nx * p->g.ny, sn = (uint64_t)p->A * p->g.D;
+    const uint64_t img = (uint64_t)p->g.nx * p->g.ny, sn = (uint64_t)p->A * (p->g.D + 2 * kPad);
     const uint64_t pad = 2 * (uint64_t)(p->g.nx + 2 * kPad) * (p->g.ny + 2 * kPad);
     if (std::max({img, sn, pad}) * 4 >= (1ull << 31))
         return fail(CMI_ERR_INVALID_ARGUMENT, "%s: one image or sinogram exceeds 2 GiB", fn);
@@ -404,9 +453,9 @@ CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vo
     CMI_HIP_TRY(hipMalloc(&p->d_meta, sizeof(float4) * A));
     CMI_HIP_TRY(hipMalloc(&p->d_ageo, sizeof(float) * 6 * A));
     CMI_HIP_TRY(hipMalloc(&p->d_ev, sizeof(float4) * rays));
-    CMI_HIP_TRY(hipMalloc(&p->d_ns, sizeof(float2) * rays));
+    CMI_HIP_TRY(hipMalloc(&p->d_ns, sizeof(float4) * rays));
     p->table_bytes = sizeof(AngleGeom) * A + sizeof(float4) * A + sizeof(float) * 6 * A +
-                     (sizeof(float4) + sizeof(float2)) * rays;
+                     2 * sizeof(float4) * rays;
     unsigned long long *d_total = nullptr;
     CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
     std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
@@ -456,11 +505,13 @@ CMI_API cmi_status cmi_plan2d_destroy(cmi_plan2d *plan) {
 CMI_API cmi_status cmi_plan2d_set_block(cmi_plan2d *plan, uint32_t block_x, uint32_t block_y) {
     clear_error();
     if (!plan) return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_set_block: plan is null");
-    if (block_x == 0 || block_x % 64 != 0 || block_y == 0 || block_x * block_y > 1024)
+    // The forward kernel is compiled for at most kMaxBlock threads per workgroup
+    // (__launch_bounds__); a larger workgroup must never reach the launch.
+    if (block_x == 0 || block_x % 64 != 0 || block_y == 0 || block_x * block_y > kMaxBlock)
         return fail(CMI_ERR_INVALID_ARGUMENT,
                     "plan2d_set_block: need block_x a multiple of 64 and block_x*block_y <= "
-                    "1024 (got %u x %u)",
-                    block_x, block_y);
+                    "%d (got %u x %u)",
+                    kMaxBlock, block_x, block_y);
     plan->block_x = block_x;
     plan->block_y = block_y;
     return CMI_OK;
@@ -536,17 +587,28 @@ CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol
     CMI_TRY(guard.enter(plan->device));
     const hipStream_t s = (hipStream_t)stream;
     const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
-    return dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
+    const int Dp = D + 2 * kPad;
+    void *scratch = nullptr;
+    CMI_HIP_TRY(hipMallocAsync(&scratch, (size_t)Dp * A * dtype_size(sino_dtype) * batch, s));
+    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
         using SV = decltype(sv);
         using SS = decltype(ss);
+        using RS = typename SS::raw;
+        dim3 pb(256), pg((unsigned)((Dp + 255) / 256), (unsigned)A, (unsigned)batch);
+        hipLaunchKernelGGL((k_stage_sino2d<SS>), pg, pb, 0, s, (const RS *)sinogram,
+                           (RS *)scratch, D, A);
         dim3 b(64, 4), g((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)batch);
         hipLaunchKernelGGL((k_bp2d<SV, SS>), g, b, 0, s, (typename SV::raw *)volume,
-                           (const typename SS::raw *)sinogram, plan->d_meta, nx, ny, D, A,
-                           (float)plan->g.sx, (float)plan->g.sy, (float)plan->g.ox,
-                           (float)plan->g.oy, (float)(1.0 / plan->g.du), (float)D / 2.0f);
+                           (const RS *)scratch, plan->d_meta, nx, ny, D, A, (float)plan->g.sx,
+                           (float)plan->g.sy, (float)plan->g.ox, (float)plan->g.oy,
+                           (float)(1.0 / plan->g.du), (float)D / 2.0f);
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });
+    const hipError_t fe = hipFreeAsync(scratch, s);
+    if (st != CMI_OK) return st;
+    CMI_HIP_TRY(fe);
+    return CMI_OK;
 }
 
 CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,

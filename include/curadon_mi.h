@@ -89,8 +89,8 @@ CMI_API cmi_status cmi_plan2d_destroy(cmi_plan2d *plan);
 
 /* Launch-shape tunables, the analogue of the r/w forward_block_x / forward_block_y
  * properties (plan.cu:57-58, plan_2d.hpp:184-186). block_x = detector cells per
- * workgroup (multiple of 64, <= 1024), block_y = angles per workgroup, with
- * block_x * block_y <= 1024. */
+ * workgroup (a multiple of 64), block_y = angles per workgroup, with
+ * block_x * block_y <= 256; other shapes are rejected with CMI_ERR_INVALID_ARGUMENT. */
 CMI_API cmi_status cmi_plan2d_set_block(cmi_plan2d *plan, uint32_t block_x, uint32_t block_y);
 CMI_API cmi_status cmi_plan2d_get_block(const cmi_plan2d *plan, uint32_t *block_x,
                                         uint32_t *block_y);

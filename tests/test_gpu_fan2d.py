@@ -175,13 +175,24 @@ def test_forward_deterministic_and_block_independent(g1, gpu):
     plan = geom.plan
     old = (plan.forward_block_x, plan.forward_block_y)
     try:
-        plan.forward_block_x, plan.forward_block_y = 128, 2
+        plan.set_block(128, 2)
         c = cm.forward(x, geom)
     finally:
-        plan.forward_block_x, plan.forward_block_y = old
+        plan.set_block(*old)
     assert torch.equal(a, b) and torch.equal(a, c)
     y = torch.rand(360, 91, device=gpu)
     assert torch.equal(cm.adjoint(y, geom), cm.adjoint(y, geom))
+
+
+def test_block_shape_limit(g1):
+    """Workgroups above the kernels' 256-thread launch bound are refused on the host."""
+    plan = g1[1].plan
+    old = (plan.forward_block_x, plan.forward_block_y)
+    with pytest.raises(ValueError):
+        plan.forward_block_y = 8          # 64 x 8 = 512 threads
+    with pytest.raises(ValueError):
+        plan.forward_block_x = 96         # not a multiple of 64
+    assert (plan.forward_block_x, plan.forward_block_y) == old
 
 
 def test_batched_equals_single(gpu):
