@@ -7,7 +7,8 @@ Usage: ``import curadon_amd as curadon``.
 """
 from .geometry import ConeGeometry, FanGeometry
 from . import _lib
-from ._lib import CuradonError, adjoint_2d, backward_2d, forward_2d, plan_2d
+from ._lib import (CuradonError, adjoint_2d, backward_2d, backward_3d, forward_2d, forward_3d,
+                   plan_2d, plan_3d)
 from .forward import forward
 from .backward import adjoint, backward
 from .diffable import TorchAdjoint, TorchBackprojection, TorchForward
@@ -15,5 +16,5 @@ from .diffable import TorchAdjoint, TorchBackprojection, TorchForward
 __all__ = [
     "FanGeometry", "ConeGeometry", "forward", "backward", "adjoint", "TorchForward",
     "TorchBackprojection", "TorchAdjoint", "plan_2d", "forward_2d", "backward_2d",
-    "adjoint_2d", "CuradonError",
+    "adjoint_2d", "forward_3d", "backward_3d", "plan_3d", "CuradonError",
 ]

@@ -107,6 +107,7 @@ SIGNATURES = {
     "cmi_adjoint_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_plan3d_create": (_i32, [_i32, _i32, _i32, C.POINTER(Cone3DDesc), C.POINTER(_vp)]),
     "cmi_plan3d_destroy": (_i32, [_vp]),
+    "cmi_plan3d_get_info": (_i32, [_vp, C.POINTER(Plan3DInfo)]),
     "cmi_forward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_backward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_adjoint_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
@@ -329,3 +330,157 @@ def adjoint_2d(volume, sinogram, plan):
         return
     check(_lib.cmi_adjoint_2d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc, b,
                               stream_of(volume)))
+
+
+# ----------------------------------------------------------------------------- 3D -------
+
+class plan_3d:
+    """Per-geometry state of the 3D cone-beam operators. The reference has no 3D plan (it
+    rebuilds its tables on every call, forward_3d.hpp:214-274); this one holds the per-angle
+    geometry and the backprojection matrices on ``device``.
+
+    Arguments follow curadon_ext.forward_3d (forward.cu:89-104) in Python axis order:
+    vol_shape / vol_spacing / vol_offset (z, y, x), angles phi / theta / psi (A,),
+    det_shape / det_spacing / det_offset (v, u), det_rotation (pitch, roll, yaw)."""
+
+    def __init__(self, device, vol_prec, det_prec, vol_shape, vol_spacing, vol_offset, phi,
+                 theta, psi, det_shape, det_spacing, det_offset, det_rotation, DSO, DSD, COR):
+        self._h = None
+        self.device = int(device)
+        self.vol_prec = precision_code(vol_prec)
+        self.det_prec = precision_code(det_prec)
+        self._phi = _f64(phi)
+        A = self._phi.size
+        self._theta = _f64(theta, A)
+        self._psi = _f64(psi, A)
+        d = Cone3DDesc()
+        vs = [int(v) for v in np.asarray(vol_shape).reshape(-1)]
+        ds = [int(v) for v in np.asarray(det_shape).reshape(-1)]
+        if len(vs) != 3 or len(ds) != 2:
+            raise AttributeError("vol_shape needs 3 values and det_shape 2")
+        d.vol_shape[:] = vs
+        d.vol_spacing[:] = list(_f64(vol_spacing, 3))
+        d.vol_offset[:] = list(_f64(vol_offset, 3))
+        d.det_shape[:] = ds
+        d.det_spacing[:] = list(_f64(det_spacing, 2))
+        d.det_offset[:] = list(_f64(det_offset, 2))
+        d.det_rotation[:] = list(_f64(det_rotation, 3))
+        d.DSO, d.DSD, d.COR = float(DSO), float(DSD), float(COR)
+        d.nangles = A
+        d.phi = self._phi.ctypes.data_as(_dp)
+        d.theta = self._theta.ctypes.data_as(_dp)
+        d.psi = self._psi.ctypes.data_as(_dp)
+        self.vol_shape = tuple(vs)
+        self.sino_shape = (A, ds[0], ds[1])
+        self.nangles = A
+        h = _vp()
+        check(_lib.cmi_plan3d_create(self.device, self.vol_prec, self.det_prec, C.byref(d),
+                                     C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        i = Plan3DInfo()
+        check(_lib.cmi_plan3d_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in Plan3DInfo._fields_}
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h is not None and _lib is not None:
+            _lib.cmi_plan3d_destroy(h)
+
+
+def _check_pair_3d(volume, sinogram, plan, which):
+    _require_device_tensor(volume, "volume")
+    _require_device_tensor(sinogram, "sinogram")
+    if volume.device != sinogram.device:
+        raise AttributeError("Volume and sinogram must be on the same device")   # forward.cu:106
+    if volume.device.index != plan.device:
+        raise AttributeError("Volume and plan are not on the same device")
+    vc, sc = tensor_code(volume), tensor_code(sinogram)
+    if vc != plan.vol_prec:
+        raise AttributeError(f"Plan precision does not match volume precision ({_CODE_NAME[vc]})")
+    if sc != plan.det_prec:
+        raise AttributeError(
+            f"Plan precision does not match sinogram precision ({_CODE_NAME[sc]})")
+    if volume.dim() < 3 or tuple(volume.shape[-3:]) != plan.vol_shape:
+        raise ValueError(f"{which}: volume shape {tuple(volume.shape)} does not end in {plan.vol_shape}")
+    if sinogram.dim() < 3 or tuple(sinogram.shape[-3:]) != plan.sino_shape:
+        raise ValueError(f"{which}: sinogram shape {tuple(sinogram.shape)} does not end in "
+                         f"{plan.sino_shape}")
+    if tuple(volume.shape[:-3]) != tuple(sinogram.shape[:-3]):
+        raise ValueError(f"{which}: batch shapes differ")
+    b = 1
+    for v in volume.shape[:-3]:
+        b *= int(v)
+    return vc, sc, b
+
+
+def forward_3d_plan(volume, sinogram, plan):
+    """sinogram := A volume (cone beam). Rays that miss the volume write 0."""
+    vc, sc, b = _check_pair_3d(volume, sinogram, plan, "forward_3d")
+    if b:
+        check(_lib.cmi_forward_3d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc,
+                                  b, stream_of(volume)))
+
+
+def backward_3d_plan(volume, sinogram, plan, mode="reference"):
+    """volume += B sinogram; mode 'reference' (the reference's projection) or 'matched'."""
+    m = {"reference": BP3D_REFERENCE, "matched": BP3D_MATCHED}.get(mode)
+    if m is None:
+        raise ValueError(f"unknown backprojection mode {mode!r}")
+    vc, sc, b = _check_pair_3d(volume, sinogram, plan, "backward_3d")
+    if b:
+        check(_lib.cmi_backward_3d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc,
+                                   b, m, stream_of(volume)))
+
+
+def adjoint_3d_plan(volume, sinogram, plan):
+    """volume += A^T sinogram (transpose of forward_3d_plan)."""
+    vc, sc, b = _check_pair_3d(volume, sinogram, plan, "adjoint_3d")
+    if b:
+        check(_lib.cmi_adjoint_3d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc, b,
+                                  stream_of(volume)))
+
+
+_PLAN3D_CACHE = {}
+
+
+def _plan3d_for(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
+                det_spacing, det_offset, det_rotation, DSO, DSD, COR):
+    key = (x.device.index, x.dtype, sinogram.dtype,
+           tuple(np.asarray(vol_shape).reshape(-1).tolist()),
+           tuple(_f64(vol_spacing).tolist()), tuple(_f64(vol_offset).tolist()),
+           _f64(phi).tobytes(), _f64(theta).tobytes(), _f64(psi).tobytes(),
+           tuple(np.asarray(sino_shape).reshape(-1).tolist()), tuple(_f64(det_spacing).tolist()),
+           tuple(_f64(det_offset).tolist()), tuple(_f64(det_rotation).tolist()),
+           float(DSO), float(DSD), float(COR))
+    p = _PLAN3D_CACHE.get(key)
+    if p is None:
+        if len(_PLAN3D_CACHE) >= 8:
+            _PLAN3D_CACHE.pop(next(iter(_PLAN3D_CACHE)))
+        p = plan_3d(x.device.index, x.dtype, sinogram.dtype, vol_shape, vol_spacing, vol_offset,
+                    phi, theta, psi, sino_shape, det_spacing, det_offset, det_rotation, DSO, DSD,
+                    COR)
+        _PLAN3D_CACHE[key] = p
+    return p
+
+
+def forward_3d(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
+               det_spacing, det_offset, det_rotation, DSO, DSD, COR):
+    """curadon_ext.forward_3d with its 15 arguments (forward.cu:89-141); the plan for the
+    geometry is built once and cached."""
+    p = _plan3d_for(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
+                    det_spacing, det_offset, det_rotation, DSO, DSD, COR)
+    forward_3d_plan(x, sinogram, p)
+
+
+def backward_3d(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
+                det_spacing, det_offset, det_rotation, DSO, DSD, COR):
+    """curadon_ext.backward_3d with its 15 arguments (backward.cu:230-282), reference mode."""
+    p = _plan3d_for(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
+                    det_spacing, det_offset, det_rotation, DSO, DSD, COR)
+    backward_3d_plan(x, sinogram, p)

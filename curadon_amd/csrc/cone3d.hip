@@ -1,44 +1,693 @@
-// 3D cone-beam projectors for gfx950 (placeholder entry points until the kernels land).
+// 3D cone-beam projectors for gfx950: plan, forward ray march, voxel-driven backprojection
+// (reference and geometry-matched projection) and the transposed ray march.
+//
+// Data layout in HBM
+//   plan tables  fgeo [A] FwdGeom (FP64)     detector pixel (0,0), u/v steps, source (voxel frame)
+//                mref [A][12] float          reference-mode projection matrix (see bp_matrices)
+//                mmat [A][12] float          geometry-matched projection matrix
+//   per call     forward:  [B][nz+4][ny+4][nx+4] volume with a 2-voxel zero border, x fastest
+//                backward: [B][A][V+4][U+4]      panels with a 2-pixel zero border
+//                adjoint:  [B][nz][ny][nx] FP32 accumulator when the volume is stored in 16 bits
+//
+// Forward rays take their discrete quantities (hit, step count) from an FP64 set-up in the
+// kernel, identical to the host oracle's, then march in FP32 from the ray's midpoint in
+// volume-centred padded coordinates, summing in blocks of 32 samples.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
 #include "cmi_internal.hpp"
+#include "geometry3d.hpp"
 
 using namespace cmi;
+using cmi::cone3d::FwdGeom;
 
 struct cmi_plan3d {
     int device = 0;
+    int vol_prec = CMI_F32, det_prec = CMI_F32;
+    cone3d::Desc g{};
+    int64_t A = 0;
+    uint64_t total_samples = 0;
+    uint64_t table_bytes = 0;
+    FwdGeom *d_fgeo = nullptr;
+    float *d_mref = nullptr;
+    float *d_mmat = nullptr;
 };
+
+namespace {
+
+constexpr int kPad = 2;        // zero border of staged volumes and panels
+constexpr int kMaxBlock = 256; // threads per workgroup every kernel here is compiled for
+constexpr int kSumBlock = 32;  // samples per partial sum in the forward march
+constexpr int kZB = 8;         // z voxels per thread in the backprojector
+
+// ---------------------------------------------------------------- ray parameters ----------
+
+// FP32 march parameters of one ray in centred padded coordinates (voxel coordinate + kPad
+// minus the padded volume's integer centre c0): midpoint m, step st, first index j0 = -nst/2,
+// sample count n = nst + 1. Shared by the forward march and its transpose so both see the same
+// FP32 positions.
+struct March3 {
+    float m[3], st[3];
+    float j0;
+    int n;
+};
+
+__device__ __forceinline__ March3 march_params(const FwdGeom &g, int nx, int ny, int nz, int iu,
+                                               int iv, int Wp, int Hp, int Dp) {
+    March3 p{};
+    const cone3d::Ray3 r = cone3d::setup_ray3(g, nx, ny, nz, iu, iv);
+    if (!r.hit) return p; // n = 0
+    const int64_t im = r.nst / 2;
+    const double tm = r.tmin + (double)im * r.dt;
+    const double c0[3] = {(double)(Wp / 2), (double)(Hp / 2), (double)(Dp / 2)};
+    for (int d = 0; d < 3; ++d) {
+        p.m[d] = (float)(r.s[d] + tm * r.d[d] + (double)kPad - c0[d]);
+        p.st[d] = (float)(r.dt * r.d[d]);
+    }
+    p.j0 = (float)(-im);
+    p.n = (int)(r.nst + 1);
+    return p;
+}
+
+__global__ __launch_bounds__(kMaxBlock) void k_count3d(const FwdGeom *__restrict__ geo, int U,
+                                                       int V, int nx, int ny, int nz,
+                                                       unsigned long long *__restrict__ total) {
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iv = blockIdx.y;
+    const int a = blockIdx.z;
+    unsigned long long n = 0;
+    if (iu < U) {
+        const cone3d::Ray3 r = cone3d::setup_ray3(geo[a], nx, ny, nz, iu, iv);
+        n = r.hit ? (unsigned long long)(r.nst + 1) : 0ull;
+    }
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(total, n);
+}
+
+// ---------------------------------------------------------------- staging -----------------
+
+// pad[b][zp][yp][xp] = vol[b][zp-2][yp-2][xp-2], 0 in the border.
+template <class S>
+__global__ __launch_bounds__(kMaxBlock) void k_stage3d(const typename S::raw *__restrict__ vol,
+                                                       typename S::raw *__restrict__ pad, int nx,
+                                                       int ny, int nz, int Wp, int Hp, int Dp) {
+    using raw = typename S::raw;
+    const int xp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (xp >= Wp) return;
+    const int yp = blockIdx.y;
+    const int zp = blockIdx.z % Dp;
+    const size_t b = blockIdx.z / Dp;
+    const int x = xp - kPad, y = yp - kPad, z = zp - kPad;
+    raw v = (raw)0.0f;
+    if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+        v = vol[((b * nz + z) * ny + y) * (size_t)nx + x];
+    pad[((b * Dp + zp) * Hp + yp) * (size_t)Wp + xp] = v;
+}
+
+// ppad[b][a][vp][up] = sino[b][a][vp-2][up-2], 0 in the border.
+template <class S>
+__global__ __launch_bounds__(kMaxBlock) void k_stage_panel(const typename S::raw *__restrict__ sino,
+                                                           typename S::raw *__restrict__ ppad,
+                                                           int U, int V, int A) {
+    using raw = typename S::raw;
+    const int Up = U + 2 * kPad, Vp = V + 2 * kPad;
+    const int up = blockIdx.x * blockDim.x + threadIdx.x;
+    if (up >= Up) return;
+    const int vp = blockIdx.y;
+    const size_t ab = blockIdx.z; // b * A + a
+    const int u = up - kPad, v = vp - kPad;
+    raw val = (raw)0.0f;
+    if (u >= 0 && u < U && v >= 0 && v < V) val = sino[(ab * V + v) * (size_t)U + u];
+    ppad[(ab * Vp + vp) * (size_t)Up + up] = val;
+}
+
+__global__ __launch_bounds__(kMaxBlock) void k_fill_zero(float *__restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// vol[i] += acc[i] in the volume's storage type.
+template <class S>
+__global__ __launch_bounds__(kMaxBlock) void k_add_into(typename S::raw *__restrict__ vol,
+                                                        const float *__restrict__ acc, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) vol[i] = from_f32<S>(to_f32<S>(vol[i]) + acc[i]);
+}
+
+// ---------------------------------------------------------------- forward -----------------
+
+// Trilinear sample at centred padded coordinates from the x-fastest padded volume.
+// `cofs` is the element offset of the padded volume's centre; sW, sH, sHW are the byte
+// strides of one row, one slice, one slice plus one row.
+template <class SI>
+__device__ __forceinline__ float sample3d(rsrc_t R, float x, float y, float z, int Wp, int Hp,
+                                         int cofs, int sW, int sHW, int sHWW) {
+    const int ix = (int)floorf(x), iy = (int)floorf(y), iz = (int)floorf(z);
+    const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y),
+                fz = __builtin_amdgcn_fractf(z);
+    const int o = __mul24(__mul24(iz, Hp) + iy, Wp) + ix + cofs;
+    const Pair p00 = buf_pair<SI>(R, o, 0);    // (y,   z)
+    const Pair p10 = buf_pair<SI>(R, o, sW);   // (y+1, z)
+    const Pair p01 = buf_pair<SI>(R, o, sHW);  // (y,   z+1)
+    const Pair p11 = buf_pair<SI>(R, o, sHWW); // (y+1, z+1)
+    const float a00 = fmaf(fx, p00.hi - p00.lo, p00.lo);
+    const float a10 = fmaf(fx, p10.hi - p10.lo, p10.lo);
+    const float a01 = fmaf(fx, p01.hi - p01.lo, p01.lo);
+    const float a11 = fmaf(fx, p11.hi - p11.lo, p11.lo);
+    const float b0 = fmaf(fy, a10 - a00, a00);
+    const float b1 = fmaf(fy, a11 - a01, a01);
+    return fmaf(fz, b1 - b0, b0);
+}
+
+// One thread per ray (iu, iv) of projection a; a wavefront covers a 16 x 4 pixel tile.
+// sino[b][a][iv][iu] = sum_{i=0..nst} trilinear(vol, s + (tmin + i dt) d)  (forward_3d.hpp:87-93)
+template <class SI, class SS>
+__global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__restrict__ pad,
+                                                     const FwdGeom *__restrict__ geo, int nx,
+                                                     int ny, int nz, int U, int V, int A,
+                                                     typename SS::raw *__restrict__ sino) {
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iv = blockIdx.y * blockDim.y + threadIdx.y;
+    const int a = blockIdx.z % A;
+    const size_t b = blockIdx.z / A;
+    if (iu >= U || iv >= V) return;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
+    const March3 p = march_params(geo[a], nx, ny, nz, iu, iv, Wp, Hp, Dp);
+    const uint32_t one = (uint32_t)Wp * Hp * Dp;
+    const rsrc_t R = make_rsrc(pad + b * one, bytes_of<SI>(one));
+    const int es = (int)sizeof(typename SI::raw);
+    const int sW = Wp * es, sHW = Wp * Hp * es, sHWW = sHW + sW;
+    const int cofs = ((Dp / 2) * Hp + Hp / 2) * Wp + Wp / 2;
+    float acc = 0.0f, jf = p.j0;
+    int j = 0;
+    while (j < p.n) {
+        const int jend = min(j + kSumBlock, p.n);
+        float s0 = 0.0f, s1 = 0.0f;
+        for (; j + 1 < jend; j += 2, jf += 2.0f) {
+            const float jg = jf + 1.0f;
+            s0 += sample3d<SI>(R, fmaf(jf, p.st[0], p.m[0]), fmaf(jf, p.st[1], p.m[1]),
+                               fmaf(jf, p.st[2], p.m[2]), Wp, Hp, cofs, sW, sHW, sHWW);
+            s1 += sample3d<SI>(R, fmaf(jg, p.st[0], p.m[0]), fmaf(jg, p.st[1], p.m[1]),
+                               fmaf(jg, p.st[2], p.m[2]), Wp, Hp, cofs, sW, sHW, sHWW);
+        }
+        if (j < jend) {
+            s0 += sample3d<SI>(R, fmaf(jf, p.st[0], p.m[0]), fmaf(jf, p.st[1], p.m[1]),
+                               fmaf(jf, p.st[2], p.m[2]), Wp, Hp, cofs, sW, sHW, sHWW);
+            ++j;
+            jf += 1.0f;
+        }
+        acc += s0 + s1;
+    }
+    sino[((b * A + a) * V + iv) * (size_t)U + iu] = from_f32<SS>(acc);
+}
+
+// ---------------------------------------------------------------- adjoint -----------------
+
+// Transposed march: every sample of ray (iu, iv, a) adds y * w to the voxels its trilinear
+// footprint reads, at the forward march's FP32 positions. Float atomics make the summation
+// order, and so the last bits, run-dependent.
+template <class SS>
+__global__ __launch_bounds__(kMaxBlock) void k_adj3d(float *__restrict__ acc,
+                                                     const typename SS::raw *__restrict__ sino,
+                                                     const FwdGeom *__restrict__ geo, int nx,
+                                                     int ny, int nz, int U, int V, int A) {
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iv = blockIdx.y * blockDim.y + threadIdx.y;
+    const int a = blockIdx.z % A;
+    const size_t b = blockIdx.z / A;
+    if (iu >= U || iv >= V) return;
+    const float y = to_f32<SS>(sino[((b * A + a) * V + iv) * (size_t)U + iu]);
+    if (y == 0.0f) return;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
+    const March3 p = march_params(geo[a], nx, ny, nz, iu, iv, Wp, Hp, Dp);
+    const int cx = Wp / 2 - kPad, cy = Hp / 2 - kPad, cz = Dp / 2 - kPad;
+    float *vb = acc + b * (size_t)nx * ny * nz;
+    float jf = p.j0;
+    for (int j = 0; j < p.n; ++j, jf += 1.0f) {
+        const float x = fmaf(jf, p.st[0], p.m[0]), yy = fmaf(jf, p.st[1], p.m[1]),
+                    z = fmaf(jf, p.st[2], p.m[2]);
+        const int ix = (int)floorf(x) + cx, iy = (int)floorf(yy) + cy, iz = (int)floorf(z) + cz;
+        const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(yy),
+                    fz = __builtin_amdgcn_fractf(z);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int X = ix + (c & 1), Y = iy + ((c >> 1) & 1), Z = iz + (c >> 2);
+            if (X < 0 || X >= nx || Y < 0 || Y >= ny || Z < 0 || Z >= nz) continue;
+            const float w = ((c & 1) ? fx : 1.0f - fx) * (((c >> 1) & 1) ? fy : 1.0f - fy) *
+                            ((c >> 2) ? fz : 1.0f - fz);
+            atomicAdd(vb + ((size_t)Z * ny + Y) * nx + X, y * w);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- backprojection ----------
+
+// One thread per voxel column (ix, iy) and kZB consecutive z; a wavefront covers a 16 x 4 x 8
+// block. For each angle the 3x4 matrix M maps the centred voxel index h = (ix-cx, iy-cy,
+// iz-cz, 1) to (U, V, W); (U/W, V/W) is the lower-tap panel coordinate of the bilinear sample.
+// MATCHED drops voxels behind the source (W <= 0). vol += unweighted sum over angles.
+template <class SV, class SS, bool MATCHED>
+__global__ __launch_bounds__(kMaxBlock) void k_bp3d(typename SV::raw *__restrict__ vol,
+                                                    const typename SS::raw *__restrict__ ppad,
+                                                    const float *__restrict__ M, int nx, int ny,
+                                                    int nz, int U, int V, int A, float cx,
+                                                    float cy, float cz) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nzb = (nz + kZB - 1) / kZB;
+    const int iz0 = (blockIdx.z % nzb) * kZB;
+    const size_t b = blockIdx.z / nzb;
+    if (ix >= nx || iy >= ny) return;
+    const int Up = U + 2 * kPad, Vp = V + 2 * kPad;
+    const uint32_t panel = (uint32_t)Up * Vp;
+    const rsrc_t R = make_rsrc(ppad + b * (size_t)panel * A, bytes_of<SS>((uint64_t)panel * A));
+    const int sRow = Up * (int)sizeof(typename SS::raw);
+    const float hx = (float)ix - cx, hy = (float)iy - cy, hz = (float)iz0 - cz;
+    const float ulim = (float)U, vlim = (float)V;
+    float acc[kZB];
+#pragma unroll
+    for (int z = 0; z < kZB; ++z) acc[z] = 0.0f;
+    int base = kPad * Up + kPad;
+    for (int a = 0; a < A; ++a, base += panel) {
+        const float *m = M + 12 * a; // wave-uniform: scalar loads
+        float Uh = fmaf(m[0], hx, fmaf(m[1], hy, fmaf(m[2], hz, m[3])));
+        float Vh = fmaf(m[4], hx, fmaf(m[5], hy, fmaf(m[6], hz, m[7])));
+        float Wh = fmaf(m[8], hx, fmaf(m[9], hy, fmaf(m[10], hz, m[11])));
+#pragma unroll
+        for (int z = 0; z < kZB; ++z) {
+            const float rw = __builtin_amdgcn_rcpf(Wh);
+            const float u = __builtin_amdgcn_fmed3f(Uh * rw, -2.0f, ulim);
+            const float v = __builtin_amdgcn_fmed3f(Vh * rw, -2.0f, vlim);
+            const float fu = __builtin_amdgcn_fractf(u), fv = __builtin_amdgcn_fractf(v);
+            const int o = base + __mul24((int)floorf(v), Up) + (int)floorf(u);
+            const Pair p0 = buf_pair<SS>(R, o, 0);
+            const Pair p1 = buf_pair<SS>(R, o, sRow);
+            const float r0 = fmaf(fu, p0.hi - p0.lo, p0.lo);
+            const float r1 = fmaf(fu, p1.hi - p1.lo, p1.lo);
+            float s = fmaf(fv, r1 - r0, r0);
+            if (MATCHED) s = Wh > 0.0f ? s : 0.0f;
+            acc[z] += s;
+            Uh += m[2];
+            Vh += m[6];
+            Wh += m[10];
+        }
+    }
+#pragma unroll
+    for (int z = 0; z < kZB; ++z) {
+        const int iz = iz0 + z;
+        if (iz < nz) {
+            const size_t i = ((b * nz + iz) * ny + iy) * (size_t)nx + ix;
+            vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc[z]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host helpers -----------
+
+// Projection matrices for the backprojector, rows (U, V, W) over h = (ix-cx, iy-cy, iz-cz, 1),
+// scaled so that W = 1 at the volume centre.
+//   reference: the reference's frame (backward_3d.hpp:79-94): dir = O + ix dx + iy dy + iz dz
+//              - s, t = (DSO - DSD - s.z) / dir.z, u = dir.x t + s.x + floor(U/2) (likewise
+//              v); the lower-tap coordinate is u - 0.5.
+//   matched:   the forward frame: voxel P = src + t (org + iu du + iv dv - src) solved for
+//              (t, t iu, t iv); the lower-tap coordinate of pixel centre iu is iu.
+void bp_matrices(const cone3d::Desc &g, const double *phi, const double *theta, const double *psi,
+                 int64_t A, std::vector<float> &mref, std::vector<float> &mmat) {
+    mref.assign(12 * A, 0.0f);
+    mmat.assign(12 * A, 0.0f);
+    const double c[3] = {((double)g.nx - 1.0) / 2.0, ((double)g.ny - 1.0) / 2.0,
+                         ((double)g.nz - 1.0) / 2.0};
+    auto finish = [&](double rows[3][4], float *out) {
+        for (int r = 0; r < 3; ++r)
+            rows[r][3] += rows[r][0] * c[0] + rows[r][1] * c[1] + rows[r][2] * c[2];
+        const double w = rows[2][3];
+        const double sc = (w != 0.0 && std::isfinite(w)) ? 1.0 / w : 1.0;
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 4; ++k) out[4 * r + k] = (float)(rows[r][k] * sc);
+    };
+    const double hu = (double)(g.nu / 2), hv = (double)(g.nv / 2);
+    for (int64_t a = 0; a < A; ++a) {
+        const cone3d::BpGeom bg = cone3d::bp_angle(g, phi[a], theta[a], psi[a]);
+        const double cc = (g.dso - g.dsd) - bg.src[2];
+        double d0[3];
+        for (int d = 0; d < 3; ++d) d0[d] = bg.org[d] - bg.src[d];
+        double rr[3][4];
+        const double rowW[4] = {bg.dx[2], bg.dy[2], bg.dz[2], d0[2]};
+        const double rowX[4] = {bg.dx[0], bg.dy[0], bg.dz[0], d0[0]};
+        const double rowY[4] = {bg.dx[1], bg.dy[1], bg.dz[1], d0[1]};
+        for (int k = 0; k < 4; ++k) {
+            rr[0][k] = cc * rowX[k] + (bg.src[0] + hu - 0.5) * rowW[k];
+            rr[1][k] = cc * rowY[k] + (bg.src[1] + hv - 0.5) * rowW[k];
+            rr[2][k] = rowW[k];
+        }
+        finish(rr, mref.data() + 12 * a);
+
+        const FwdGeom fg = cone3d::fwd_angle(g, phi[a], theta[a], psi[a]);
+        double Mc[3][3]; // columns org - src, du, dv
+        for (int d = 0; d < 3; ++d) {
+            Mc[d][0] = fg.org[d] - fg.src[d];
+            Mc[d][1] = fg.du[d];
+            Mc[d][2] = fg.dv[d];
+        }
+        const double det = Mc[0][0] * (Mc[1][1] * Mc[2][2] - Mc[1][2] * Mc[2][1]) -
+                           Mc[0][1] * (Mc[1][0] * Mc[2][2] - Mc[1][2] * Mc[2][0]) +
+                           Mc[0][2] * (Mc[1][0] * Mc[2][1] - Mc[1][1] * Mc[2][0]);
+        double inv[3][3];
+        inv[0][0] = (Mc[1][1] * Mc[2][2] - Mc[1][2] * Mc[2][1]) / det;
+        inv[0][1] = (Mc[0][2] * Mc[2][1] - Mc[0][1] * Mc[2][2]) / det;
+        inv[0][2] = (Mc[0][1] * Mc[1][2] - Mc[0][2] * Mc[1][1]) / det;
+        inv[1][0] = (Mc[1][2] * Mc[2][0] - Mc[1][0] * Mc[2][2]) / det;
+        inv[1][1] = (Mc[0][0] * Mc[2][2] - Mc[0][2] * Mc[2][0]) / det;
+        inv[1][2] = (Mc[0][2] * Mc[1][0] - Mc[0][0] * Mc[1][2]) / det;
+        inv[2][0] = (Mc[1][0] * Mc[2][1] - Mc[1][1] * Mc[2][0]) / det;
+        inv[2][1] = (Mc[0][1] * Mc[2][0] - Mc[0][0] * Mc[2][1]) / det;
+        inv[2][2] = (Mc[0][0] * Mc[1][1] - Mc[0][1] * Mc[1][0]) / det;
+        // rows ordered (t iu, t iv, t) = (U, V, W)
+        const int order[3] = {1, 2, 0};
+        for (int r = 0; r < 3; ++r) {
+            const double *ir = inv[order[r]];
+            rr[r][0] = ir[0];
+            rr[r][1] = ir[1];
+            rr[r][2] = ir[2];
+            rr[r][3] = -(ir[0] * fg.src[0] + ir[1] * fg.src[1] + ir[2] * fg.src[2]);
+        }
+        finish(rr, mmat.data() + 12 * a);
+    }
+}
+
+template <class F>
+cmi_status dispatch2(int dt_a, int dt_b, F &&f) {
+    auto inner = [&](auto ta) -> cmi_status {
+        switch (dt_b) {
+        case CMI_F32: return f(ta, F32{});
+        case CMI_F16: return f(ta, F16{});
+        case CMI_BF16: return f(ta, BF16{});
+        default: return fail(CMI_ERR_DTYPE, "unsupported dtype %d", dt_b);
+        }
+    };
+    switch (dt_a) {
+    case CMI_F32: return inner(F32{});
+    case CMI_F16: return inner(F16{});
+    case CMI_BF16: return inner(BF16{});
+    default: return fail(CMI_ERR_DTYPE, "unsupported dtype %d", dt_a);
+    }
+}
+
+cmi_status check_call(const cmi_plan3d *p, const void *vol, int vol_dtype, const void *sino,
+                      int sino_dtype, uint64_t batch, const char *fn) {
+    if (!p) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: plan is null", fn);
+    if (!vol) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: volume pointer is null", fn);
+    if (!sino) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: sinogram pointer is null", fn);
+    if (batch == 0) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: batch must be >= 1", fn);
+    if (!dtype_ok(vol_dtype) || !dtype_ok(sino_dtype))
+        return fail(CMI_ERR_DTYPE, "%s: unsupported dtype (volume %d, sinogram %d)", fn,
+                    vol_dtype, sino_dtype);
+    if (vol_dtype != p->vol_prec)
+        return fail(CMI_ERR_DTYPE, "%s: plan precision does not match volume precision (%s)",
+                    fn, dtype_name(vol_dtype));
+    if (sino_dtype != p->det_prec)
+        return fail(CMI_ERR_DTYPE,
+                    "%s: plan precision does not match sinogram precision (%s)", fn,
+                    dtype_name(sino_dtype));
+    if (batch * (uint64_t)p->A > 65535 ||
+        batch * (uint64_t)((p->g.nz + kZB - 1) / kZB) > 65535 ||
+        batch * (uint64_t)(p->g.nz + 2 * kPad) > 65535)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "%s: batch too large for one launch", fn);
+    CMI_TRY(check_device_ptr(vol, p->device, "volume"));
+    CMI_TRY(check_device_ptr(sino, p->device, "sinogram"));
+    return CMI_OK;
+}
+
+} // namespace
+
+// ===================================================================== C-ABI ===============
 
 extern "C" {
 
-CMI_API cmi_status cmi_plan3d_create(int, int, int, const cmi_cone3d_desc *, cmi_plan3d **plan) {
+CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
+                                     const cmi_cone3d_desc *d, cmi_plan3d **plan) {
     clear_error();
-    if (plan) *plan = nullptr;
-    return fail(CMI_ERR_NOT_SUPPORTED, "plan3d_create: 3D projectors are not built yet");
+    if (!plan) return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: output pointer is null");
+    *plan = nullptr;
+    if (!d) return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: descriptor is null");
+    if (!dtype_ok(vol_prec) || !dtype_ok(det_prec))
+        return fail(CMI_ERR_DTYPE, "plan3d_create: precision must be f32, f16 or bf16");
+    if (!d->phi || !d->theta || !d->psi)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: null angle array");
+    for (int i = 0; i < 3; ++i)
+        if (d->vol_shape[i] == 0 || d->vol_shape[i] > 4096 || !(d->vol_spacing[i] > 0))
+            return fail(CMI_ERR_INVALID_ARGUMENT,
+                        "plan3d_create: volume shape / spacing out of range (axis %d)", i);
+    for (int i = 0; i < 2; ++i)
+        if (d->det_shape[i] == 0 || d->det_shape[i] > 16384 || !(d->det_spacing[i] > 0))
+            return fail(CMI_ERR_INVALID_ARGUMENT,
+                        "plan3d_create: detector shape / spacing out of range (axis %d)", i);
+    if (d->nangles == 0 || d->nangles > 65535)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: nangles must be in [1, 65535]");
+    const uint64_t padvol = (d->vol_shape[0] + 2 * kPad) * (d->vol_shape[1] + 2 * kPad) *
+                            (d->vol_shape[2] + 2 * kPad);
+    const uint64_t padpan = (d->det_shape[0] + 2 * kPad) * (d->det_shape[1] + 2 * kPad) *
+                            d->nangles;
+    if (padvol * 4 >= (1ull << 31) || padpan * 4 >= (1ull << 31))
+        return fail(CMI_ERR_INVALID_ARGUMENT,
+                    "plan3d_create: volume or projection stack exceeds 2 GiB");
+    if (!std::isfinite(d->DSO) || !std::isfinite(d->DSD) || !std::isfinite(d->COR))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: non-finite distance");
+    for (uint64_t a = 0; a < d->nangles; ++a)
+        if (!std::isfinite(d->phi[a]) || !std::isfinite(d->theta[a]) || !std::isfinite(d->psi[a]))
+            return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: non-finite angle %llu",
+                        (unsigned long long)a);
+
+    auto p = std::make_unique<cmi_plan3d>();
+    p->device = device;
+    p->vol_prec = vol_prec;
+    p->det_prec = det_prec;
+    // Python order (z, y, x) / (v, u) -> (x, y, z) / (u, v), as forward.cu:113-124
+    cone3d::Desc &g = p->g;
+    g.nx = (int64_t)d->vol_shape[2];
+    g.ny = (int64_t)d->vol_shape[1];
+    g.nz = (int64_t)d->vol_shape[0];
+    g.sx = d->vol_spacing[2];
+    g.sy = d->vol_spacing[1];
+    g.sz = d->vol_spacing[0];
+    g.ox = d->vol_offset[2];
+    g.oy = d->vol_offset[1];
+    g.oz = d->vol_offset[0];
+    g.nu = (int64_t)d->det_shape[1];
+    g.nv = (int64_t)d->det_shape[0];
+    g.du = d->det_spacing[1];
+    g.dv = d->det_spacing[0];
+    g.off_u = d->det_offset[1];
+    g.off_v = d->det_offset[0];
+    g.pitch = d->det_rotation[0]; // forward.cu:136-138
+    g.roll = d->det_rotation[1];
+    g.yaw = d->det_rotation[2];
+    g.dso = d->DSO;
+    g.dsd = d->DSD;
+    g.cor = d->COR;
+    p->A = (int64_t)d->nangles;
+    const int64_t A = p->A;
+
+    std::vector<FwdGeom> fgeo(A);
+    for (int64_t a = 0; a < A; ++a) fgeo[a] = cone3d::fwd_angle(g, d->phi[a], d->theta[a], d->psi[a]);
+    std::vector<float> mref, mmat;
+    bp_matrices(g, d->phi, d->theta, d->psi, A, mref, mmat);
+
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(device));
+    struct Cleanup {
+        cmi_plan3d *p;
+        bool armed = true;
+        ~Cleanup() {
+            if (!armed) return;
+            (void)hipFree(p->d_fgeo);
+            (void)hipFree(p->d_mref);
+            (void)hipFree(p->d_mmat);
+        }
+    } cleanup{p.get()};
+    CMI_HIP_TRY(hipMalloc(&p->d_fgeo, sizeof(FwdGeom) * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_mref, sizeof(float) * 12 * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_mmat, sizeof(float) * 12 * A));
+    p->table_bytes = (sizeof(FwdGeom) + 2 * 12 * sizeof(float)) * A;
+    unsigned long long *d_total = nullptr;
+    CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
+    std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
+        d_total, [](unsigned long long *q) { (void)hipFree(q); });
+    hipStream_t s;
+    CMI_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<ihipStream_t, void (*)(hipStream_t)> stream_guard(
+        s, [](hipStream_t q) { (void)hipStreamDestroy(q); });
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_fgeo, fgeo.data(), sizeof(FwdGeom) * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_mref, mref.data(), sizeof(float) * 12 * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_mmat, mmat.data(), sizeof(float) * 12 * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(unsigned long long), s));
+    dim3 blk(256), grd((unsigned)((g.nu + 255) / 256), (unsigned)g.nv, (unsigned)A);
+    hipLaunchKernelGGL(k_count3d, grd, blk, 0, s, p->d_fgeo, (int)g.nu, (int)g.nv, (int)g.nx,
+                       (int)g.ny, (int)g.nz, d_total);
+    CMI_HIP_TRY(hipGetLastError());
+    unsigned long long total = 0;
+    CMI_HIP_TRY(hipMemcpyAsync(&total, d_total, sizeof total, hipMemcpyDeviceToHost, s));
+    CMI_HIP_TRY(hipStreamSynchronize(s));
+    p->total_samples = total;
+    cleanup.armed = false;
+    *plan = p.release();
+    return CMI_OK;
 }
 
 CMI_API cmi_status cmi_plan3d_destroy(cmi_plan3d *plan) {
     clear_error();
+    if (!plan) return CMI_OK;
+    DeviceGuard guard;
+    const cmi_status st = guard.enter(plan->device);
+    if (st == CMI_OK) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(plan->d_fgeo);
+        (void)hipFree(plan->d_mref);
+        (void)hipFree(plan->d_mmat);
+    }
     delete plan;
+    return st;
+}
+
+CMI_API cmi_status cmi_plan3d_get_info(const cmi_plan3d *plan, cmi_plan3d_info *info) {
+    clear_error();
+    if (!plan || !info) return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_get_info: null argument");
+    info->device = plan->device;
+    info->vol_prec = plan->vol_prec;
+    info->det_prec = plan->det_prec;
+    info->nx = (uint64_t)plan->g.nx;
+    info->ny = (uint64_t)plan->g.ny;
+    info->nz = (uint64_t)plan->g.nz;
+    info->nu = (uint64_t)plan->g.nu;
+    info->nv = (uint64_t)plan->g.nv;
+    info->nangles = (uint64_t)plan->A;
+    info->total_samples = plan->total_samples;
+    info->table_bytes = plan->table_bytes;
     return CMI_OK;
 }
 
-CMI_API cmi_status cmi_forward_3d(const cmi_plan3d *, const void *, int, void *, int, uint64_t,
-                                  void *) {
+CMI_API cmi_status cmi_forward_3d(const cmi_plan3d *plan, const void *volume, int vol_dtype,
+                                  void *sinogram, int sino_dtype, uint64_t batch,
+                                  void *stream) {
     clear_error();
-    return fail(CMI_ERR_NOT_SUPPORTED, "forward_3d: not built yet");
+    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "forward_3d"));
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(plan->device));
+    const hipStream_t s = (hipStream_t)stream;
+    const cone3d::Desc &g = plan->g;
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, U = (int)g.nu, V = (int)g.nv,
+              A = (int)plan->A;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
+    void *scratch = nullptr;
+    CMI_HIP_TRY(hipMallocAsync(&scratch, (size_t)Wp * Hp * Dp * dtype_size(vol_dtype) * batch, s));
+    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto si, auto ss) -> cmi_status {
+        using SI = decltype(si);
+        using SS = decltype(ss);
+        using RI = typename SI::raw;
+        dim3 pb(256), pg((unsigned)((Wp + 255) / 256), (unsigned)Hp, (unsigned)(Dp * batch));
+        hipLaunchKernelGGL((k_stage3d<SI>), pg, pb, 0, s, (const RI *)volume, (RI *)scratch, nx,
+                           ny, nz, Wp, Hp, Dp);
+        dim3 fb(16, 16), fg((unsigned)((U + 15) / 16), (unsigned)((V + 15) / 16),
+                            (unsigned)(A * batch));
+        hipLaunchKernelGGL((k_fwd3d<SI, SS>), fg, fb, 0, s, (const RI *)scratch, plan->d_fgeo,
+                           nx, ny, nz, U, V, A, (typename SS::raw *)sinogram);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+    const hipError_t fe = hipFreeAsync(scratch, s);
+    if (st != CMI_OK) return st;
+    CMI_HIP_TRY(fe);
+    return CMI_OK;
 }
 
-CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *, void *, int, const void *, int, uint64_t,
-                                   int, void *) {
+CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol_dtype,
+                                   const void *sinogram, int sino_dtype, uint64_t batch,
+                                   int mode, void *stream) {
     clear_error();
-    return fail(CMI_ERR_NOT_SUPPORTED, "backward_3d: not built yet");
+    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "backward_3d"));
+    if (mode != CMI_BP3D_REFERENCE && mode != CMI_BP3D_MATCHED)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "backward_3d: unknown mode %d", mode);
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(plan->device));
+    const hipStream_t s = (hipStream_t)stream;
+    const cone3d::Desc &g = plan->g;
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, U = (int)g.nu, V = (int)g.nv,
+              A = (int)plan->A;
+    const int Up = U + 2 * kPad, Vp = V + 2 * kPad;
+    void *scratch = nullptr;
+    CMI_HIP_TRY(
+        hipMallocAsync(&scratch, (size_t)Up * Vp * A * dtype_size(sino_dtype) * batch, s));
+    const float cx = (float)((nx - 1) / 2.0), cy = (float)((ny - 1) / 2.0),
+                cz = (float)((nz - 1) / 2.0);
+    const float *M = mode == CMI_BP3D_MATCHED ? plan->d_mmat : plan->d_mref;
+    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
+        using SV = decltype(sv);
+        using SS = decltype(ss);
+        using RS = typename SS::raw;
+        dim3 pb(256), pg((unsigned)((Up + 255) / 256), (unsigned)Vp, (unsigned)(A * batch));
+        hipLaunchKernelGGL((k_stage_panel<SS>), pg, pb, 0, s, (const RS *)sinogram,
+                           (RS *)scratch, U, V, A);
+        const int nzb = (nz + kZB - 1) / kZB;
+        dim3 bb(16, 16), bg((unsigned)((nx + 15) / 16), (unsigned)((ny + 15) / 16),
+                            (unsigned)(nzb * batch));
+        if (mode == CMI_BP3D_MATCHED)
+            hipLaunchKernelGGL((k_bp3d<SV, SS, true>), bg, bb, 0, s, (typename SV::raw *)volume,
+                               (const RS *)scratch, M, nx, ny, nz, U, V, A, cx, cy, cz);
+        else
+            hipLaunchKernelGGL((k_bp3d<SV, SS, false>), bg, bb, 0, s, (typename SV::raw *)volume,
+                               (const RS *)scratch, M, nx, ny, nz, U, V, A, cx, cy, cz);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+    const hipError_t fe = hipFreeAsync(scratch, s);
+    if (st != CMI_OK) return st;
+    CMI_HIP_TRY(fe);
+    return CMI_OK;
 }
 
-CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *, void *, int, const void *, int, uint64_t,
-                                  void *) {
+CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_dtype,
+                                  const void *sinogram, int sino_dtype, uint64_t batch,
+                                  void *stream) {
     clear_error();
-    return fail(CMI_ERR_NOT_SUPPORTED, "adjoint_3d: not built yet");
+    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "adjoint_3d"));
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(plan->device));
+    const hipStream_t s = (hipStream_t)stream;
+    const cone3d::Desc &g = plan->g;
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, U = (int)g.nu, V = (int)g.nv,
+              A = (int)plan->A;
+    const size_t nvox = (size_t)nx * ny * nz * batch;
+    float *acc = nullptr;
+    if (vol_dtype != CMI_F32) CMI_HIP_TRY(hipMallocAsync((void **)&acc, nvox * sizeof(float), s));
+    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
+        using SV = decltype(sv);
+        using SS = decltype(ss);
+        float *target = acc ? acc : (float *)volume;
+        const unsigned nb = (unsigned)((nvox + 255) / 256);
+        if (acc) hipLaunchKernelGGL(k_fill_zero, dim3(nb), dim3(256), 0, s, acc, nvox);
+        dim3 fb(16, 16), fg((unsigned)((U + 15) / 16), (unsigned)((V + 15) / 16),
+                            (unsigned)(A * batch));
+        hipLaunchKernelGGL((k_adj3d<SS>), fg, fb, 0, s, target,
+                           (const typename SS::raw *)sinogram, plan->d_fgeo, nx, ny, nz, U, V, A);
+        if (acc)
+            hipLaunchKernelGGL((k_add_into<SV>), dim3(nb), dim3(256), 0, s,
+                               (typename SV::raw *)volume, (const float *)acc, nvox);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+    if (acc) {
+        const hipError_t fe = hipFreeAsync(acc, s);
+        if (st != CMI_OK) return st;
+        CMI_HIP_TRY(fe);
+    }
+    return st;
 }
 
 } // extern "C"

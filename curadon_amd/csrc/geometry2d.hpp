@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdint>
 
+#pragma float_control(push)
 #pragma clang fp contract(off)
 
 namespace cmi::fan2d {
@@ -100,4 +101,4 @@ __host__ __device__ inline Ray setup_ray(const AngleGeom &ag, int64_t nx, int64_
 
 } // namespace cmi::fan2d
 
-#pragma clang fp contract(on)
+#pragma float_control(pop)

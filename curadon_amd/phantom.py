@@ -87,17 +87,26 @@ def _rot_zyz(al, be, ga):
 
 def ellipsoids_3d(n: int, seed: int = 0, count: int = 20, params=None,
                   dtype=np.float32) -> np.ndarray:
-    """(n, n, n) volume in (z, y, x) order; voxel centres at index - n/2 + .5."""
+    """(n, n, n) volume in (z, y, x) order; voxel centres at index - n/2 + .5. Each shape is
+    evaluated inside its bounding box only."""
     p = ellipsoid_params(n, seed, count) if params is None else params
     c = np.arange(n, dtype=np.float64) - n / 2.0 + 0.5
     vol = np.zeros((n, n, n), dtype=dtype)
-    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
     for cx, cy, cz, a, b, cc, al, be, ga, rho in p:
         R = _rot_zyz(al, be, ga)
+        r = max(a, b, cc)
+        sl = []
+        for ctr in (cz, cy, cx):
+            lo = int(np.clip(np.floor(ctr - r + n / 2.0 - 0.5), 0, n))
+            hi = int(np.clip(np.ceil(ctr + r + n / 2.0 - 0.5) + 1, 0, n))
+            sl.append(slice(lo, hi))
+        if any(s.stop <= s.start for s in sl):
+            continue
+        Z, Y, X = np.meshgrid(c[sl[0]], c[sl[1]], c[sl[2]], indexing="ij")
         d = np.stack([X - cx, Y - cy, Z - cz], axis=-1)
         q = d @ R  # coordinates in the ellipsoid frame
         inside = (q[..., 0] / a) ** 2 + (q[..., 1] / b) ** 2 + (q[..., 2] / cc) ** 2 <= 1.0
-        vol[inside] += dtype(rho) if isinstance(dtype, type) else rho
+        vol[sl[0], sl[1], sl[2]][inside] += rho
     return vol
 
 
@@ -123,4 +132,23 @@ def ellipse_line_integrals(params: np.ndarray, p0: np.ndarray, d: np.ndarray) ->
         disc = B * B - 4 * A * Cc
         chord = np.where(disc > 0, np.sqrt(np.maximum(disc, 0)) / A, 0.0)
         out += rho * chord
+    return out
+
+
+def ellipsoid_line_integrals(params: np.ndarray, p0: np.ndarray, d: np.ndarray) -> np.ndarray:
+    """Exact line integrals (over arc length) of the ellipsoid phantom along rays p0 + t d,
+    world coordinates (x, y, z). p0, d: (..., 3)."""
+    p0 = np.asarray(p0, dtype=np.float64)
+    d = np.asarray(d, dtype=np.float64)
+    dn = d / np.linalg.norm(d, axis=-1, keepdims=True)
+    out = np.zeros(p0.shape[:-1])
+    for cx, cy, cz, a, b, c, al, be, ga, rho in params:
+        R = _rot_zyz(al, be, ga)
+        q = (p0 - np.array([cx, cy, cz])) @ R / np.array([a, b, c])
+        e = dn @ R / np.array([a, b, c])
+        A = (e * e).sum(-1)
+        B = 2 * (q * e).sum(-1)
+        Cc = (q * q).sum(-1) - 1
+        disc = B * B - 4 * A * Cc
+        out += rho * np.where(disc > 0, np.sqrt(np.maximum(disc, 0)) / A, 0.0)
     return out

@@ -169,6 +169,17 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
                                      const cmi_cone3d_desc *desc, cmi_plan3d **plan);
 CMI_API cmi_status cmi_plan3d_destroy(cmi_plan3d *plan);
 
+/* Shape queries. total_samples = sum over rays that hit the clip box of the sample count
+ * nst + 1 of the forward march. */
+typedef struct cmi_plan3d_info {
+    int device;
+    int vol_prec, det_prec;
+    uint64_t nx, ny, nz, nu, nv, nangles;
+    uint64_t total_samples;
+    uint64_t table_bytes;
+} cmi_plan3d_info;
+CMI_API cmi_status cmi_plan3d_get_info(const cmi_plan3d *plan, cmi_plan3d_info *info);
+
 /*
  * Cone-beam forward projection, sinogram := A volume (unscaled sum of trilinear samples, as
  * forward_3d.hpp:87-93). Replaces curadon_ext.forward_3d (forward.cu:89-141 ->
@@ -189,8 +200,10 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
                                    const void *sinogram, int sino_dtype, uint64_t batch,
                                    int mode, void *stream);
 
-/* Matched adjoint of cmi_forward_3d, volume += A^T sinogram (exact transpose, deterministic
- * gather form). No counterpart in the reference. */
+/* Matched adjoint of cmi_forward_3d, volume += A^T sinogram: every forward sample scatters
+ * its trilinear weights times the ray's value, at the forward march's own positions. The sums
+ * use float atomics, so the last bits depend on the order in which they land. No counterpart
+ * in the reference. */
 CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_dtype,
                                   const void *sinogram, int sino_dtype, uint64_t batch,
                                   void *stream);

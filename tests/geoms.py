@@ -42,3 +42,37 @@ def as_fan_geometry(rec, **kw):
                           vol_shape=rec.vol_shape, det_shape=rec.det_shape,
                           det_spacing=rec.det_spacing, det_offset=rec.det_offset,
                           vol_spacing=rec.vol_spacing, vol_offset=rec.vol_offset, **kw)
+
+
+def cone(n, U, V, A, dso, dsd, du=(1.0, 1.0), angles=None, vol_spacing=(1.0, 1.0, 1.0),
+         vol_offset=(0.0, 0.0, 0.0), det_offset=(0.0, 0.0), det_rotation=(0.0, 0.0, 0.0), cor=0.0,
+         nz=None, ny=None):
+    """Cone-beam record in the Python axis order of ConeGeometry: vol_shape (z, y, x),
+    det_shape (v, u), angles (3, A) = (phi, theta, psi)."""
+    nz = n if nz is None else nz
+    ny = n if ny is None else ny
+    if angles is None:
+        angles = np.linspace(0, 2 * np.pi, A, endpoint=False)
+    ang = np.asarray(angles, float)
+    if ang.ndim == 1:
+        ang = np.vstack([ang, np.zeros_like(ang), np.zeros_like(ang)])
+    return SimpleNamespace(
+        vol_shape=np.array([nz, ny, n]), vol_spacing=np.array(vol_spacing, float),
+        vol_offset=np.array(vol_offset, float), det_shape=np.array([V, U]),
+        det_spacing=np.array(du, float), det_offset=np.array(det_offset, float),
+        det_rotation=np.array(det_rotation, float), angles=ang, DSO=float(dso), DSD=float(dsd),
+        COR=float(cor), nangles=ang.shape[1])
+
+
+def c3():
+    """256^3 volume, 384 x 384 panel, 360 angles, DSO 512, DSD 768."""
+    return cone(256, 384, 384, 360, 512.0, 768.0)
+
+
+def as_cone_geometry(rec, **kw):
+    import curadon_amd as cm
+    return cm.ConeGeometry(DSD=rec.DSD, DSO=rec.DSO, angles=rec.angles, vol_shape=rec.vol_shape,
+                           det_shape=rec.det_shape, det_spacing=rec.det_spacing,
+                           det_offset=rec.det_offset, det_rotation=rec.det_rotation,
+                           vol_spacing=rec.vol_spacing, vol_offset=rec.vol_offset, COR=rec.COR,
+                           **kw)

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-from geoms import tiny  # noqa: E402
+from geoms import cone, tiny  # noqa: E402
 from oracle import oracle  # noqa: E402
 
 
@@ -31,6 +31,13 @@ def main():
              sino=oracle.fwd2d(g, image), sino_in=sino_in, bp=oracle.bp2d(g, sino_in),
              adj=oracle.adj2d(g, sino_in))
     print("wrote fan2d_tiny.npz")
+    c = cone(8, 6, 5, 8, 16.0, 28.0)
+    vol = r.random((8, 8, 8))
+    proj_in = r.random((8, 5, 6))
+    np.savez(os.path.join(HERE, "cone3d_tiny.npz"), angles=c.angles, volume=vol,
+             sino=oracle.fwd3d(c, vol), sino_in=proj_in, bp=oracle.bp3d(c, proj_in),
+             bp_matched=oracle.bp3d(c, proj_in, matched=True), adj=oracle.adj3d(c, proj_in))
+    print("wrote cone3d_tiny.npz")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,203 @@
+"""GPU parity of the 3D cone-beam HIP kernels against the FP64 oracle, through the C-ABI.
+
+Tolerances (FP32 storage and arithmetic vs FP64):
+  forward                  rel-L2 <= 1e-5, max-abs <= 2e-5 * max|ref|
+  backprojection (both)    rel-L2 <= 1e-5, max-abs <= 2e-5 * max|ref|
+  adjoint                  rel-L2 <= 1e-5, max-abs <= 2e-5 * max|ref|
+  dot test                 |<Ax,y> - <x,A^T y>| / |<Ax,y>| <= 1e-5
+At the 256^3 workload the oracle runs on projection / slab subsets.
+"""
+import numpy as np
+import pytest
+import torch
+
+import curadon_amd as cm
+from curadon_amd import phantom
+from geoms import as_cone_geometry, c3, cone
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def check_close(got, ref, what, rtol=1e-5, maxabs=2e-5):
+    r = rel_l2(got, ref)
+    m = float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
+    print(f"{what}: rel-L2 {r:.3e}  max-abs/max {m:.3e}", flush=True)
+    assert r <= rtol, (what, r)
+    assert m <= maxabs, (what, m)
+
+
+def dev(x, gpu, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(x)).to(device=gpu, dtype=dtype)
+
+
+def host(t):
+    return t.double().cpu().numpy()
+
+
+SMALL = dict(n=32, U=48, V=40, A=24, dso=64.0, dsd=96.0)
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    rec = cone(SMALL["n"], SMALL["U"], SMALL["V"], SMALL["A"], SMALL["dso"], SMALL["dsd"])
+    return rec, as_cone_geometry(rec)
+
+
+@pytest.fixture(scope="module")
+def general(gpu):
+    """Non-circular trajectory, rotated and offset panel, anisotropic voxels, COR shift."""
+    r = np.random.default_rng(0)
+    ang = np.vstack([r.uniform(0, 2 * np.pi, 10), r.uniform(-0.3, 0.3, 10),
+                     r.uniform(-0.3, 0.3, 10)])
+    rec = cone(24, 36, 30, 10, 60.0, 100.0, du=(1.2, 0.9), angles=ang,
+               vol_spacing=(1.1, 0.9, 1.0), vol_offset=(0.5, -0.7, 0.3),
+               det_offset=(1.5, -2.0), det_rotation=(0.04, -0.06, 0.03), cor=0.8, nz=20, ny=22)
+    return rec, as_cone_geometry(rec)
+
+
+def test_small_forward(oracle, small, gpu):
+    rec, geom = small
+    vol = phantom.ellipsoids_3d(32, seed=1, dtype=np.float64)
+    ref = oracle.fwd3d(rec, vol)
+    check_close(host(cm.forward(dev(vol, gpu), geom)), ref, "small fwd3d")
+
+
+def test_small_backward_reference(oracle, small, gpu):
+    rec, geom = small
+    y = np.random.default_rng(2).random((24, 40, 48))
+    ref = oracle.bp3d(rec, y)
+    check_close(host(cm.backward(dev(y, gpu), geom)), ref, "small bp3d reference")
+
+
+def test_small_backward_matched(oracle, small, gpu):
+    rec, geom = small
+    y = np.random.default_rng(3).random((24, 40, 48))
+    ref = oracle.bp3d(rec, y, matched=True)
+    check_close(host(cm.backward(dev(y, gpu), geom, mode="matched")), ref, "small bp3d matched")
+
+
+def test_small_adjoint_and_dot(oracle, small, gpu):
+    rec, geom = small
+    y = np.random.default_rng(4).random((24, 40, 48))
+    ref = oracle.adj3d(rec, y)
+    got = cm.adjoint(dev(y, gpu), geom)
+    check_close(host(got), ref, "small adj3d")
+    x = torch.rand(32, 32, 32, device=gpu, generator=torch.Generator(device=gpu).manual_seed(5))
+    lhs = torch.vdot(cm.forward(x, geom).double().ravel(), dev(y, gpu).double().ravel())
+    rhs = torch.vdot(x.double().ravel(), got.double().ravel())
+    r = abs(float(lhs - rhs)) / abs(float(lhs))
+    print(f"small dot test: {r:.3e}")
+    assert r <= 1e-5
+
+
+def test_general_geometry(oracle, general, gpu):
+    rec, geom = general
+    vol = np.random.default_rng(6).random((20, 22, 24))
+    check_close(host(cm.forward(dev(vol, gpu), geom)), oracle.fwd3d(rec, vol), "general fwd3d")
+    y = np.random.default_rng(7).random((10, 30, 36))
+    check_close(host(cm.backward(dev(y, gpu), geom)), oracle.bp3d(rec, y), "general bp3d ref")
+    check_close(host(cm.backward(dev(y, gpu), geom, mode="matched")),
+                oracle.bp3d(rec, y, matched=True), "general bp3d matched")
+    check_close(host(cm.adjoint(dev(y, gpu), geom)), oracle.adj3d(rec, y), "general adj3d")
+
+
+def test_tiny_golden_3d(gpu):
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "cone3d_tiny.npz"))
+    geom = as_cone_geometry(cone(8, 6, 5, 8, 16.0, 28.0))
+    check_close(host(cm.forward(dev(d["volume"], gpu), geom)), d["sino"], "tiny fwd3d")
+    check_close(host(cm.backward(dev(d["sino_in"], gpu), geom)), d["bp"], "tiny bp3d")
+    check_close(host(cm.backward(dev(d["sino_in"], gpu), geom, mode="matched")),
+                d["bp_matched"], "tiny bp3d matched")
+    check_close(host(cm.adjoint(dev(d["sino_in"], gpu), geom)), d["adj"], "tiny adj3d")
+
+
+def test_batched_equals_single_3d(small, gpu):
+    rec, geom = small
+    xs = torch.rand(2, 32, 32, 32, device=gpu)
+    sb = cm.forward(xs, geom)
+    ys = torch.rand(2, 24, 40, 48, device=gpu)
+    vb = cm.backward(ys, geom)
+    for i in range(2):
+        assert torch.equal(sb[i], cm.forward(xs[i].contiguous(), geom))
+        assert torch.equal(vb[i], cm.backward(ys[i].contiguous(), geom))
+
+
+def test_reference_binding_signature(oracle, small, gpu):
+    """curadon_ext.forward_3d / backward_3d with the reference's 15 positional arguments."""
+    rec, geom = small
+    vol = np.random.default_rng(8).random((32, 32, 32))
+    sino = torch.zeros(24, 40, 48, device=gpu)
+    cm.forward_3d(dev(vol, gpu), geom.vol_shape, geom.vol_spacing, geom.vol_offset, sino,
+                  geom.angles[0], geom.angles[1], geom.angles[2], geom.det_shape,
+                  geom.det_spacing, geom.det_offset, geom.det_rotation, geom.DSO, geom.DSD,
+                  geom.COR)
+    check_close(host(sino), oracle.fwd3d(rec, vol), "forward_3d binding")
+    out = torch.zeros(32, 32, 32, device=gpu)
+    cm.backward_3d(out, geom.vol_shape, geom.vol_spacing, geom.vol_offset, sino, geom.angles[0],
+                   geom.angles[1], geom.angles[2], geom.det_shape, geom.det_spacing,
+                   geom.det_offset, geom.det_rotation, geom.DSO, geom.DSD, geom.COR)
+    check_close(host(out), oracle.bp3d(rec, host(sino)), "backward_3d binding")
+
+
+@pytest.mark.parametrize("prec,tol", [(torch.float16, 2e-3), (torch.bfloat16, 1.5e-2)])
+def test_half_storage_3d(oracle, small, gpu, prec, tol):
+    rec, _ = small
+    geom = as_cone_geometry(rec, vol_prec=prec, sino_prec=prec)
+    vol = phantom.ellipsoids_3d(32, seed=1, dtype=np.float64)
+    r = rel_l2(host(cm.forward(dev(vol, gpu, prec), geom)), oracle.fwd3d(rec, vol))
+    y = np.random.default_rng(9).random((24, 40, 48))
+    rb = rel_l2(host(cm.backward(dev(y, gpu, prec), geom)), oracle.bp3d(rec, y))
+    ra = rel_l2(host(cm.adjoint(dev(y, gpu, prec), geom)), oracle.adj3d(rec, y))
+    print(f"3d {prec}: fwd {r:.3e} bp {rb:.3e} adj {ra:.3e}")
+    assert max(r, rb, ra) <= tol
+
+
+@pytest.fixture(scope="module")
+def big(gpu):
+    rec = c3()
+    geom = as_cone_geometry(rec)
+    vol = phantom.ellipsoids_3d(256, seed=0)
+    return rec, geom, vol
+
+
+def test_c3_forward_subset(oracle, big, gpu):
+    rec, geom, vol = big
+    out = host(cm.forward(dev(vol, gpu), geom))
+    got, ref = [], []
+    for a0 in (0, 45, 90, 200):
+        r = oracle.fwd3d(rec, vol.astype(np.float64), angles=(a0, a0 + 1))
+        got.append(out[a0])
+        ref.append(r[a0])
+    check_close(np.stack(got), np.stack(ref), "c3 fwd3d (4 projections)")
+
+
+def test_c3_backward_subset(oracle, big, gpu):
+    rec, geom, vol = big
+    y = np.random.default_rng(10).random((360, 384, 384)).astype(np.float32)
+    yt = dev(y, gpu)
+    for mode in ("reference", "matched"):
+        out = host(cm.backward(yt, geom, mode=mode))
+        got, ref = [], []
+        for z0 in (0, 128, 255):
+            r = oracle.bp3d(rec, y.astype(np.float64), slabs=(z0, z0 + 1),
+                            matched=mode == "matched")
+            got.append(out[z0])
+            ref.append(r[z0])
+        check_close(np.stack(got), np.stack(ref), f"c3 bp3d {mode} (3 slices)")
+
+
+def test_c3_dot_product(big, gpu):
+    rec, geom, _ = big
+    g = torch.Generator(device=gpu).manual_seed(11)
+    x = torch.rand(256, 256, 256, device=gpu, generator=g)
+    y = torch.rand(360, 384, 384, device=gpu, generator=g)
+    lhs = torch.vdot(cm.forward(x, geom).double().ravel(), y.double().ravel())
+    rhs = torch.vdot(x.double().ravel(), cm.adjoint(y, geom).double().ravel())
+    r = abs(float(lhs - rhs)) / abs(float(lhs))
+    print(f"c3 dot test: {r:.3e}")
+    assert r <= 1e-5
