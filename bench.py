@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fan2d_1024.json")
 
 WORKLOAD = dict(n=1024, D=1449, A=720, DSO=2048.0, DSD=3072.0)
 
@@ -53,6 +54,81 @@ def algorithmic_bytes(info, w, s=4):
     return fwd, bwd
 
 
+def time_ops(torch, stream, fns, reps):
+    """Mean ms per call of each callable, HIP events on the launch stream."""
+    out = {}
+    for name, fn in fns:
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        out[name] = a.elapsed_time(b) / reps
+    return out
+
+
+def cone_case(torch, cm, phantom, np, stream, reps):
+    """256^3 FP32 volume of 20 ellipsoids, 384 x 384 panel, 360 angles, DSO 512, DSD 768."""
+    angles = np.linspace(0, 2 * np.pi, 360, endpoint=False)
+    geom = cm.ConeGeometry(DSD=768.0, DSO=512.0, angles=angles, vol_shape=(256, 256, 256),
+                           det_shape=(384, 384))
+    vol = torch.tensor(phantom.ellipsoids_3d(256, seed=0), device="cuda:0")
+    sino = torch.empty(360, 384, 384, device="cuda:0")
+    bvol = torch.zeros_like(vol)
+    info = geom.plan.info()
+    t = time_ops(torch, stream, [
+        ("fwd3d", lambda: cm.forward(vol, sino, geom)),
+        ("bp3d", lambda: cm.backward(sino, bvol, geom)),
+        ("bp3d_matched", lambda: cm.backward(sino, bvol, geom, mode="matched")),
+        ("adj3d", lambda: cm.adjoint(sino, bvol, geom)),
+    ], reps)
+    n3 = 256 ** 3
+    fwd_b = info["total_samples"] * 8 * 4 + 360 * 384 * 384 * 4
+    bp_b = 360 * n3 * 4 * 4 + 2 * n3 * 4
+    return {
+        "workload": "cone3d 256^3 f32, 384x384 panel, 360 angles, DSO 512, DSD 768",
+        "total_samples": info["total_samples"],
+        "fwd3d": {"ms": t["fwd3d"], "alg_GB_per_s": fwd_b / t["fwd3d"] / 1e6},
+        "bp3d": {"ms": t["bp3d"], "alg_GB_per_s": bp_b / t["bp3d"] / 1e6},
+        "bp3d_matched": {"ms": t["bp3d_matched"], "alg_GB_per_s": bp_b / t["bp3d_matched"] / 1e6},
+        "adj3d": {"ms": t["adj3d"]},
+        "fwd_bwd_pairs_per_s": 1e3 / (t["fwd3d"] + t["bp3d"]),
+    }
+
+
+def batch_case(torch, cm, phantom, np, stream, reps):
+    """16 images of 512^2 FP32, 724 cells, 512 angles, DSO 1024, DSD 1536, one batched call."""
+    angles = np.linspace(0, 2 * np.pi, 512, endpoint=False)
+    geom = cm.FanGeometry(DSD=1536.0, DSO=1024.0, angles=angles, vol_shape=(512, 512),
+                          det_shape=724)
+    x = torch.stack([torch.tensor(phantom.ellipses_2d(512, seed=s), dtype=torch.float32)
+                     for s in range(16)]).to("cuda:0")
+    sino = torch.empty(16, 512, 724, device="cuda:0")
+    g = torch.zeros_like(x)
+    t = time_ops(torch, stream, [
+        ("fwd", lambda: cm.forward(x, sino, geom)),
+        ("adj", lambda: cm.adjoint(sino, g, geom)),
+        ("bwd", lambda: cm.backward(sino, g, geom)),
+    ], reps)
+    return {"workload": "fan2d batch 16 x 512^2 f32, 724 cells, 512 angles (one call)",
+            "fwd_ms": t["fwd"], "adj_ms": t["adj"], "bwd_ms": t["bwd"],
+            "fwd_adj_images_per_s": 16e3 / (t["fwd"] + t["adj"])}
+
+
+def copy_calibration(torch, mib, reps):
+    """Plain device-to-device copies of `mib` MiB, for scaling HBM byte counters."""
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"bench: {reps} copies of {mib} MiB done")
+
+
 def cpu_baseline(np, geom_rec):
     """Times the oracle's FP32 port (OpenMP) on a bounded sample: forward on 144 of 720
     angles and backprojection on 256 of 1024 image rows, scaled to whole operators."""
@@ -79,6 +155,16 @@ def cpu_baseline(np, geom_rec):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -95,6 +181,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary workloads (256^3 cone beam, batched 512^2)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run plain device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -107,6 +197,10 @@ def main():
 
     import curadon_amd as cm
     from curadon_amd import phantom
+
+    if args.copy_calibration:
+        copy_calibration(torch, args.copy_calibration, max(1, args.steps))
+        return
 
     log("bench: building plan and inputs")
     geom, img, sino, vol = build_case(torch, cm, phantom, np)
@@ -156,6 +250,13 @@ def main():
     adj_ms = ea.elapsed_time(eb) / n_adj
     log(f"bench: adjoint {adj_ms:.3f} ms")
 
+    extra = None
+    if not args.no_extra:
+        log("bench: secondary workloads")
+        extra = {"cone3d": cone_case(torch, cm, phantom, np, stream, 3),
+                 "batch2d": batch_case(torch, cm, phantom, np, stream, 3)}
+        log(f"bench: cone3d {extra['cone3d']['fwd3d']['ms']:.2f} / {extra['cone3d']['bp3d']['ms']:.2f} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle FP32 port)")
@@ -203,12 +304,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": dominant, "achieved": dom_gbs, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
             "alg_bytes_per_launch": dom_bytes,
             "note": "achieved = algorithmic bytes (every interpolation tap counted as an HBM "
                     "read) / mean op time from HIP events on the launch stream",
         },
         "cpu_baseline": cpu,
+        "extra": extra,
         "device": {"name": props.name, "cus": props.multi_processor_count,
                    "gcn_arch": getattr(props, "gcnArchName", "")},
     }
