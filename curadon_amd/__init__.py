@@ -2,7 +2,8 @@
 
 Drop-in names (python/standalone/src/curadon/__init__.py:1-15): FanGeometry, ConeGeometry,
 forward, backward, TorchForward, TorchBackprojection, plan_2d, forward_2d, backward_2d,
-forward_3d, backward_3d. Additions: adjoint / adjoint_2d / TorchAdjoint (matched transpose).
+forward_3d, backward_3d. Additions: adjoint / adjoint_2d / TorchAdjoint (matched transpose),
+fbp / fbp_filter (filtered backprojection, FDK for cone beam).
 Usage: ``import curadon_amd as curadon``.
 """
 from .geometry import ConeGeometry, FanGeometry
@@ -12,9 +13,10 @@ from ._lib import (CuradonError, adjoint_2d, backward_2d, backward_3d, forward_2
 from .forward import forward
 from .backward import adjoint, backward
 from .diffable import TorchAdjoint, TorchBackprojection, TorchForward
+from .reconstruction import fbp, fbp_filter
 
 __all__ = [
     "FanGeometry", "ConeGeometry", "forward", "backward", "adjoint", "TorchForward",
     "TorchBackprojection", "TorchAdjoint", "plan_2d", "forward_2d", "backward_2d",
-    "adjoint_2d", "forward_3d", "backward_3d", "plan_3d", "CuradonError",
+    "adjoint_2d", "forward_3d", "backward_3d", "plan_3d", "CuradonError", "fbp", "fbp_filter",
 ]

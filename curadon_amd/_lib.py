@@ -41,7 +41,8 @@ ERR_OUT_OF_MEMORY = 5
 ERR_NOT_SUPPORTED = 6
 
 F16, F32, BF16 = 16, 32, 1016
-BP3D_REFERENCE, BP3D_MATCHED = 0, 1
+BP3D_REFERENCE, BP3D_MATCHED, BP3D_FDK = 0, 1, 2
+BP2D_REFERENCE, BP2D_FBP = 0, 1
 
 _DTYPE_CODE = {torch.float32: F32, torch.float16: F16, torch.bfloat16: BF16}
 _CODE_DTYPE = {v: k for k, v in _DTYPE_CODE.items()}
@@ -104,6 +105,7 @@ SIGNATURES = {
     "cmi_plan2d_get_info": (_i32, [_vp, C.POINTER(Plan2DInfo)]),
     "cmi_forward_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_backward_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_backward_2d_weighted": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_adjoint_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_plan3d_create": (_i32, [_i32, _i32, _i32, C.POINTER(Cone3DDesc), C.POINTER(_vp)]),
     "cmi_plan3d_destroy": (_i32, [_vp]),
@@ -314,13 +316,17 @@ def forward_2d(volume, sinogram, plan):
                               stream_of(volume)))
 
 
-def backward_2d(volume, sinogram, plan):
-    """volume += B sinogram, the reference's voxel-driven backprojection."""
+def backward_2d(volume, sinogram, plan, weighting="reference"):
+    """volume += B sinogram, the reference's voxel-driven backprojection; weighting="fbp"
+    squares the magnification weight (fan-beam filtered backprojection)."""
+    w = {"reference": BP2D_REFERENCE, "fbp": BP2D_FBP}.get(weighting)
+    if w is None:
+        raise ValueError(f"unknown weighting {weighting!r}")
     vc, sc, b = _check_pair_2d(volume, sinogram, plan, "backward_2d")
     if b == 0:
         return
-    check(_lib.cmi_backward_2d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc, b,
-                               stream_of(volume)))
+    check(_lib.cmi_backward_2d_weighted(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(),
+                                        sc, b, w, stream_of(volume)))
 
 
 def adjoint_2d(volume, sinogram, plan):
@@ -428,8 +434,9 @@ def forward_3d_plan(volume, sinogram, plan):
 
 
 def backward_3d_plan(volume, sinogram, plan, mode="reference"):
-    """volume += B sinogram; mode 'reference' (the reference's projection) or 'matched'."""
-    m = {"reference": BP3D_REFERENCE, "matched": BP3D_MATCHED}.get(mode)
+    """volume += B sinogram; mode 'reference' (the reference's projection), 'matched' (the
+    forward model's projection) or 'fdk' (matched, weighted by (DSO / depth)^2)."""
+    m = {"reference": BP3D_REFERENCE, "matched": BP3D_MATCHED, "fdk": BP3D_FDK}.get(mode)
     if m is None:
         raise ValueError(f"unknown backprojection mode {mode!r}")
     vc, sc, b = _check_pair_3d(volume, sinogram, plan, "backward_3d")

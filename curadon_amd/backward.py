@@ -2,8 +2,8 @@
 backprojection, accumulated (+=) into a caller-provided volume, which is returned.
 
 Also accepted: ``backward(sinogram, geom)`` (the volume is then allocated with zeros) and
-``backward(sinogram, geom, volume=...)``. For 3D, ``mode`` selects the reference's projection
-('reference', the default) or the forward model's geometry ('matched').
+``backward(sinogram, geom, volume=...)``. ``mode`` selects the weighting / projection (see
+``backward``).
 """
 from __future__ import annotations
 
@@ -15,13 +15,18 @@ from .geometry import ConeGeometry, FanGeometry
 
 
 def backward(sinogram, volume=None, geom=None, *, volume_out=None, mode="reference"):
+    """``mode``: 'reference' (default, the reference's weighting and projection), 'fbp' (2D:
+    squared magnification weight), 'matched' / 'fdk' (3D: the forward model's projection,
+    unweighted / FDK-weighted)."""
     volume, geom = _split(volume, geom, None, volume_out, "volume")
     if isinstance(geom, FanGeometry):
+        if mode not in ("reference", "fbp"):
+            raise ValueError(f"fan-beam backward: unknown mode {mode!r}")
         if volume is None:
             volume = torch.zeros(tuple(sinogram.shape[:-2]) + tuple(int(v) for v in geom.vol_shape),
                                  dtype=_C.dtype_of_code(geom.plan.vol_prec),
                                  device=sinogram.device)
-        _C.backward_2d(volume, sinogram, geom.plan)
+        _C.backward_2d(volume, sinogram, geom.plan, weighting=mode)
         return volume
     if isinstance(geom, ConeGeometry):
         if volume is None:

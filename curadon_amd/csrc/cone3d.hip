@@ -5,6 +5,7 @@
 //   plan tables  fgeo [A] FwdGeom (FP64)     detector pixel (0,0), u/v steps, source (voxel frame)
 //                mref [A][12] float          reference-mode projection matrix (see bp_matrices)
 //                mmat [A][12] float          geometry-matched projection matrix
+//                mfdk [A][12] float          matched matrix scaled so that W = (depth / DSO)
 //   per call     forward:  [B][nz+4][ny+4][nx+4] volume with a 2-voxel zero border, x fastest
 //                backward: [B][A][V+4][U+4]      panels with a 2-pixel zero border
 //                adjoint:  [B][nz][ny][nx] FP32 accumulator when the volume is stored in 16 bits
@@ -19,6 +20,7 @@
 #include <memory>
 #include <vector>
 
+#include "bp_tiles.hpp"
 #include "cmi_internal.hpp"
 #include "geometry3d.hpp"
 
@@ -35,6 +37,7 @@ struct cmi_plan3d {
     FwdGeom *d_fgeo = nullptr;
     float *d_mref = nullptr;
     float *d_mmat = nullptr;
+    float *d_mfdk = nullptr;
 };
 
 namespace {
@@ -42,7 +45,7 @@ namespace {
 constexpr int kPad = 2;        // zero border of staged volumes and panels
 constexpr int kMaxBlock = 256; // threads per workgroup every kernel here is compiled for
 constexpr int kSumBlock = 32;  // samples per partial sum in the forward march
-constexpr int kZB = 8;         // z voxels per thread in the backprojector
+static_assert(kPad == kPadDet, "panels are staged with the backprojector's border");
 
 // ---------------------------------------------------------------- ray parameters ----------
 
@@ -247,10 +250,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(float *__restrict__ acc,
 // ---------------------------------------------------------------- backprojection ----------
 
 // One thread per voxel column (ix, iy) and kZB consecutive z; a wavefront covers a 16 x 4 x 8
-// block. For each angle the 3x4 matrix M maps the centred voxel index h = (ix-cx, iy-cy,
-// iz-cz, 1) to (U, V, W); (U/W, V/W) is the lower-tap panel coordinate of the bilinear sample.
-// MATCHED drops voxels behind the source (W <= 0). vol += unweighted sum over angles.
-template <class SV, class SS, bool MATCHED>
+// block (bp3d_column). vol += sum over angles.
+template <class SV, class SS, bool MATCHED, bool FDK>
 __global__ __launch_bounds__(kMaxBlock) void k_bp3d(typename SV::raw *__restrict__ vol,
                                                     const typename SS::raw *__restrict__ ppad,
                                                     const float *__restrict__ M, int nx, int ny,
@@ -262,40 +263,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp3d(typename SV::raw *__restrict
     const int iz0 = (blockIdx.z % nzb) * kZB;
     const size_t b = blockIdx.z / nzb;
     if (ix >= nx || iy >= ny) return;
-    const int Up = U + 2 * kPad, Vp = V + 2 * kPad;
-    const uint32_t panel = (uint32_t)Up * Vp;
-    const rsrc_t R = make_rsrc(ppad + b * (size_t)panel * A, bytes_of<SS>((uint64_t)panel * A));
-    const int sRow = Up * (int)sizeof(typename SS::raw);
-    const float hx = (float)ix - cx, hy = (float)iy - cy, hz = (float)iz0 - cz;
-    const float ulim = (float)U, vlim = (float)V;
+    const uint32_t stack = (uint32_t)(U + 2 * kPad) * (V + 2 * kPad) * A;
+    const rsrc_t R = make_rsrc(ppad + b * (size_t)stack, bytes_of<SS>(stack));
     float acc[kZB];
 #pragma unroll
     for (int z = 0; z < kZB; ++z) acc[z] = 0.0f;
-    int base = kPad * Up + kPad;
-    for (int a = 0; a < A; ++a, base += panel) {
-        const float *m = M + 12 * a; // wave-uniform: scalar loads
-        float Uh = fmaf(m[0], hx, fmaf(m[1], hy, fmaf(m[2], hz, m[3])));
-        float Vh = fmaf(m[4], hx, fmaf(m[5], hy, fmaf(m[6], hz, m[7])));
-        float Wh = fmaf(m[8], hx, fmaf(m[9], hy, fmaf(m[10], hz, m[11])));
-#pragma unroll
-        for (int z = 0; z < kZB; ++z) {
-            const float rw = __builtin_amdgcn_rcpf(Wh);
-            const float u = __builtin_amdgcn_fmed3f(Uh * rw, -2.0f, ulim);
-            const float v = __builtin_amdgcn_fmed3f(Vh * rw, -2.0f, vlim);
-            const float fu = __builtin_amdgcn_fractf(u), fv = __builtin_amdgcn_fractf(v);
-            const int o = base + __mul24((int)floorf(v), Up) + (int)floorf(u);
-            const Pair p0 = buf_pair<SS>(R, o, 0);
-            const Pair p1 = buf_pair<SS>(R, o, sRow);
-            const float r0 = fmaf(fu, p0.hi - p0.lo, p0.lo);
-            const float r1 = fmaf(fu, p1.hi - p1.lo, p1.lo);
-            float s = fmaf(fv, r1 - r0, r0);
-            if (MATCHED) s = Wh > 0.0f ? s : 0.0f;
-            acc[z] += s;
-            Uh += m[2];
-            Vh += m[6];
-            Wh += m[10];
-        }
-    }
+    bp3d_column<SS, MATCHED, FDK>(R, M, A, U, V, (float)ix - cx, (float)iy - cy,
+                                  (float)iz0 - cz, acc);
 #pragma unroll
     for (int z = 0; z < kZB; ++z) {
         const int iz = iz0 + z;
@@ -316,16 +290,20 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp3d(typename SV::raw *__restrict
 //   matched:   the forward frame: voxel P = src + t (org + iu du + iv dv - src) solved for
 //              (t, t iu, t iv); the lower-tap coordinate of pixel centre iu is iu.
 void bp_matrices(const cone3d::Desc &g, const double *phi, const double *theta, const double *psi,
-                 int64_t A, std::vector<float> &mref, std::vector<float> &mmat) {
+                 int64_t A, std::vector<float> &mref, std::vector<float> &mmat,
+                 std::vector<float> &mfdk) {
     mref.assign(12 * A, 0.0f);
     mmat.assign(12 * A, 0.0f);
+    mfdk.assign(12 * A, 0.0f);
     const double c[3] = {((double)g.nx - 1.0) / 2.0, ((double)g.ny - 1.0) / 2.0,
                          ((double)g.nz - 1.0) / 2.0};
-    auto finish = [&](double rows[3][4], float *out) {
+    // fixed_scale > 0: multiply the rows by it instead of normalising W to 1 at the centre
+    auto finish = [&](double rows[3][4], float *out, double fixed_scale) {
         for (int r = 0; r < 3; ++r)
             rows[r][3] += rows[r][0] * c[0] + rows[r][1] * c[1] + rows[r][2] * c[2];
         const double w = rows[2][3];
-        const double sc = (w != 0.0 && std::isfinite(w)) ? 1.0 / w : 1.0;
+        const double sc = fixed_scale > 0.0 ? fixed_scale
+                                            : ((w != 0.0 && std::isfinite(w)) ? 1.0 / w : 1.0);
         for (int r = 0; r < 3; ++r)
             for (int k = 0; k < 4; ++k) out[4 * r + k] = (float)(rows[r][k] * sc);
     };
@@ -344,7 +322,7 @@ void bp_matrices(const cone3d::Desc &g, const double *phi, const double *theta, 
             rr[1][k] = cc * rowY[k] + (bg.src[1] + hv - 0.5) * rowW[k];
             rr[2][k] = rowW[k];
         }
-        finish(rr, mref.data() + 12 * a);
+        finish(rr, mref.data() + 12 * a, 0.0);
 
         const FwdGeom fg = cone3d::fwd_angle(g, phi[a], theta[a], psi[a]);
         double Mc[3][3]; // columns org - src, du, dv
@@ -375,8 +353,64 @@ void bp_matrices(const cone3d::Desc &g, const double *phi, const double *theta, 
             rr[r][2] = ir[2];
             rr[r][3] = -(ir[0] * fg.src[0] + ir[1] * fg.src[1] + ir[2] * fg.src[2]);
         }
-        finish(rr, mmat.data() + 12 * a);
+        double rk[3][4];
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 4; ++k) rk[r][k] = rr[r][k];
+        finish(rr, mmat.data() + 12 * a, 0.0);
+        // W = t (the voxel's parameter on its ray, 1 at the panel); depth = t DSD
+        finish(rk, mfdk.data() + 12 * a, g.dsd / g.dso);
     }
+}
+
+// Python order (z, y, x) / (v, u) -> (x, y, z) / (u, v), as forward.cu:113-124
+cone3d::Desc desc_of(const cmi_cone3d_desc *d) {
+    cone3d::Desc g{};
+    g.nx = (int64_t)d->vol_shape[2];
+    g.ny = (int64_t)d->vol_shape[1];
+    g.nz = (int64_t)d->vol_shape[0];
+    g.sx = d->vol_spacing[2];
+    g.sy = d->vol_spacing[1];
+    g.sz = d->vol_spacing[0];
+    g.ox = d->vol_offset[2];
+    g.oy = d->vol_offset[1];
+    g.oz = d->vol_offset[0];
+    g.nu = (int64_t)d->det_shape[1];
+    g.nv = (int64_t)d->det_shape[0];
+    g.du = d->det_spacing[1];
+    g.dv = d->det_spacing[0];
+    g.off_u = d->det_offset[1];
+    g.off_v = d->det_offset[0];
+    g.pitch = d->det_rotation[0]; // forward.cu:136-138
+    g.roll = d->det_rotation[1];
+    g.yaw = d->det_rotation[2];
+    g.dso = d->DSO;
+    g.dsd = d->DSD;
+    g.cor = d->COR;
+
+    return g;
+}
+
+cmi_status check_desc(const cmi_cone3d_desc *d, const char *fn) {
+    if (!d) return fail(CMI_ERR_INVALID_ARGUMENT, "%s: descriptor is null", fn);
+    if (!d->phi || !d->theta || !d->psi)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "%s: null angle array", fn);
+    for (int i = 0; i < 3; ++i)
+        if (d->vol_shape[i] == 0 || d->vol_shape[i] > 4096 || !(d->vol_spacing[i] > 0))
+            return fail(CMI_ERR_INVALID_ARGUMENT,
+                        "%s: volume shape / spacing out of range (axis %d)", fn, i);
+    for (int i = 0; i < 2; ++i)
+        if (d->det_shape[i] == 0 || d->det_shape[i] > 16384 || !(d->det_spacing[i] > 0))
+            return fail(CMI_ERR_INVALID_ARGUMENT,
+                        "%s: detector shape / spacing out of range (axis %d)", fn, i);
+    if (d->nangles == 0 || d->nangles > 65535)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "%s: nangles must be in [1, 65535]", fn);
+    if (!std::isfinite(d->DSO) || !std::isfinite(d->DSD) || !std::isfinite(d->COR))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "%s: non-finite distance", fn);
+    for (uint64_t a = 0; a < d->nangles; ++a)
+        if (!std::isfinite(d->phi[a]) || !std::isfinite(d->theta[a]) || !std::isfinite(d->psi[a]))
+            return fail(CMI_ERR_INVALID_ARGUMENT, "%s: non-finite angle %llu", fn,
+                        (unsigned long long)a);
+    return CMI_OK;
 }
 
 template <class F>
@@ -433,21 +467,9 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
     clear_error();
     if (!plan) return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: output pointer is null");
     *plan = nullptr;
-    if (!d) return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: descriptor is null");
+    CMI_TRY(check_desc(d, "plan3d_create"));
     if (!dtype_ok(vol_prec) || !dtype_ok(det_prec))
         return fail(CMI_ERR_DTYPE, "plan3d_create: precision must be f32, f16 or bf16");
-    if (!d->phi || !d->theta || !d->psi)
-        return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: null angle array");
-    for (int i = 0; i < 3; ++i)
-        if (d->vol_shape[i] == 0 || d->vol_shape[i] > 4096 || !(d->vol_spacing[i] > 0))
-            return fail(CMI_ERR_INVALID_ARGUMENT,
-                        "plan3d_create: volume shape / spacing out of range (axis %d)", i);
-    for (int i = 0; i < 2; ++i)
-        if (d->det_shape[i] == 0 || d->det_shape[i] > 16384 || !(d->det_spacing[i] > 0))
-            return fail(CMI_ERR_INVALID_ARGUMENT,
-                        "plan3d_create: detector shape / spacing out of range (axis %d)", i);
-    if (d->nangles == 0 || d->nangles > 65535)
-        return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: nangles must be in [1, 65535]");
     const uint64_t padvol = (d->vol_shape[0] + 2 * kPad) * (d->vol_shape[1] + 2 * kPad) *
                             (d->vol_shape[2] + 2 * kPad);
     const uint64_t padpan = (d->det_shape[0] + 2 * kPad) * (d->det_shape[1] + 2 * kPad) *
@@ -455,47 +477,20 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
     if (padvol * 4 >= (1ull << 31) || padpan * 4 >= (1ull << 31))
         return fail(CMI_ERR_INVALID_ARGUMENT,
                     "plan3d_create: volume or projection stack exceeds 2 GiB");
-    if (!std::isfinite(d->DSO) || !std::isfinite(d->DSD) || !std::isfinite(d->COR))
-        return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: non-finite distance");
-    for (uint64_t a = 0; a < d->nangles; ++a)
-        if (!std::isfinite(d->phi[a]) || !std::isfinite(d->theta[a]) || !std::isfinite(d->psi[a]))
-            return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create: non-finite angle %llu",
-                        (unsigned long long)a);
 
     auto p = std::make_unique<cmi_plan3d>();
     p->device = device;
     p->vol_prec = vol_prec;
     p->det_prec = det_prec;
-    // Python order (z, y, x) / (v, u) -> (x, y, z) / (u, v), as forward.cu:113-124
-    cone3d::Desc &g = p->g;
-    g.nx = (int64_t)d->vol_shape[2];
-    g.ny = (int64_t)d->vol_shape[1];
-    g.nz = (int64_t)d->vol_shape[0];
-    g.sx = d->vol_spacing[2];
-    g.sy = d->vol_spacing[1];
-    g.sz = d->vol_spacing[0];
-    g.ox = d->vol_offset[2];
-    g.oy = d->vol_offset[1];
-    g.oz = d->vol_offset[0];
-    g.nu = (int64_t)d->det_shape[1];
-    g.nv = (int64_t)d->det_shape[0];
-    g.du = d->det_spacing[1];
-    g.dv = d->det_spacing[0];
-    g.off_u = d->det_offset[1];
-    g.off_v = d->det_offset[0];
-    g.pitch = d->det_rotation[0]; // forward.cu:136-138
-    g.roll = d->det_rotation[1];
-    g.yaw = d->det_rotation[2];
-    g.dso = d->DSO;
-    g.dsd = d->DSD;
-    g.cor = d->COR;
+    p->g = desc_of(d);
+    const cone3d::Desc &g = p->g;
     p->A = (int64_t)d->nangles;
     const int64_t A = p->A;
 
     std::vector<FwdGeom> fgeo(A);
     for (int64_t a = 0; a < A; ++a) fgeo[a] = cone3d::fwd_angle(g, d->phi[a], d->theta[a], d->psi[a]);
-    std::vector<float> mref, mmat;
-    bp_matrices(g, d->phi, d->theta, d->psi, A, mref, mmat);
+    std::vector<float> mref, mmat, mfdk;
+    bp_matrices(g, d->phi, d->theta, d->psi, A, mref, mmat, mfdk);
 
     DeviceGuard guard;
     CMI_TRY(guard.enter(device));
@@ -507,12 +502,14 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
             (void)hipFree(p->d_fgeo);
             (void)hipFree(p->d_mref);
             (void)hipFree(p->d_mmat);
+            (void)hipFree(p->d_mfdk);
         }
     } cleanup{p.get()};
     CMI_HIP_TRY(hipMalloc(&p->d_fgeo, sizeof(FwdGeom) * A));
     CMI_HIP_TRY(hipMalloc(&p->d_mref, sizeof(float) * 12 * A));
     CMI_HIP_TRY(hipMalloc(&p->d_mmat, sizeof(float) * 12 * A));
-    p->table_bytes = (sizeof(FwdGeom) + 2 * 12 * sizeof(float)) * A;
+    CMI_HIP_TRY(hipMalloc(&p->d_mfdk, sizeof(float) * 12 * A));
+    p->table_bytes = (sizeof(FwdGeom) + 3 * 12 * sizeof(float)) * A;
     unsigned long long *d_total = nullptr;
     CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
     std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
@@ -526,6 +523,8 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
     CMI_HIP_TRY(hipMemcpyAsync(p->d_mref, mref.data(), sizeof(float) * 12 * A,
                                hipMemcpyHostToDevice, s));
     CMI_HIP_TRY(hipMemcpyAsync(p->d_mmat, mmat.data(), sizeof(float) * 12 * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_mfdk, mfdk.data(), sizeof(float) * 12 * A,
                                hipMemcpyHostToDevice, s));
     CMI_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(unsigned long long), s));
     dim3 blk(256), grd((unsigned)((g.nu + 255) / 256), (unsigned)g.nv, (unsigned)A);
@@ -541,6 +540,20 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
     return CMI_OK;
 }
 
+CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *d, int mode, float *out) {
+    clear_error();
+    CMI_TRY(check_desc(d, "cone3d_bp_matrices"));
+    if (!out) return fail(CMI_ERR_INVALID_ARGUMENT, "cone3d_bp_matrices: null output");
+    if (mode != CMI_BP3D_REFERENCE && mode != CMI_BP3D_MATCHED && mode != CMI_BP3D_FDK)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "cone3d_bp_matrices: unknown mode %d", mode);
+    const cone3d::Desc g = desc_of(d);
+    std::vector<float> mref, mmat, mfdk;
+    bp_matrices(g, d->phi, d->theta, d->psi, (int64_t)d->nangles, mref, mmat, mfdk);
+    const std::vector<float> &m = mode == CMI_BP3D_MATCHED ? mmat : mode == CMI_BP3D_FDK ? mfdk : mref;
+    std::copy(m.begin(), m.end(), out);
+    return CMI_OK;
+}
+
 CMI_API cmi_status cmi_plan3d_destroy(cmi_plan3d *plan) {
     clear_error();
     if (!plan) return CMI_OK;
@@ -551,6 +564,7 @@ CMI_API cmi_status cmi_plan3d_destroy(cmi_plan3d *plan) {
         (void)hipFree(plan->d_fgeo);
         (void)hipFree(plan->d_mref);
         (void)hipFree(plan->d_mmat);
+        (void)hipFree(plan->d_mfdk);
     }
     delete plan;
     return st;
@@ -612,7 +626,7 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
                                    int mode, void *stream) {
     clear_error();
     CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "backward_3d"));
-    if (mode != CMI_BP3D_REFERENCE && mode != CMI_BP3D_MATCHED)
+    if (mode != CMI_BP3D_REFERENCE && mode != CMI_BP3D_MATCHED && mode != CMI_BP3D_FDK)
         return fail(CMI_ERR_INVALID_ARGUMENT, "backward_3d: unknown mode %d", mode);
     DeviceGuard guard;
     CMI_TRY(guard.enter(plan->device));
@@ -626,7 +640,9 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
         hipMallocAsync(&scratch, (size_t)Up * Vp * A * dtype_size(sino_dtype) * batch, s));
     const float cx = (float)((nx - 1) / 2.0), cy = (float)((ny - 1) / 2.0),
                 cz = (float)((nz - 1) / 2.0);
-    const float *M = mode == CMI_BP3D_MATCHED ? plan->d_mmat : plan->d_mref;
+    const float *M = mode == CMI_BP3D_MATCHED ? plan->d_mmat
+                     : mode == CMI_BP3D_FDK   ? plan->d_mfdk
+                                              : plan->d_mref;
     const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
         using SV = decltype(sv);
         using SS = decltype(ss);
@@ -637,12 +653,11 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
         const int nzb = (nz + kZB - 1) / kZB;
         dim3 bb(16, 16), bg((unsigned)((nx + 15) / 16), (unsigned)((ny + 15) / 16),
                             (unsigned)(nzb * batch));
-        if (mode == CMI_BP3D_MATCHED)
-            hipLaunchKernelGGL((k_bp3d<SV, SS, true>), bg, bb, 0, s, (typename SV::raw *)volume,
-                               (const RS *)scratch, M, nx, ny, nz, U, V, A, cx, cy, cz);
-        else
-            hipLaunchKernelGGL((k_bp3d<SV, SS, false>), bg, bb, 0, s, (typename SV::raw *)volume,
-                               (const RS *)scratch, M, nx, ny, nz, U, V, A, cx, cy, cz);
+        auto kern = mode == CMI_BP3D_MATCHED ? k_bp3d<SV, SS, true, false>
+                    : mode == CMI_BP3D_FDK   ? k_bp3d<SV, SS, true, true>
+                                             : k_bp3d<SV, SS, false, false>;
+        hipLaunchKernelGGL(kern, bg, bb, 0, s, (typename SV::raw *)volume, (const RS *)scratch, M,
+                           nx, ny, nz, U, V, A, cx, cy, cz);
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });

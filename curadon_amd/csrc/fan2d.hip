@@ -29,6 +29,7 @@
 #include <memory>
 #include <vector>
 
+#include "bp_tiles.hpp"
 #include "cmi_internal.hpp"
 #include "geometry2d.hpp"
 
@@ -55,6 +56,7 @@ namespace {
 
 constexpr int kPad = 2;        // zero border of the staged image copies, in texels
 constexpr int kMaxBlock = 256; // threads per workgroup every kernel here is compiled for
+static_assert(kPad == kPadDet, "sinograms are staged with the backprojector's border");
 
 // ---------------------------------------------------------------- ray table ----------------
 
@@ -209,39 +211,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd2d(const typename SI::raw *__r
 // ---------------------------------------------------------------- backprojection ---------
 
 // One thread per pixel, all angles in one pass (backward_2d.hpp:31-99):
-// vol[b][y][x] += (1/du) sum_a w_a lerp(sino[b][a], w_a eta_a + D/2),  w_a = DSD/(xi_a + DSO).
-template <class SV, class SS>
+// vol[b][y][x] += (1/du) sum_a w_a^WPOW lerp(sino[b][a], w_a eta_a + D/2),  w_a = DSD/(xi_a + DSO).
+// WPOW = 1 is the reference's weighting; WPOW = 2 the fan-beam FBP distance weight.
+template <class SV, class SS, int WPOW>
 __global__ __launch_bounds__(kMaxBlock) void k_bp2d(typename SV::raw *__restrict__ vol,
-                                              const typename SS::raw *__restrict__ spad,
-                                              const float4 *__restrict__ meta, int nx, int ny,
-                                              int D, int A, float sx, float sy, float ox,
-                                              float oy, float inv_du, float half_d) {
+                                                    const typename SS::raw *__restrict__ spad,
+                                                    const float4 *__restrict__ meta, int nx,
+                                                    int ny, int D, int A, float sx, float sy,
+                                                    float ox, float oy, float inv_du,
+                                                    float half_d) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
     if (ix >= nx || iy >= ny) return;
     const size_t b = blockIdx.z;
-    const int Dp = D + 2 * kPad;
-    const uint32_t sino_elems = (uint32_t)A * Dp;
+    const uint32_t sino_elems = (uint32_t)A * (D + 2 * kPadDet);
     const rsrc_t R = make_rsrc(spad + b * sino_elems, bytes_of<SS>(sino_elems));
     const float x = ((float)ix - (float)nx / 2.0f) * sx + ox + 0.5f;
     const float y = ((float)iy - (float)ny / 2.0f) * sy + oy + 0.5f;
-    const float px = x * inv_du, py = y * inv_du;
-    // Lower-tap coordinate u - 0.5, clamped into [-2, D]: outside the detector both taps then
-    // fall in the zero border, so no per-tap test is needed.
-    const float hi = (float)D;
-    float acc = 0.0f;
-    int row = kPad;
-#pragma unroll 4
-    for (int a = 0; a < A; ++a, row += Dp) {
-        const float4 m = meta[a]; // wave-uniform: scalar loads
-        const float xi = m.x * x - m.y * y;
-        const float eta = m.y * px + m.x * py;
-        const float w = m.z * __builtin_amdgcn_rcpf(xi + m.w);
-        const float u = __builtin_amdgcn_fmed3f(fmaf(w, eta, half_d) - 0.5f, -2.0f, hi);
-        const float al = __builtin_amdgcn_fractf(u);
-        const Pair p = buf_pair<SS>(R, row + (int)floorf(u), 0);
-        acc = fmaf(fmaf(al, p.hi - p.lo, p.lo), w, acc);
-    }
+    const float acc = bp2d_pixel<SS, WPOW>(R, meta, A, D, x, y, x * inv_du, y * inv_du, half_d);
     const size_t i = b * (size_t)nx * ny + (size_t)iy * nx + ix;
     vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc * inv_du);
 }
@@ -484,6 +471,20 @@ CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vo
     return CMI_OK;
 }
 
+CMI_API cmi_status cmi_fan2d_bp_table(uint64_t nangles, const double *angles, const double *DSO,
+                                      const double *DSD, float *table) {
+    clear_error();
+    if (!angles || !DSO || !DSD || !table)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "fan2d_bp_table: null argument");
+    for (uint64_t a = 0; a < nangles; ++a) {
+        table[4 * a + 0] = (float)-std::sin(angles[a]);
+        table[4 * a + 1] = (float)std::cos(angles[a]);
+        table[4 * a + 2] = (float)DSD[a];
+        table[4 * a + 3] = (float)DSO[a];
+    }
+    return CMI_OK;
+}
+
 CMI_API cmi_status cmi_plan2d_destroy(cmi_plan2d *plan) {
     clear_error();
     if (!plan) return CMI_OK;
@@ -578,11 +579,13 @@ CMI_API cmi_status cmi_forward_2d(const cmi_plan2d *plan, const void *volume, in
     return CMI_OK;
 }
 
-CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,
-                                   const void *sinogram, int sino_dtype, uint64_t batch,
-                                   void *stream) {
+CMI_API cmi_status cmi_backward_2d_weighted(const cmi_plan2d *plan, void *volume, int vol_dtype,
+                                            const void *sinogram, int sino_dtype,
+                                            uint64_t batch, int weighting, void *stream) {
     clear_error();
     CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "backward_2d"));
+    if (weighting != CMI_BP2D_REFERENCE && weighting != CMI_BP2D_FBP)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "backward_2d: unknown weighting %d", weighting);
     DeviceGuard guard;
     CMI_TRY(guard.enter(plan->device));
     const hipStream_t s = (hipStream_t)stream;
@@ -598,10 +601,11 @@ CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol
         hipLaunchKernelGGL((k_stage_sino2d<SS>), pg, pb, 0, s, (const RS *)sinogram,
                            (RS *)scratch, D, A);
         dim3 b(64, 4), g((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)batch);
-        hipLaunchKernelGGL((k_bp2d<SV, SS>), g, b, 0, s, (typename SV::raw *)volume,
-                           (const RS *)scratch, plan->d_meta, nx, ny, D, A, (float)plan->g.sx,
-                           (float)plan->g.sy, (float)plan->g.ox, (float)plan->g.oy,
-                           (float)(1.0 / plan->g.du), (float)D / 2.0f);
+        auto kern = weighting == CMI_BP2D_FBP ? k_bp2d<SV, SS, 2> : k_bp2d<SV, SS, 1>;
+        hipLaunchKernelGGL(kern, g, b, 0, s, (typename SV::raw *)volume, (const RS *)scratch,
+                           plan->d_meta, nx, ny, D, A, (float)plan->g.sx, (float)plan->g.sy,
+                           (float)plan->g.ox, (float)plan->g.oy, (float)(1.0 / plan->g.du),
+                           (float)D / 2.0f);
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });
@@ -609,6 +613,13 @@ CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol
     if (st != CMI_OK) return st;
     CMI_HIP_TRY(fe);
     return CMI_OK;
+}
+
+CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,
+                                   const void *sinogram, int sino_dtype, uint64_t batch,
+                                   void *stream) {
+    return cmi_backward_2d_weighted(plan, volume, vol_dtype, sinogram, sino_dtype, batch,
+                                    CMI_BP2D_REFERENCE, stream);
 }
 
 CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,

@@ -1,0 +1,141 @@
+"""Filtered backprojection around the operators (the pipeline of examples/fbp.py:27-99).
+
+Fan beam (flat detector, full 2 pi scan):
+    f = (pi / A) * (DSO / DSD) * B_fbp[ (cos_w * p) (*) h ]
+FDK cone beam (flat panel, circular scan):
+    f = (pi / A) * DSD / (du * DSO) * B_fdk[ (cos_w * p) (*)_u h ]
+where cos_w = DSD / sqrt(DSD^2 + u^2 [+ v^2]) is the cosine pre-weight, (*) the convolution
+along detector rows with the band-limited ramp h of unit cell spacing (Kak and Slaney,
+"Principles of Computerized Tomographic Imaging", eq. 3.61: h[0] = 1/4, h[k odd] =
+-1/(pi k)^2, h[k even] = 0) times a frequency window, B_fbp the 2D backprojector with the
+squared magnification weight and B_fdk the geometry-matched 3D backprojector with the
+(DSO / depth)^2 weight. The convolution runs through an FFT of length >= 2 D (zero padded, so
+it is linear, not circular). The result is in the image's density units.
+
+Windows follow examples/fbp.py:27-48 over the normalised frequency |W| in [0, 1] (1 = Nyquist):
+ram-lak 1, shepp-logan sinc(W / (2 s)), cosine cos(W pi / (2 s)) (the reference multiplies by
+an undefined H there), hamming 0.54 + 0.46 cos(W pi / s), hann cos(W pi / (2 s))^2, and a
+cutoff |W| <= s (the reference tests W <= s, which keeps every negative frequency).
+
+`reference_filter` reproduces examples/fbp.py exactly (circular FFT filtering with
+fftshift(|linspace(-1, 1, D)|), times pi / (2 A)), for callers that want its output.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from .geometry import ConeGeometry, FanGeometry
+
+WINDOWS = ("ram-lak", "shepp-logan", "cosine", "hamming", "hann")
+
+
+def _window(W: torch.Tensor, name, s: float) -> torch.Tensor:
+    a = W.abs()
+    if name is None:
+        return torch.ones_like(a)
+    if name == "ram-lak":
+        w = torch.ones_like(a)
+    elif name == "shepp-logan":
+        w = torch.sinc(W / (2 * s))
+    elif name == "cosine":
+        w = torch.cos(W * math.pi / (2 * s))
+    elif name == "hamming":
+        w = 0.54 + 0.46 * torch.cos(a * math.pi / s)
+    elif name == "hann":
+        w = torch.cos(a * math.pi / (2 * s)) ** 2
+    else:
+        raise RuntimeError(f"Unknown Filter {name}")
+    return w * (a <= s)
+
+
+def ramp_response(n_det: int, window="ram-lak", frequency_scaling: float = 1.0,
+                  device=None) -> torch.Tensor:
+    """Real frequency response (rfft layout) of the windowed band-limited ramp for rows of
+    n_det cells, FFT length L = next power of two >= 2 n_det. Returns (L // 2 + 1,) float32."""
+    L = 1 << max(1, math.ceil(math.log2(2 * n_det)))
+    k = torch.arange(L, dtype=torch.float64)
+    m = torch.where(k <= L // 2, k, k - L)                   # signed index, FFT order
+    h = torch.zeros(L, dtype=torch.float64)
+    h[0] = 0.25
+    odd = (m.remainder(2) != 0)
+    h[odd] = -1.0 / (math.pi * m[odd]) ** 2
+    H = torch.fft.rfft(h).real                               # symmetric kernel: real response
+    W = torch.arange(L // 2 + 1, dtype=torch.float64) / (L // 2)
+    H = H * _window(W, window, frequency_scaling)
+    return H.to(torch.float32).to(device)
+
+
+def filter_rows(p: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
+    """Linear convolution of every row (last axis) of p with the filter of response H, in
+    FP64 for FP64 input and FP32 otherwise."""
+    D = p.shape[-1]
+    L = 2 * (H.shape[0] - 1)
+    dt = torch.float64 if p.dtype == torch.float64 else torch.float32
+    P = torch.fft.rfft(p.to(dt), n=L, dim=-1)
+    return torch.fft.irfft(P * H.to(dt), n=L, dim=-1)[..., :D]
+
+
+def cosine_weights_fan(geom: FanGeometry, device=None) -> torch.Tensor:
+    """(A, D) cosine pre-weights DSD / sqrt(DSD^2 + u^2), u the cell centre on the detector."""
+    D = geom.det_shape
+    u = (torch.arange(D, dtype=torch.float64) - (D - 1) / 2) * geom.det_spacing
+    dsd = torch.as_tensor(geom.DSD, dtype=torch.float64)[:, None]
+    return (dsd / torch.sqrt(dsd ** 2 + u[None, :] ** 2)).to(torch.float32).to(device)
+
+
+def cosine_weights_cone(geom: ConeGeometry, device=None) -> torch.Tensor:
+    """(V, U) cosine pre-weights DSD / sqrt(DSD^2 + u^2 + v^2)."""
+    V, U = (int(v) for v in geom.det_shape)
+    dv, du = (float(v) for v in geom.det_spacing)
+    u = (torch.arange(U, dtype=torch.float64) - (U - 1) / 2) * du
+    v = (torch.arange(V, dtype=torch.float64) - (V - 1) / 2) * dv
+    r2 = v[:, None] ** 2 + u[None, :] ** 2
+    return (geom.DSD / torch.sqrt(geom.DSD ** 2 + r2)).to(torch.float32).to(device)
+
+
+def fbp_filter(sinogram: torch.Tensor, geom, window="ram-lak", frequency_scaling=1.0):
+    """Cosine-weighted, ramp-filtered and scaled projections (FP32), ready for the weighted
+    backprojection (`backward(..., mode="fbp")` for fan beam, mode="fdk" for cone beam)."""
+    dev = sinogram.device
+    dt = torch.float64 if sinogram.dtype == torch.float64 else torch.float32
+    if isinstance(geom, FanGeometry):
+        H = ramp_response(geom.det_shape, window, frequency_scaling, dev)
+        cw = cosine_weights_fan(geom, dev).to(dt)
+        scale = (math.pi / geom.nangles) * torch.as_tensor(
+            geom.DSO / geom.DSD, dtype=dt, device=dev)[:, None]
+        return filter_rows(sinogram.to(dt) * cw, H) * scale
+    if isinstance(geom, ConeGeometry):
+        H = ramp_response(int(geom.det_shape[1]), window, frequency_scaling, dev)
+        cw = cosine_weights_cone(geom, dev).to(dt)
+        scale = (math.pi / geom.nangles) * geom.DSD / (float(geom.det_spacing[1]) * geom.DSO)
+        return filter_rows(sinogram.to(dt) * cw, H) * scale
+    raise ValueError("Unknown geometry type")
+
+
+def fbp(sinogram: torch.Tensor, geom, window="ram-lak", frequency_scaling=1.0, volume=None):
+    """Filtered backprojection (fan beam) / FDK (cone beam) into `volume` (+=) or a new one.
+    The projections are filtered in FP32 (FP64 for FP64 input) and the filtered projections
+    cast to the geometry's sinogram precision before the backprojection. Pass the raw
+    projections in FP32: rounding them to FP16 before the ramp filter costs ~1.5e-3 relative L2
+    on its own, storing the filtered ones in FP16 ~2e-5."""
+    from . import _lib
+    from .backward import backward
+    q = fbp_filter(sinogram, geom, window, frequency_scaling)
+    q = q.to(_lib.dtype_of_code(geom.plan.det_prec)).contiguous()
+    mode = "fbp" if isinstance(geom, FanGeometry) else "fdk"
+    return backward(q, volume, geom, mode=mode)
+
+
+def reference_filter(sinogram: torch.Tensor, filter_name="ram-lak", frequency_scaling=1.0):
+    """examples/fbp.py's filtering verbatim: circular FFT convolution with
+    fftshift(window(|linspace(-1, 1, D)|)), times pi / (2 A)."""
+    A, D = sinogram.shape[-2], sinogram.shape[-1]
+    W = torch.linspace(-1, 1, D, dtype=torch.float64)
+    w = W.abs() * (_window(W, filter_name, frequency_scaling) if filter_name else 1.0)
+    if filter_name is None:
+        w = torch.ones(D, dtype=torch.float64)
+    w = torch.fft.fftshift(w).to(torch.float32).to(sinogram.device)
+    out = torch.fft.ifft(torch.fft.fft(sinogram.to(torch.float32), dim=-1) * w, dim=-1).real
+    return out * (math.pi / (2 * A))

@@ -56,7 +56,17 @@ typedef enum cmi_status {
 typedef enum cmi_dtype { CMI_F16 = 16, CMI_F32 = 32, CMI_BF16 = 1016 } cmi_dtype;
 
 /* Backprojector variants for 3D (see cmi_backward_3d). */
-typedef enum cmi_bp3d_mode { CMI_BP3D_REFERENCE = 0, CMI_BP3D_MATCHED = 1 } cmi_bp3d_mode;
+typedef enum cmi_bp3d_mode {
+    CMI_BP3D_REFERENCE = 0, /* the reference's projection, unweighted */
+    CMI_BP3D_MATCHED = 1,   /* the forward model's projection, unweighted */
+    CMI_BP3D_FDK = 2        /* the forward model's projection, weight (DSO / depth)^2 */
+} cmi_bp3d_mode;
+
+/* Weightings of the 2D backprojector (see cmi_backward_2d_weighted). */
+typedef enum cmi_bp2d_weighting {
+    CMI_BP2D_REFERENCE = 0, /* w = DSD / (DSO + xi), the reference's magnification weight */
+    CMI_BP2D_FBP = 1        /* w^2, the fan-beam FBP distance weight */
+} cmi_bp2d_weighting;
 
 /* Message of the last failed call on this host thread ("" if none). */
 CMI_API const char *cmi_last_error(void);
@@ -128,6 +138,15 @@ CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol
                                    void *stream);
 
 /*
+ * cmi_backward_2d with a choice of weighting: CMI_BP2D_REFERENCE is cmi_backward_2d,
+ * CMI_BP2D_FBP squares the magnification weight, which a fan-beam filtered backprojection needs
+ * (the reference notes its weighting is not the FBP one, backward_2d.hpp:94-95).
+ */
+CMI_API cmi_status cmi_backward_2d_weighted(const cmi_plan2d *plan, void *volume, int vol_dtype,
+                                            const void *sinogram, int sino_dtype,
+                                            uint64_t batch, int weighting, void *stream);
+
+/*
  * Matched adjoint of cmi_forward_2d, volume += A^T sinogram: the exact transpose of the
  * ray-march above (same sample positions and bilinear weights), in deterministic gather form.
  * No counterpart in the reference, whose autograd pairs the forward with the voxel-driven
@@ -195,6 +214,9 @@ CMI_API cmi_status cmi_forward_3d(const cmi_plan3d *plan, const void *volume, in
  * mode CMI_BP3D_REFERENCE reproduces the reference's projection (detector plane at
  * z = DSO - DSD, integer detector centre, detector spacing ignored; backward_3d.hpp:79-94).
  * mode CMI_BP3D_MATCHED projects each voxel through the forward model's geometry instead.
+ * mode CMI_BP3D_FDK projects like CMI_BP3D_MATCHED and weights each sample by
+ * (DSO / (t DSD))^2, t the voxel's parameter on its ray (t = 1 at the panel): the FDK distance
+ * weight for an untilted panel.
  */
 CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol_dtype,
                                    const void *sinogram, int sino_dtype, uint64_t batch,

@@ -240,7 +240,8 @@ void orc_adj2d(const orc_fan2d *g, const double *sino, double *img, int64_t a0, 
 
 /* ------------------------------------------------------ 2D backprojection (A.3) ------- */
 
-void orc_bp2d(const orc_fan2d *g, const double *sino, double *img, int64_t y0, int64_t y1) {
+void orc_bp2d_w(const orc_fan2d *g, const double *sino, double *img, int64_t y0, int64_t y1,
+                int wpow) {
     const int64_t A = g->nangles, D = g->det_count;
     double *sn = (double *)malloc((size_t)A * sizeof(double));
     double *cs = (double *)malloc((size_t)A * sizeof(double));
@@ -263,13 +264,18 @@ void orc_bp2d(const orc_fan2d *g, const double *sino, double *img, int64_t y0, i
                 const double yi = cs[a] * sx + sn[a] * sy;         /* :86 */
                 const double w = g->dsd[a] / (xi + g->dso[a]);     /* :89 */
                 const double u = w * yi + half_d;                  /* :92 */
-                acc += orc_lerp1(sino + a * D, D, u) * w;          /* :96 */
+                const double wt = wpow == 2 ? w * w : w;
+                acc += orc_lerp1(sino + a * D, D, u) * wt;         /* :96 */
             }
             img[iy * g->nx + ix] += acc * inv_du;                  /* :99 */
         }
     }
     free(sn);
     free(cs);
+}
+
+void orc_bp2d(const orc_fan2d *g, const double *sino, double *img, int64_t y0, int64_t y1) {
+    orc_bp2d_w(g, sino, img, y0, y1, 1);
 }
 
 /* ------------------------------------------------ FP32 ports for the CPU baseline ----- */
@@ -621,8 +627,8 @@ static void solve3(const double c0[3], const double c1[3], const double c2[3], c
 #undef DET3
 }
 
-void orc_bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int64_t z0,
-                      int64_t z1) {
+static void bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int64_t z0,
+                         int64_t z1, int fdk) {
     const size_t n3 = (size_t)g->nangles * 3;
     double *org = malloc(n3 * sizeof(double)), *du = malloc(n3 * sizeof(double));
     double *dv = malloc(n3 * sizeof(double)), *src = malloc(n3 * sizeof(double));
@@ -644,7 +650,12 @@ void orc_bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int6
                     solve3(c0, du + 3 * a, dv + 3 * a, w, x);
                     if (!(x[0] > 0.0)) continue;
                     const double iu = x[1] / x[0], iv = x[2] / x[0];
-                    acc += panel_bilerp(sino + a * UV, g->nu, g->nv, iu + 0.5, iv + 0.5);
+                    double wt = 1.0;
+                    if (fdk) {
+                        const double r = g->dso / (x[0] * g->dsd);
+                        wt = r * r;
+                    }
+                    acc += wt * panel_bilerp(sino + a * UV, g->nu, g->nv, iu + 0.5, iv + 0.5);
                 }
                 vol[(iz * g->ny + iy) * g->nx + ix] += acc;
             }
@@ -652,4 +663,13 @@ void orc_bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int6
     free(du);
     free(dv);
     free(src);
+}
+
+void orc_bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int64_t z0,
+                      int64_t z1) {
+    bp3d_matched(g, sino, vol, z0, z1, 0);
+}
+
+void orc_bp3d_fdk(const orc_cone3d *g, const double *sino, double *vol, int64_t z0, int64_t z1) {
+    bp3d_matched(g, sino, vol, z0, z1, 1);
 }

@@ -67,6 +67,12 @@ void orc_adj2d(const orc_fan2d *g, const double *sino, double *img, int64_t a0, 
  * img[y][x] += (1/du) * sum_a w_a * lerp(sino[a], u_a(x, y)). */
 void orc_bp2d(const orc_fan2d *g, const double *sino, double *img, int64_t y0, int64_t y1);
 
+/* Same pixel positions and detector coordinates as orc_bp2d with the magnification weight
+ * raised to the power wpow: wpow = 1 is orc_bp2d, wpow = 2 the fan-beam FBP distance weight
+ * (DSD / (DSO + xi))^2. */
+void orc_bp2d_w(const orc_fan2d *g, const double *sino, double *img, int64_t y0, int64_t y1,
+                int wpow);
+
 /* FP32 ports of the two hot loops, same arithmetic in single precision, OpenMP over
  * angles / image rows. Used only to time the reference math on host cores. */
 void orc_fwd2d_f32(const orc_fan2d *g, const float *img, float *sino, int64_t a0, int64_t a1);
@@ -136,6 +142,10 @@ void orc_bp3d(const orc_cone3d *g, const double *sino, double *vol, int64_t z0, 
  * detector plane, rotation direction and detector spacing taken from the forward model. */
 void orc_bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int64_t z0,
                       int64_t z1);
+
+/* orc_bp3d_matched with the FDK distance weight (DSO / (t DSD))^2, where t is the voxel's
+ * parameter on its ray (t = 1 at the panel), i.e. (DSO / depth)^2 for an untilted panel. */
+void orc_bp3d_fdk(const orc_cone3d *g, const double *sino, double *vol, int64_t z0, int64_t z1);
 
 #ifdef __cplusplus
 }

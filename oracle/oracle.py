@@ -62,6 +62,8 @@ def lib():
             "orc_fwd2d": (None, [F2, _dp, _dp, _i64, _i64, C.POINTER(_i64)]),
             "orc_adj2d": (None, [F2, _dp, _dp, _i64, _i64]),
             "orc_bp2d": (None, [F2, _dp, _dp, _i64, _i64]),
+            "orc_bp2d_w": (None, [F2, _dp, _dp, _i64, _i64, C.c_int]),
+            "orc_bp3d_fdk": (None, [F3, _dp, _dp, _i64, _i64]),
             "orc_fwd2d_f32": (None, [F2, _fp, _fp, _i64, _i64]),
             "orc_bp2d_f32": (None, [F2, _fp, _fp, _i64, _i64]),
             "orc_num_threads": (C.c_int, []),
@@ -167,14 +169,15 @@ def adj2d(geom, sino, angles=None):
     return img
 
 
-def bp2d(geom, sino, rows=None, img=None):
-    """FP64 voxel-driven backprojection; adds into ``img`` (zeros if None)."""
+def bp2d(geom, sino, rows=None, img=None, wpow=1):
+    """FP64 voxel-driven backprojection; adds into ``img`` (zeros if None). wpow = 2 uses the
+    fan-beam FBP weight (squared magnification)."""
     f = Fan(geom)
     sino = _c64(sino)
     assert sino.shape == (f.A, f.D)
     out = np.zeros((f.ny, f.nx)) if img is None else _c64(img).copy()
     y0, y1 = (0, f.ny) if rows is None else rows
-    lib().orc_bp2d(C.byref(f.s), _d(sino), _d(out), y0, y1)
+    lib().orc_bp2d_w(C.byref(f.s), _d(sino), _d(out), y0, y1, int(wpow))
     return out
 
 
@@ -304,12 +307,14 @@ def adj3d(geom, sino, angles=None):
     return vol
 
 
-def bp3d(geom, sino, slabs=None, vol=None, matched=False):
+def bp3d(geom, sino, slabs=None, vol=None, matched=False, fdk=False):
+    """FP64 3D backprojection: the reference's (default), geometry-matched, or matched with
+    the FDK distance weight."""
     c = Cone(geom)
     sino = _c64(sino)
     assert sino.shape == c.shape_sino
     out = np.zeros(c.shape_vol) if vol is None else _c64(vol).copy()
     z0, z1 = (0, c.shape_vol[0]) if slabs is None else slabs
-    fn = lib().orc_bp3d_matched if matched else lib().orc_bp3d
+    fn = lib().orc_bp3d_fdk if fdk else (lib().orc_bp3d_matched if matched else lib().orc_bp3d)
     fn(C.byref(c.s), _d(sino), _d(out), z0, z1)
     return out
