@@ -43,6 +43,8 @@ ERR_NOT_SUPPORTED = 6
 F16, F32, BF16 = 16, 32, 1016
 BP3D_REFERENCE, BP3D_MATCHED, BP3D_FDK = 0, 1, 2
 BP2D_REFERENCE, BP2D_FBP = 0, 1
+_BP3D_MODES = {"reference": BP3D_REFERENCE, "matched": BP3D_MATCHED, "fdk": BP3D_FDK}
+_BP2D_MODES = {"reference": BP2D_REFERENCE, "fbp": BP2D_FBP}
 
 _DTYPE_CODE = {torch.float32: F32, torch.float16: F16, torch.bfloat16: BF16}
 _CODE_DTYPE = {v: k for k, v in _DTYPE_CODE.items()}
@@ -93,6 +95,18 @@ class Plan3DInfo(C.Structure):
     ]
 
 
+class BPJob(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("mode", C.c_int32), ("sino", _vp), ("table", _vp),
+        ("volume", _vp), ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+        ("nu", C.c_uint32), ("nv", C.c_uint32), ("nangles", C.c_uint32), ("sx", C.c_float),
+        ("sy", C.c_float), ("ox", C.c_float), ("oy", C.c_float), ("inv_du", C.c_float),
+        ("tile_begin", C.c_uint32), ("tile_end", C.c_uint32),
+    ]
+
+
+JOB_FAN2D, JOB_CONE3D = 0, 1
+
 # Every symbol the public header declares, with its ctypes signature.
 SIGNATURES = {
     "cmi_last_error": (C.c_char_p, []),
@@ -113,6 +127,14 @@ SIGNATURES = {
     "cmi_forward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_backward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_adjoint_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_fan2d_bp_table": (_i32, [_u64, _dp, _dp, _dp, C.POINTER(C.c_float)]),
+    "cmi_cone3d_bp_matrices": (_i32, [C.POINTER(Cone3DDesc), _i32, C.POINTER(C.c_float)]),
+    "cmi_stage_projections": (_i32, [_i32, _vp, _u64, _u64, _u64, _u64, C.c_uint32, _vp,
+                                     C.c_float, _vp, _i32, _vp]),
+    "cmi_bp_job_tiles": (_i32, [C.POINTER(BPJob), _u64p]),
+    "cmi_server_create": (_i32, [_i32, C.c_uint32, C.POINTER(_vp)]),
+    "cmi_server_destroy": (_i32, [_vp]),
+    "cmi_serve_backproject": (_i32, [_vp, C.POINTER(BPJob), C.c_uint32, _i32, _i32, _vp]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():
@@ -376,6 +398,7 @@ class plan_3d:
         d.phi = self._phi.ctypes.data_as(_dp)
         d.theta = self._theta.ctypes.data_as(_dp)
         d.psi = self._psi.ctypes.data_as(_dp)
+        self._desc = d
         self.vol_shape = tuple(vs)
         self.sino_shape = (A, ds[0], ds[1])
         self.nangles = A
@@ -387,6 +410,13 @@ class plan_3d:
     @property
     def handle(self):
         return self._h
+
+    def bp_matrices(self, mode="fdk") -> np.ndarray:
+        """(A, 3, 4) FP32 projection matrices cmi_backward_3d uses in `mode`."""
+        out = np.zeros((self.nangles, 3, 4), dtype=np.float32)
+        check(_lib.cmi_cone3d_bp_matrices(C.byref(self._desc), _BP3D_MODES[mode],
+                                          out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
 
     def info(self) -> dict:
         i = Plan3DInfo()
@@ -436,7 +466,7 @@ def forward_3d_plan(volume, sinogram, plan):
 def backward_3d_plan(volume, sinogram, plan, mode="reference"):
     """volume += B sinogram; mode 'reference' (the reference's projection), 'matched' (the
     forward model's projection) or 'fdk' (matched, weighted by (DSO / depth)^2)."""
-    m = {"reference": BP3D_REFERENCE, "matched": BP3D_MATCHED, "fdk": BP3D_FDK}.get(mode)
+    m = _BP3D_MODES.get(mode)
     if m is None:
         raise ValueError(f"unknown backprojection mode {mode!r}")
     vc, sc, b = _check_pair_3d(volume, sinogram, plan, "backward_3d")
@@ -491,3 +521,86 @@ def backward_3d(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi
     p = _plan3d_for(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
                     det_spacing, det_offset, det_rotation, DSO, DSD, COR)
     backward_3d_plan(x, sinogram, p)
+
+
+# ----------------------------------------------------------------------------- serving --
+
+def fan2d_bp_table(angles, DSO, DSD) -> np.ndarray:
+    """(A, 4) FP32 backprojection table (-sin, cos, DSD, DSO), as the 2D plan builds it."""
+    ang = _f64(angles)
+    A = ang.size
+    dso, dsd = _f64(DSO, A), _f64(DSD, A)
+    out = np.zeros((A, 4), dtype=np.float32)
+    check(_lib.cmi_fan2d_bp_table(A, ang.ctypes.data_as(_dp), dso.ctypes.data_as(_dp),
+                                  dsd.ctypes.data_as(_dp),
+                                  out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out
+
+
+def stage_projections(src, dst, row_scale=None, scale=1.0):
+    """dst (A, D + 4) or (A, V + 4, U + 4) := zero-bordered scale * row_scale[a] * src, where
+    src is FP32 (A, D) / (A, V, U) or a wider buffer whose first U entries per row are used
+    (for instance an inverse-FFT output of length L >= U). Enqueued on the current stream."""
+    _require_device_tensor(dst, "destination")
+    if src.dtype != torch.float32 or src.device != dst.device:
+        raise TypeError("source must be FP32 on the destination's device")
+    if src.stride(-1) != 1:
+        raise TypeError("source rows must be contiguous")
+    if dst.dim() == 2:
+        A, Dp = dst.shape
+        V, U, vpad = 1, Dp - 4, 0
+        if src.dim() != 2 or src.shape[0] != A or src.shape[1] < U:
+            raise ValueError("source shape does not match the staged layout")
+        stride = src.stride(0)
+    elif dst.dim() == 3:
+        A, Vp, Up = dst.shape
+        V, U, vpad = Vp - 4, Up - 4, 2
+        if src.dim() != 3 or src.shape[0] != A or src.shape[1] != V or src.shape[2] < U:
+            raise ValueError("source shape does not match the staged layout")
+        stride = src.stride(1)
+        if src.stride(0) != V * stride:
+            raise ValueError("source angles must be evenly strided")
+    else:
+        raise ValueError("destination must be 2D or 3D")
+    rs = None
+    if row_scale is not None:
+        rs = row_scale.to(device=dst.device, dtype=torch.float32).contiguous()
+        if rs.numel() != A:
+            raise ValueError("row_scale needs one value per angle")
+    check(_lib.cmi_stage_projections(dst.device.index or 0, src.data_ptr(), stride, A, V, U,
+                                     vpad, rs.data_ptr() if rs is not None else None,
+                                     float(scale), dst.data_ptr(), tensor_code(dst),
+                                     stream_of(dst)))
+    return rs  # keep alive until the copy has run
+
+
+def bp_job_tiles(job: BPJob) -> int:
+    n = _u64()
+    check(_lib.cmi_bp_job_tiles(C.byref(job), C.byref(n)))
+    return int(n.value)
+
+
+class Server:
+    """Multi-job backprojection launcher (cmi_server): one persistent launch per call of
+    `backproject` over a list of BPJob descriptors."""
+
+    def __init__(self, device=0, max_jobs=256):
+        self._h = None
+        self.device = int(device)
+        self.max_jobs = int(max_jobs)
+        h = _vp()
+        check(_lib.cmi_server_create(self.device, self.max_jobs, C.byref(h)))
+        self._h = h
+
+    def backproject(self, jobs, vol_dtype, sino_dtype, stream=None):
+        n = len(jobs)
+        arr = (BPJob * n)(*jobs)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+        check(_lib.cmi_serve_backproject(self._h, arr, n, precision_code(vol_dtype),
+                                         precision_code(sino_dtype), stream))
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h is not None and _lib is not None:
+            _lib.cmi_server_destroy(h)

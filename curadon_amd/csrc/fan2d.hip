@@ -58,6 +58,11 @@ constexpr int kPad = 2;        // zero border of the staged image copies, in tex
 constexpr int kMaxBlock = 256; // threads per workgroup every kernel here is compiled for
 static_assert(kPad == kPadDet, "sinograms are staged with the backprojector's border");
 
+// Per-angle constants of the backprojector: (-sin, cos, DSD, DSO) (backward_2d.hpp:58-62).
+float4 bp2d_meta(double angle, double dso, double dsd) {
+    return make_float4((float)-std::sin(angle), (float)std::cos(angle), (float)dsd, (float)dso);
+}
+
 // ---------------------------------------------------------------- ray table ----------------
 
 __global__ __launch_bounds__(kMaxBlock) void k_ray_table(const AngleGeom *__restrict__ geo, int A,
@@ -412,8 +417,7 @@ CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vo
     std::vector<float> ageo(6 * A);
     for (int64_t a = 0; a < A; ++a) {
         geo[a] = fan2d::plan_angle(p->g, angles[a], DSO[a], DSD[a]);
-        meta[a] = make_float4((float)-std::sin(angles[a]), (float)std::cos(angles[a]),
-                              (float)DSD[a], (float)DSO[a]);
+        meta[a] = bp2d_meta(angles[a], DSO[a], DSD[a]);
         for (int d = 0; d < 2; ++d) {
             ageo[6 * a + d] = (float)geo[a].src[d];
             ageo[6 * a + 2 + d] = (float)geo[a].org[d];
@@ -477,10 +481,11 @@ CMI_API cmi_status cmi_fan2d_bp_table(uint64_t nangles, const double *angles, co
     if (!angles || !DSO || !DSD || !table)
         return fail(CMI_ERR_INVALID_ARGUMENT, "fan2d_bp_table: null argument");
     for (uint64_t a = 0; a < nangles; ++a) {
-        table[4 * a + 0] = (float)-std::sin(angles[a]);
-        table[4 * a + 1] = (float)std::cos(angles[a]);
-        table[4 * a + 2] = (float)DSD[a];
-        table[4 * a + 3] = (float)DSO[a];
+        const float4 m = bp2d_meta(angles[a], DSO[a], DSD[a]);
+        table[4 * a + 0] = m.x;
+        table[4 * a + 1] = m.y;
+        table[4 * a + 2] = m.z;
+        table[4 * a + 3] = m.w;
     }
     return CMI_OK;
 }

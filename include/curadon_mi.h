@@ -230,6 +230,85 @@ CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_
                                   const void *sinogram, int sino_dtype, uint64_t batch,
                                   void *stream);
 
+/* ============================================================ serving: multi-job launch == */
+
+/*
+ * Host helpers that build the backprojection tables a serving job carries, from the same
+ * geometry arguments as the plans (no counterpart in the reference, whose backprojectors
+ * rebuild them per call: backward_2d.hpp:58-62, backward_3d.hpp:110-178).
+ *   cmi_fan2d_bp_table: table[4a..4a+3] = (-sin a, cos a, DSD[a], DSO[a]) in FP32.
+ *   cmi_cone3d_bp_matrices: 12 floats per angle, the 3x4 matrix of `mode` (cmi_bp3d_mode)
+ *   that cmi_backward_3d uses for the same description.
+ */
+CMI_API cmi_status cmi_fan2d_bp_table(uint64_t nangles, const double *angles, const double *DSO,
+                                      const double *DSD, float *table);
+CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *desc, int mode, float *out);
+
+/*
+ * Copies filtered projections into the zero-bordered layout the serving backprojector reads,
+ * converting to `dst_dtype`:
+ *   dst[a][vp][up] = scale * row_scale[a] * src[(a * V + vp - vpad) * src_stride + up - 2]
+ * for vp - vpad in [0, V) and up - 2 in [0, U), and 0 elsewhere; dst is [A][V + 2 vpad][U + 4].
+ * Fan-beam jobs pass V = 1, vpad = 0 ([A][D + 4]); cone-beam jobs V = panel rows, vpad = 2.
+ * `src` is FP32 device memory (for instance the output of an inverse real FFT of length
+ * src_stride >= U); row_scale may be null (all ones). Enqueued on `stream`.
+ */
+CMI_API cmi_status cmi_stage_projections(int device, const float *src, uint64_t src_stride,
+                                         uint64_t A, uint64_t V, uint64_t U, uint32_t vpad,
+                                         const float *row_scale, float scale, void *dst,
+                                         int dst_dtype, void *stream);
+
+/* Job kinds of the serving backprojector. */
+typedef enum cmi_job_kind { CMI_JOB_FAN2D = 0, CMI_JOB_CONE3D = 1 } cmi_job_kind;
+
+/*
+ * One backprojection job of a multi-job launch: volume += B sinogram over the tile range
+ * [tile_begin, tile_end) of the job's volume. Fan-beam tiles are 64 x 4 pixels (row-major
+ * over ceil(nx / 64) x ceil(ny / 4)); cone-beam tiles are 16 x 16 x 8 voxels (x fastest, then
+ * y, then z blocks). The arithmetic of every voxel equals cmi_backward_2d_weighted /
+ * cmi_backward_3d for the same tables, so a job's result does not depend on the other jobs of
+ * the launch or on how its tiles are split between launches.
+ *   sino    staged projections: [nangles][nu + 4] (fan) or [nangles][nv + 4][nu + 4] (cone)
+ *   table   device copy of cmi_fan2d_bp_table / cmi_cone3d_bp_matrices output
+ *   volume  [ny][nx] / [nz][ny][nx], accumulated into
+ *   mode    fan: cmi_bp2d_weighting; cone: cmi_bp3d_mode (the mode the matrices were made for)
+ *   fan only: sx, sy pixel spacing, ox, oy volume offset, inv_du = 1 / detector spacing
+ * No counterpart in the reference, which runs one operator call per image.
+ */
+typedef struct cmi_bp_job {
+    int32_t kind;
+    int32_t mode;
+    const void *sino;
+    const void *table;
+    void *volume;
+    uint32_t nx, ny, nz;
+    uint32_t nu, nv, nangles;
+    float sx, sy, ox, oy, inv_du;
+    uint32_t tile_begin, tile_end;
+} cmi_bp_job;
+
+/* Number of tiles of a job's volume (independent of tile_begin / tile_end). */
+CMI_API cmi_status cmi_bp_job_tiles(const cmi_bp_job *job, uint64_t *tiles);
+
+typedef struct cmi_server cmi_server;
+
+/* A multi-job launcher on `device`: owns a ring of device job tables (max_jobs entries each)
+ * and tile counters. */
+CMI_API cmi_status cmi_server_create(int device, uint32_t max_jobs, cmi_server **server);
+CMI_API cmi_status cmi_server_destroy(cmi_server *server);
+
+/*
+ * One persistent launch over `njobs` jobs whose volumes are stored as vol_dtype and sinograms
+ * as sino_dtype: a grid sized to the device's resident capacity pulls tiles from an atomic
+ * counter (zeroed by a fill kernel enqueued before it) until all tiles of all jobs are done.
+ * Tiles are handed out in job order, so callers list the costliest tiles first. Enqueued on
+ * `stream`; the job array is copied, the caller may reuse it on return. Jobs of one launch
+ * must write disjoint volume regions.
+ */
+CMI_API cmi_status cmi_serve_backproject(cmi_server *server, const cmi_bp_job *jobs,
+                                         uint32_t njobs, int vol_dtype, int sino_dtype,
+                                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif

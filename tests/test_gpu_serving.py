@@ -1,0 +1,167 @@
+"""GPU tests of the continuous-batching FBP server (curadon_amd.serving) and of the native
+multi-job backprojector behind it.
+
+Job recipe (fan and cone): DSO = 2 n, DSD = 3 n (isocentre-to-detector n), A = 2 n angles over
+2 pi; fan detector ceil(sqrt(2) n) cells, cone panel 1.5 n x 1.5 n. Raw projections are the
+FP32 forward projections of seeded ellipse / ellipsoid phantoms.
+
+Bars:
+  * served volume == curadon_amd.fbp with the same storage precision, bit for bit;
+  * a job's volume is bit-identical alone, in a mix, and split over many launches;
+  * FP16 storage: rel-L2 <= 1e-3 against the FP64 pipeline (FP64 filtering + the oracle's
+    weighted backprojector); BF16 storage: rel-L2 <= 5e-3;
+  * admission never holds more than the HBM budget, and jobs that arrive while others run
+    join their launches.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import curadon_amd as cm
+from curadon_amd import phantom
+from curadon_amd import reconstruction as F
+from curadon_amd.serving import FBPJob, FBPServer, footprint
+from geoms import cone, fan
+
+pytestmark = pytest.mark.gpu
+
+
+def fan_case(n, seed, storage=torch.float16):
+    rec = fan(n, int(math.ceil(math.sqrt(2) * n)), 2 * n, 2.0 * n, 3.0 * n)
+    g = cm.FanGeometry(DSD=rec.DSD, DSO=rec.DSO, angles=rec.angles, vol_shape=rec.vol_shape,
+                       det_shape=rec.det_shape, vol_prec=storage, sino_prec=storage)
+    g32 = cm.FanGeometry(DSD=rec.DSD, DSO=rec.DSO, angles=rec.angles, vol_shape=rec.vol_shape,
+                         det_shape=rec.det_shape)
+    img = torch.tensor(phantom.ellipses_2d(n, seed=seed), dtype=torch.float32, device="cuda")
+    raw = cm.forward(img, g32)
+    return rec, g, raw
+
+
+def cone_case(n, seed, storage=torch.float16):
+    p = int(1.5 * n)
+    rec = cone(n, p, p, 2 * n, 2.0 * n, 3.0 * n)
+    kw = dict(DSD=rec.DSD, DSO=rec.DSO, angles=rec.angles, vol_shape=rec.vol_shape,
+              det_shape=rec.det_shape)
+    g = cm.ConeGeometry(**kw, vol_prec=storage, sino_prec=storage)
+    g32 = cm.ConeGeometry(**kw)
+    vol = torch.tensor(phantom.ellipsoids_3d(n, seed=seed), device="cuda")
+    raw = cm.forward(vol, g32)
+    return rec, g, raw
+
+
+def jobs_of(cases, arrivals=None):
+    out = []
+    for i, (_, g, raw) in enumerate(cases):
+        out.append(FBPJob(geom=g, projections=raw, arrival=0.0 if arrivals is None
+                          else arrivals[i], tag=f"job{i}"))
+    torch.cuda.synchronize()
+    return out
+
+
+@pytest.fixture(scope="module")
+def mix(gpu):
+    return [fan_case(64, 1), cone_case(32, 2), fan_case(128, 3), cone_case(48, 4),
+            fan_case(96, 5), fan_case(64, 6)]
+
+
+def test_served_equals_plain_fbp(mix):
+    srv = FBPServer()
+    jobs = jobs_of(mix)
+    st = srv.run(jobs)
+    print(f"served {st['jobs']} jobs in {srv.rounds} launches, p50 {st['latency_ms_p50']:.2f} ms")
+    for (rec, g, raw), j in zip(mix, jobs):
+        ref = cm.fbp(raw, g)
+        assert j.volume.dtype == torch.float16
+        assert torch.equal(j.volume, ref), j.tag
+
+
+def test_alone_equals_mixed_and_split(mix):
+    target = 2  # the 128^2 fan job
+    alone = jobs_of([mix[target]])
+    FBPServer().run(alone)
+    mixed = jobs_of(mix, arrivals=[0.0, 0.0, 0.001, 0.0, 0.002, 0.0005])
+    FBPServer().run(mixed)
+    # a round budget of one tile per job: every job is split over many launches
+    split = jobs_of(mix)
+    srv = FBPServer(round_cost=1.0, oldest_share=0.0)
+    srv.run(split)
+    assert srv.rounds >= 64
+    assert torch.equal(alone[0].volume, mixed[target].volume)
+    assert torch.equal(alone[0].volume, split[target].volume)
+    for a, b in zip(mixed, split):
+        assert torch.equal(a.volume, b.volume), a.tag
+
+
+@pytest.mark.parametrize("storage,tol", [(torch.float16, 1e-3), (torch.bfloat16, 5e-3)])
+def test_error_against_fp64_pipeline(oracle, gpu, storage, tol):
+    cases = [fan_case(128, 7, storage), cone_case(48, 8, storage)]
+    jobs = jobs_of(cases)
+    FBPServer(storage=storage).run(jobs)
+    for (rec, g, raw), j in zip(cases, jobs):
+        q = F.fbp_filter(raw.double().cpu(), g).numpy()
+        if isinstance(g, cm.FanGeometry):
+            ref = oracle.bp2d(rec, q, wpow=2)
+        else:
+            ref = oracle.bp3d(rec, q, fdk=True)
+        got = j.volume.double().cpu().numpy()
+        r = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        print(f"{type(g).__name__} {tuple(g.vol_shape)} {storage}: rel-L2 vs FP64 {r:.3e}")
+        assert r <= tol
+
+
+def test_admission_respects_budget(mix):
+    jobs = jobs_of(mix)
+    biggest = max(sum(footprint(j, torch.float16)) for j in jobs)
+    srv = FBPServer(hbm_budget=biggest + 1024)
+    srv.run(jobs)
+    assert srv.max_used_seen <= srv.hbm_budget
+    assert all(j.t_done >= j.t_admit for j in jobs)
+    for (rec, g, raw), j in zip(mix, jobs):
+        assert torch.equal(j.volume, cm.fbp(raw, g)), j.tag
+    with pytest.raises(MemoryError):
+        FBPServer(hbm_budget=1024).run(jobs_of(mix[:1]))
+
+
+def test_late_arrivals_join_running_launches(gpu):
+    big = cone_case(96, 9)
+    small = fan_case(64, 10)
+    jobs = jobs_of([big, small], arrivals=[0.0, 0.002])
+    srv = FBPServer(round_cost=2048 * 192 * 2.0, oldest_share=0.0)
+    srv.run(jobs)
+    first_big = next(i for i, r in enumerate(srv.launch_log) if any(j == 0 for j, _, _ in r))
+    shared = [i for i, r in enumerate(srv.launch_log)
+              if {j for j, _, _ in r} == {0, 1}]
+    print(f"{srv.rounds} launches; big job from launch {first_big}, both jobs in {len(shared)}")
+    assert jobs[1].t_admit > 0.0
+    assert shared and shared[0] > first_big
+    assert jobs[1].t_done < jobs[0].t_done
+    assert torch.equal(jobs[0].volume, cm.fbp(big[2], big[1]))
+    assert torch.equal(jobs[1].volume, cm.fbp(small[2], small[1]))
+
+
+def test_native_job_validation(gpu):
+    from curadon_amd import _lib
+    srv = _lib.Server(0, max_jobs=2)
+    v = torch.zeros(8, 8, device="cuda", dtype=torch.float16)
+    s = torch.zeros(4, 11, device="cuda", dtype=torch.float16)
+    t = torch.zeros(4, 4, device="cuda")
+    j = _lib.BPJob(kind=_lib.JOB_FAN2D, mode=_lib.BP2D_FBP, sino=s.data_ptr(),
+                   table=t.data_ptr(), volume=v.data_ptr(), nx=8, ny=8, nz=1, nu=7, nv=1,
+                   nangles=4, sx=1.0, sy=1.0, ox=0.0, oy=0.0, inv_du=1.0, tile_begin=0,
+                   tile_end=2)
+    assert _lib.bp_job_tiles(j) == 2
+    srv.backproject([j], torch.float16, torch.float16)
+    bad = _lib.BPJob.from_buffer_copy(j)
+    bad.tile_end = 3
+    with pytest.raises(ValueError):
+        srv.backproject([bad], torch.float16, torch.float16)
+    with pytest.raises(ValueError):
+        srv.backproject([j, j, j], torch.float16, torch.float16)
+    host = torch.zeros(8, 8, dtype=torch.float16)
+    bad = _lib.BPJob.from_buffer_copy(j)
+    bad.volume = host.data_ptr()
+    with pytest.raises(AttributeError):
+        srv.backproject([bad], torch.float16, torch.float16)
+    torch.cuda.synchronize()
