@@ -118,6 +118,123 @@ def batch_case(torch, cm, phantom, np, stream, reps):
             "fwd_adj_images_per_s": 16e3 / (t["fwd"] + t["adj"])}
 
 
+# ----------------------------------------------------------------------------- serving ---
+# FBP / FDK job mix: 70 % fan-beam jobs, n in {128, 256, 512, 1024, 2048}, D = ceil(sqrt(2) n);
+# 30 % cone-beam jobs, n in {128, 256, 384}, panel 1.5 n x 1.5 n; A = 2 n angles over 2 pi;
+# DSO = 2 n, DSD = 3 n. Each job: cosine weights, ramp filter, weighted backprojection.
+SERVING_FAN = (128, 256, 512, 1024, 2048)
+SERVING_CONE = (128, 256, 384)
+
+
+def serving_geometry(cm, np, kind, n, storage):
+    import math
+    A = 2 * n
+    angles = np.linspace(0, 2 * np.pi, A, endpoint=False)
+    if kind == "fan":
+        return cm.FanGeometry(DSD=3.0 * n, DSO=2.0 * n, angles=angles, vol_shape=(n, n),
+                              det_shape=int(math.ceil(math.sqrt(2) * n)), vol_prec=storage,
+                              sino_prec=storage)
+    p = (3 * n) // 2
+    return cm.ConeGeometry(DSD=3.0 * n, DSO=2.0 * n, angles=angles, vol_shape=(n, n, n),
+                           det_shape=(p, p), vol_prec=storage, sino_prec=storage)
+
+
+def serving_case(torch, cm, phantom, np, n_jobs, seed=4):
+    """Closed-loop throughput of the continuous-batching server (FP16 storage) against plain
+    per-call fbp (FP32 and FP16 storage, one synchronised call per job), then an open-loop run
+    with Poisson arrivals at 70 % of the measured server capacity for p50 / p99 latency."""
+    from curadon_amd.serving import FBPJob, FBPServer
+    rng = np.random.default_rng(seed)
+    classes = [("fan", n) for n in SERVING_FAN] + [("cone", n) for n in SERVING_CONE]
+    prob = [0.7 / len(SERVING_FAN)] * len(SERVING_FAN) + [0.3 / len(SERVING_CONE)] * len(SERVING_CONE)
+    draw = rng.choice(len(classes), size=n_jobs, p=prob)
+    data = {}
+    for ci in sorted(set(draw.tolist())):
+        kind, n = classes[ci]
+        g32 = serving_geometry(cm, np, kind, n, 32)
+        g16 = serving_geometry(cm, np, kind, n, torch.float16)
+        if kind == "fan":
+            x = torch.tensor(phantom.ellipses_2d(n, seed=ci), dtype=torch.float32, device="cuda")
+        else:
+            x = torch.tensor(phantom.ellipsoids_3d(n, seed=ci), device="cuda")
+        raw = cm.forward(x, g32)
+        del x
+        data[ci] = (kind, n, g32, g16, raw)
+    torch.cuda.synchronize()
+    log(f"bench: serving inputs ready ({n_jobs} jobs, {len(data)} classes)")
+
+    def jobs(arrivals=None):
+        return [FBPJob(geom=data[c][3], projections=data[c][4], tag=f"{data[c][0]}{data[c][1]}",
+                       arrival=0.0 if arrivals is None else float(arrivals[i]))
+                for i, c in enumerate(draw.tolist())]
+
+    # warm-up: FFT plans and kernels of every class, both paths
+    for c in data:
+        cm.fbp(data[c][4], data[c][2])
+        cm.fbp(data[c][4], data[c][3])
+    FBPServer().run([FBPJob(geom=data[c][3], projections=data[c][4]) for c in data])
+    torch.cuda.synchronize()
+
+    def plain(which):
+        t0 = time.perf_counter()
+        for c in draw.tolist():
+            cm.fbp(data[c][4], data[c][which])
+            torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    t_plain32 = plain(2)
+    t_plain16 = plain(3)
+    closed = jobs()
+    srv = FBPServer(hbm_budget=96 << 30)
+    st_closed = srv.run(closed)
+    log(f"bench: serving closed loop {st_closed['makespan_s']:.3f} s in {srv.rounds} launches; "
+        f"plain FP32 {t_plain32:.3f} s, plain FP16 {t_plain16:.3f} s")
+    lam = 0.7 * st_closed["jobs_per_s"]
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=n_jobs))
+    arrivals -= arrivals[0]
+    opened = jobs(arrivals)
+    srv2 = FBPServer(hbm_budget=96 << 30)
+    st_open = srv2.run(opened)
+    per_class = {}
+    for j in opened:
+        per_class.setdefault(j.tag, []).append(j.latency * 1e3)
+    # storage error of each class: served FP16 volume against the plain FP32 operator chain
+    errors = {}
+    for j, c in zip(closed, draw.tolist()):
+        if j.tag in errors:
+            continue
+        ref = cm.fbp(data[c][4], data[c][2]).double()
+        errors[j.tag] = float(torch.linalg.vector_norm(j.volume.double() - ref) /
+                              torch.linalg.vector_norm(ref))
+    log(f"bench: serving open loop p50 {st_open['latency_ms_p50']:.2f} ms, "
+        f"p99 {st_open['latency_ms_p99']:.2f} ms at {lam:.1f} jobs/s")
+    return {
+        "workload": "FBP/FDK job mix: 70% fan (n in 128..2048, D = ceil(sqrt2 n)), 30% cone "
+                    "(n in 128..384, panel 1.5n); A = 2n over 2pi, DSO 2n, DSD 3n; FP32 raw "
+                    "projections resident in HBM; cosine weight + ramp filter + weighted "
+                    "backprojection",
+        "jobs": n_jobs,
+        "mix": {t: int(sum(1 for j in closed if j.tag == t)) for t in sorted({j.tag for j in closed})},
+        "storage": "f16 (filtered projections and volume), FP32 filtering and accumulation",
+        "closed_loop": {
+            "serving_jobs_per_s": st_closed["jobs_per_s"],
+            "plain_f32_jobs_per_s": n_jobs / t_plain32,
+            "plain_f16_jobs_per_s": n_jobs / t_plain16,
+            "speedup_vs_plain_f32": (n_jobs / st_closed["makespan_s"]) / (n_jobs / t_plain32),
+            "speedup_vs_plain_f16": (n_jobs / st_closed["makespan_s"]) / (n_jobs / t_plain16),
+            "launches": srv.rounds,
+        },
+        "open_loop": dict(st_open, arrival_rate_per_s=lam, launches=srv2.rounds,
+                          max_active=srv2.max_active_seen,
+                          per_class_p50_ms={k: float(np.percentile(v, 50))
+                                            for k, v in sorted(per_class.items())},
+                          per_class_p99_ms={k: float(np.percentile(v, 99))
+                                            for k, v in sorted(per_class.items())}),
+        "rel_l2_vs_plain_f32": errors,
+    }
+
+
+
 def copy_calibration(torch, mib, reps):
     """Plain device-to-device copies of `mib` MiB, for scaling HBM byte counters."""
     n = mib * (1 << 20) // 4
@@ -183,6 +300,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary workloads (256^3 cone beam, batched 512^2)")
+    ap.add_argument("--serving-jobs", type=int, default=40,
+                    help="jobs in the FBP serving workload (0 skips it)")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the FBP serving workload and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run plain device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -200,6 +321,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(torch, args.copy_calibration, max(1, args.steps))
+        return
+    if args.serving_only:
+        print(json.dumps(serving_case(torch, cm, phantom, np, max(1, args.serving_jobs))),
+              flush=True)
         return
 
     log("bench: building plan and inputs")
@@ -257,6 +382,11 @@ def main():
                  "batch2d": batch_case(torch, cm, phantom, np, stream, 3)}
         log(f"bench: cone3d {extra['cone3d']['fwd3d']['ms']:.2f} / {extra['cone3d']['bp3d']['ms']:.2f} ms")
 
+    serving = None
+    if args.serving_jobs > 0:
+        log("bench: FBP serving workload")
+        serving = serving_case(torch, cm, phantom, np, args.serving_jobs)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle FP32 port)")
@@ -311,6 +441,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "extra": extra,
+        "serving": serving,
         "device": {"name": props.name, "cus": props.multi_processor_count,
                    "gcn_arch": getattr(props, "gcnArchName", "")},
     }

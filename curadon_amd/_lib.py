@@ -97,7 +97,8 @@ class Plan3DInfo(C.Structure):
 
 class BPJob(C.Structure):
     _fields_ = [
-        ("kind", C.c_int32), ("mode", C.c_int32), ("sino", _vp), ("table", _vp),
+        ("kind", C.c_int32), ("mode", C.c_int32), ("columns", C.c_int32), ("sino", _vp),
+        ("table", _vp),
         ("volume", _vp), ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
         ("nu", C.c_uint32), ("nv", C.c_uint32), ("nangles", C.c_uint32), ("sx", C.c_float),
         ("sy", C.c_float), ("ox", C.c_float), ("oy", C.c_float), ("inv_du", C.c_float),
@@ -128,12 +129,14 @@ SIGNATURES = {
     "cmi_backward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_adjoint_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_fan2d_bp_table": (_i32, [_u64, _dp, _dp, _dp, C.POINTER(C.c_float)]),
-    "cmi_cone3d_bp_matrices": (_i32, [C.POINTER(Cone3DDesc), _i32, C.POINTER(C.c_float)]),
+    "cmi_cone3d_bp_matrices": (_i32, [C.POINTER(Cone3DDesc), _i32, C.POINTER(C.c_float),
+                                      C.POINTER(C.c_int)]),
     "cmi_stage_projections": (_i32, [_i32, _vp, _u64, _u64, _u64, _u64, C.c_uint32, _vp,
                                      C.c_float, _vp, _i32, _vp]),
     "cmi_bp_job_tiles": (_i32, [C.POINTER(BPJob), _u64p]),
     "cmi_server_create": (_i32, [_i32, C.c_uint32, C.POINTER(_vp)]),
     "cmi_server_destroy": (_i32, [_vp]),
+    "cmi_server_get_capacity": (_i32, [_vp, C.POINTER(C.c_uint32)]),
     "cmi_serve_backproject": (_i32, [_vp, C.POINTER(BPJob), C.c_uint32, _i32, _i32, _vp]),
 }
 
@@ -411,12 +414,16 @@ class plan_3d:
     def handle(self):
         return self._h
 
-    def bp_matrices(self, mode="fdk") -> np.ndarray:
-        """(A, 3, 4) FP32 projection matrices cmi_backward_3d uses in `mode`."""
+    def bp_matrices(self, mode="fdk", with_layout=False):
+        """(A, 3, 4) FP32 projection matrices cmi_backward_3d uses in `mode`; with_layout also
+        returns whether their U and W rows are free of y (the backprojectors then run along
+        y)."""
         out = np.zeros((self.nangles, 3, 4), dtype=np.float32)
+        yinv = C.c_int()
         check(_lib.cmi_cone3d_bp_matrices(C.byref(self._desc), _BP3D_MODES[mode],
-                                          out.ctypes.data_as(C.POINTER(C.c_float))))
-        return out
+                                          out.ctypes.data_as(C.POINTER(C.c_float)),
+                                          C.byref(yinv)))
+        return (out, bool(yinv.value)) if with_layout else out
 
     def info(self) -> dict:
         i = Plan3DInfo()
@@ -591,6 +598,9 @@ class Server:
         h = _vp()
         check(_lib.cmi_server_create(self.device, self.max_jobs, C.byref(h)))
         self._h = h
+        cap = C.c_uint32()
+        check(_lib.cmi_server_get_capacity(self._h, C.byref(cap)))
+        self.capacity = int(cap.value)
 
     def backproject(self, jobs, vol_dtype, sino_dtype, stream=None):
         n = len(jobs)

@@ -12,7 +12,13 @@
 namespace cmi {
 
 constexpr int kPadDet = 2; // zero border of staged sinograms and panels, in cells
-constexpr int kZB = 8;     // z voxels per thread in the 3D backprojector
+constexpr int kZB = 8;     // z voxels per thread in the 3D backprojector (general geometry)
+constexpr int kYB = 8;     // y voxels per thread in the 3D backprojector (y-invariant U, W)
+
+// Linear interpolation of a pair at weight f, FP32 arithmetic.
+__device__ __forceinline__ float lerp_pair(const Pair &p, float f) {
+    return fmaf(f, p.hi - p.lo, p.lo);
+}
 
 // Sum over angles for one pixel at world position (x, y) (backward_2d.hpp:58-96):
 //   sum_a w_a^WPOW lerp(sino[a], w_a eta_a + D/2),  w_a = DSD / (xi_a + DSO)
@@ -76,6 +82,45 @@ __device__ __forceinline__ void bp3d_column(rsrc_t R, const float *__restrict__ 
             Uh += m[2];
             Vh += m[6];
             Wh += m[10];
+        }
+    }
+}
+
+// Sum over angles for kYB consecutive voxels of one (x, z) row along y, for matrices whose U and
+// W rows do not depend on y (M[a][1] == M[a][9] == 0: circular source orbits about the y axis
+// with an untilted panel). The perspective divide, the u tap and weight and the distance
+// weight are then shared by the whole run; only V advances along y. (hx, hy, hz) is the centred
+// index of the run's first voxel. Semantics as bp3d_column.
+template <class SS, bool MATCHED, bool FDK>
+__device__ __forceinline__ void bp3d_yrun(rsrc_t R, const float *__restrict__ M, int A, int U,
+                                         int V, float hx, float hy, float hz,
+                                         float (&acc)[kYB]) {
+    const int Up = U + 2 * kPadDet, Vp = V + 2 * kPadDet;
+    const int panel = Up * Vp;
+    const int sRow = Up * (int)sizeof(typename SS::raw);
+    const float ulim = (float)U, vlim = (float)V;
+    int base = kPadDet * Up + kPadDet;
+    for (int a = 0; a < A; ++a, base += panel) {
+        const float *m = M + 12 * a; // wave-uniform: scalar loads
+        const float Uh = fmaf(m[0], hx, fmaf(m[2], hz, m[3]));
+        const float Wh = fmaf(m[8], hx, fmaf(m[10], hz, m[11]));
+        float Vh = fmaf(m[4], hx, fmaf(m[5], hy, fmaf(m[6], hz, m[7])));
+        const float rw = __builtin_amdgcn_rcpf(Wh);
+        const float u = __builtin_amdgcn_fmed3f(Uh * rw, -2.0f, ulim);
+        const float fu = __builtin_amdgcn_fractf(u);
+        const int col = base + (int)floorf(u);
+        float w = FDK ? rw * rw : 1.0f;
+        if (MATCHED) w = Wh > 0.0f ? w : 0.0f;
+#pragma unroll
+        for (int y = 0; y < kYB; ++y) {
+            const float v = __builtin_amdgcn_fmed3f(Vh * rw, -2.0f, vlim);
+            const float fv = __builtin_amdgcn_fractf(v);
+            const int o = col + __mul24((int)floorf(v), Up);
+            const float r0 = lerp_pair(buf_pair<SS>(R, o, 0), fu);
+            const float r1 = lerp_pair(buf_pair<SS>(R, o, sRow), fu);
+            const float s = fmaf(fv, r1 - r0, r0);
+            acc[y] = (MATCHED || FDK) ? fmaf(s, w, acc[y]) : acc[y] + s;
+            Vh += m[5];
         }
     }
 }

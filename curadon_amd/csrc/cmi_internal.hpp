@@ -85,6 +85,14 @@ __device__ __forceinline__ typename S::raw from_f32(float v) {
     return static_cast<typename S::raw>(v);
 }
 
+// floor(x) as int32 in one instruction (v_cvt_flr_i32_f32) for finite |x| < 2^31; the compiler
+// emits a floor and a conversion for (int)floorf(x).
+__device__ __forceinline__ int floor_i32(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // ------------------------------------------------------------------ buffer loads -------
 // Raw buffer resources give 32-bit offsets from a scalar base and hardware range checking:
 // a load that lies wholly beyond num_records bytes returns 0 instead of touching memory.
@@ -121,9 +129,10 @@ __device__ __forceinline__ float buf_load<BF16>(rsrc_t r, int elem_offset) {
     return __builtin_bit_cast(float, (uint32_t)u << 16);
 }
 
-// Two adjacent elements (offsets o and o + 1) in FP32. F32 pairs use one 8-byte load (dword
-// alignment suffices for multi-dword buffer loads); 16-bit pairs use two 2-byte loads because
-// a 4-byte load of a pair need not be dword aligned. `soff` is a wave-uniform byte offset.
+// Two adjacent elements (offsets o and o + 1) in FP32, with one load: 8 bytes for F32 pairs,
+// 4 bytes for 16-bit pairs. The load need not be aligned to its size (an F32 pair sits at any
+// dword, a 16-bit pair at any 2-byte boundary): the ROCm driver runs gfx9-family shaders in
+// unaligned memory-access mode. `soff` is a wave-uniform byte offset.
 struct Pair {
     float lo, hi;
 };
@@ -139,18 +148,45 @@ __device__ __forceinline__ Pair buf_pair<F32>(rsrc_t r, int elem_offset, int sof
 }
 template <>
 __device__ __forceinline__ Pair buf_pair<F16>(rsrc_t r, int elem_offset, int soff_bytes) {
-    const unsigned short a = __builtin_amdgcn_raw_buffer_load_b16(r, elem_offset * 2, soff_bytes, 0);
-    const unsigned short b =
-        __builtin_amdgcn_raw_buffer_load_b16(r, elem_offset * 2 + 2, soff_bytes, 0);
-    return {static_cast<float>(__builtin_bit_cast(_Float16, a)),
-            static_cast<float>(__builtin_bit_cast(_Float16, b))};
+    const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, elem_offset * 2, soff_bytes, 0);
+    return {static_cast<float>(__builtin_bit_cast(_Float16, (unsigned short)(v & 0xffffu))),
+            static_cast<float>(__builtin_bit_cast(_Float16, (unsigned short)(v >> 16)))};
 }
 template <>
 __device__ __forceinline__ Pair buf_pair<BF16>(rsrc_t r, int elem_offset, int soff_bytes) {
-    const unsigned short a = __builtin_amdgcn_raw_buffer_load_b16(r, elem_offset * 2, soff_bytes, 0);
-    const unsigned short b =
-        __builtin_amdgcn_raw_buffer_load_b16(r, elem_offset * 2 + 2, soff_bytes, 0);
-    return {__uint_as_float((uint32_t)a << 16), __uint_as_float((uint32_t)b << 16)};
+    const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, elem_offset * 2, soff_bytes, 0);
+    return {__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u)};
+}
+
+// Four consecutive elements (offsets o .. o + 3) in FP32 with one load (16 bytes for F32,
+// 8 for 16-bit storage); any element alignment.
+struct Quad {
+    float v[4];
+};
+
+template <class S>
+__device__ __forceinline__ Quad buf_quad(rsrc_t r, int elem_offset);
+
+template <>
+__device__ __forceinline__ Quad buf_quad<F32>(rsrc_t r, int elem_offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, elem_offset * 4, 0, 0);
+    const unsigned a = v[0], b = v[1], c = v[2], d = v[3];
+    return {{__uint_as_float(a), __uint_as_float(b), __uint_as_float(c), __uint_as_float(d)}};
+}
+template <>
+__device__ __forceinline__ Quad buf_quad<F16>(rsrc_t r, int elem_offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, elem_offset * 2, 0, 0);
+    const unsigned a = v[0], b = v[1];
+    auto h = [](unsigned short x) { return static_cast<float>(__builtin_bit_cast(_Float16, x)); };
+    return {{h((unsigned short)(a & 0xffffu)), h((unsigned short)(a >> 16)),
+             h((unsigned short)(b & 0xffffu)), h((unsigned short)(b >> 16))}};
+}
+template <>
+__device__ __forceinline__ Quad buf_quad<BF16>(rsrc_t r, int elem_offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, elem_offset * 2, 0, 0);
+    const unsigned a = v[0], b = v[1];
+    return {{__uint_as_float(a << 16), __uint_as_float(a & 0xffff0000u), __uint_as_float(b << 16),
+             __uint_as_float(b & 0xffff0000u)}};
 }
 
 // Byte size of n elements, for the range check of a resource.

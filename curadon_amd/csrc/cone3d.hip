@@ -38,6 +38,7 @@ struct cmi_plan3d {
     float *d_mref = nullptr;
     float *d_mmat = nullptr;
     float *d_mfdk = nullptr;
+    bool ycols[3] = {false, false, false}; // per cmi_bp3d_mode: U and W rows free of y
 };
 
 namespace {
@@ -280,7 +281,48 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp3d(typename SV::raw *__restrict
     }
 }
 
+// One thread per (ix, iz) and kYB consecutive y, for matrices whose U and W rows do not depend
+// on y (bp3d_yrun); a wavefront covers 64 consecutive x of one (z, y-run), so each y step writes
+// 64 consecutive voxels. vol += sum over angles.
+constexpr int kYRunX = 64, kYRunZ = 4;
+
+template <class SV, class SS, bool MATCHED, bool FDK>
+__global__ __launch_bounds__(kMaxBlock) void k_bp3d_y(typename SV::raw *__restrict__ vol,
+                                                      const typename SS::raw *__restrict__ ppad,
+                                                      const float *__restrict__ M, int nx, int ny,
+                                                      int nz, int U, int V, int A, float cx,
+                                                      float cy, float cz) {
+    const int ix = blockIdx.x * kYRunX + threadIdx.x;
+    const int nzq = (nz + kYRunZ - 1) / kYRunZ;
+    const int iz = (blockIdx.z % nzq) * kYRunZ + threadIdx.y;
+    const size_t b = blockIdx.z / nzq;
+    const int iy0 = blockIdx.y * kYB;
+    if (ix >= nx || iz >= nz) return;
+    const uint32_t stack = (uint32_t)(U + 2 * kPad) * (V + 2 * kPad) * A;
+    const rsrc_t R = make_rsrc(ppad + b * (size_t)stack, bytes_of<SS>(stack));
+    float acc[kYB];
+#pragma unroll
+    for (int y = 0; y < kYB; ++y) acc[y] = 0.0f;
+    bp3d_yrun<SS, MATCHED, FDK>(R, M, A, U, V, (float)ix - cx, (float)iy0 - cy, (float)iz - cz,
+                                acc);
+#pragma unroll
+    for (int y = 0; y < kYB; ++y) {
+        const int iy = iy0 + y;
+        if (iy < ny) {
+            const size_t i = ((b * nz + iz) * ny + iy) * (size_t)nx + ix;
+            vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc[y]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- host helpers -----------
+
+// True when no angle's U or W row depends on y, so k_bp3d_y applies.
+bool y_invariant(const std::vector<float> &m, int64_t A) {
+    for (int64_t a = 0; a < A; ++a)
+        if (m[12 * a + 1] != 0.0f || m[12 * a + 9] != 0.0f) return false;
+    return true;
+}
 
 // Projection matrices for the backprojector, rows (U, V, W) over h = (ix-cx, iy-cy, iz-cz, 1),
 // scaled so that W = 1 at the volume centre.
@@ -491,6 +533,9 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
     for (int64_t a = 0; a < A; ++a) fgeo[a] = cone3d::fwd_angle(g, d->phi[a], d->theta[a], d->psi[a]);
     std::vector<float> mref, mmat, mfdk;
     bp_matrices(g, d->phi, d->theta, d->psi, A, mref, mmat, mfdk);
+    p->ycols[CMI_BP3D_REFERENCE] = y_invariant(mref, A);
+    p->ycols[CMI_BP3D_MATCHED] = y_invariant(mmat, A);
+    p->ycols[CMI_BP3D_FDK] = y_invariant(mfdk, A);
 
     DeviceGuard guard;
     CMI_TRY(guard.enter(device));
@@ -540,7 +585,8 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
     return CMI_OK;
 }
 
-CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *d, int mode, float *out) {
+CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *d, int mode, float *out,
+                                          int *y_invariant_out) {
     clear_error();
     CMI_TRY(check_desc(d, "cone3d_bp_matrices"));
     if (!out) return fail(CMI_ERR_INVALID_ARGUMENT, "cone3d_bp_matrices: null output");
@@ -551,6 +597,7 @@ CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *d, int mode, fl
     bp_matrices(g, d->phi, d->theta, d->psi, (int64_t)d->nangles, mref, mmat, mfdk);
     const std::vector<float> &m = mode == CMI_BP3D_MATCHED ? mmat : mode == CMI_BP3D_FDK ? mfdk : mref;
     std::copy(m.begin(), m.end(), out);
+    if (y_invariant_out) *y_invariant_out = y_invariant(m, (int64_t)d->nangles) ? 1 : 0;
     return CMI_OK;
 }
 
@@ -650,14 +697,25 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
         dim3 pb(256), pg((unsigned)((Up + 255) / 256), (unsigned)Vp, (unsigned)(A * batch));
         hipLaunchKernelGGL((k_stage_panel<SS>), pg, pb, 0, s, (const RS *)sinogram,
                            (RS *)scratch, U, V, A);
-        const int nzb = (nz + kZB - 1) / kZB;
-        dim3 bb(16, 16), bg((unsigned)((nx + 15) / 16), (unsigned)((ny + 15) / 16),
-                            (unsigned)(nzb * batch));
-        auto kern = mode == CMI_BP3D_MATCHED ? k_bp3d<SV, SS, true, false>
-                    : mode == CMI_BP3D_FDK   ? k_bp3d<SV, SS, true, true>
-                                             : k_bp3d<SV, SS, false, false>;
-        hipLaunchKernelGGL(kern, bg, bb, 0, s, (typename SV::raw *)volume, (const RS *)scratch, M,
-                           nx, ny, nz, U, V, A, cx, cy, cz);
+        if (plan->ycols[mode]) {
+            const int nzq = (nz + kYRunZ - 1) / kYRunZ;
+            dim3 bb(kYRunX, kYRunZ), bg((unsigned)((nx + kYRunX - 1) / kYRunX),
+                                        (unsigned)((ny + kYB - 1) / kYB), (unsigned)(nzq * batch));
+            auto kern = mode == CMI_BP3D_MATCHED ? k_bp3d_y<SV, SS, true, false>
+                        : mode == CMI_BP3D_FDK   ? k_bp3d_y<SV, SS, true, true>
+                                                 : k_bp3d_y<SV, SS, false, false>;
+            hipLaunchKernelGGL(kern, bg, bb, 0, s, (typename SV::raw *)volume, (const RS *)scratch,
+                               M, nx, ny, nz, U, V, A, cx, cy, cz);
+        } else {
+            const int nzb = (nz + kZB - 1) / kZB;
+            dim3 bb(16, 16), bg((unsigned)((nx + 15) / 16), (unsigned)((ny + 15) / 16),
+                                (unsigned)(nzb * batch));
+            auto kern = mode == CMI_BP3D_MATCHED ? k_bp3d<SV, SS, true, false>
+                        : mode == CMI_BP3D_FDK   ? k_bp3d<SV, SS, true, true>
+                                                 : k_bp3d<SV, SS, false, false>;
+            hipLaunchKernelGGL(kern, bg, bb, 0, s, (typename SV::raw *)volume,
+                               (const RS *)scratch, M, nx, ny, nz, U, V, A, cx, cy, cz);
+        }
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_sino2d(const typename S::ra
 template <class SI, bool YDOM>
 __device__ __forceinline__ float sample2d(rsrc_t R, float x, float y, int rstride, int soff1,
                                          int cofs) {
-    const int ix = (int)floorf(x), iy = (int)floorf(y);
+    const int ix = floor_i32(x), iy = floor_i32(y);
     const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
     const int o = (YDOM ? __mul24(iy, rstride) + ix : __mul24(ix, rstride) + iy) + cofs;
     const Pair p0 = buf_pair<SI>(R, o, 0);     // YDOM: (ix, iy),(ix+1, iy)  else (ix, iy),(ix, iy+1)
@@ -154,63 +155,238 @@ __device__ __forceinline__ float sample2d(rsrc_t R, float x, float y, int rstrid
     return fmaf(fb, u1 - u0, u0);
 }
 
-// March of one ray: sample j sits at m + (j - N/2) v. Samples are summed in blocks of
-// kSumBlock into partial sums that are then added to the total, which keeps the FP32 rounding
-// of a ~1000-sample sum near that of a 32-term sum; inside a block two partials alternate so
-// that each lane has two samples' loads in flight.
-constexpr int kSumBlock = 32;
+// Row-synchronous march through LDS bands.
+//
+// Sample j of a ray sits at m + (j - N/2) v, and the step v is +-1 along the ray's dominant
+// axis (forward_2d.hpp:63-78 normalise the step by its largest component), so a ray has one
+// sample per dominant row. A workgroup holds adjacent detector cells of one angle and walks the
+// dominant rows tau in increasing order: in row tau every lane whose dominant axis is the
+// pass's takes its sample on that row, jf = s (tau - floor(md)) relative to its midpoint (md
+// its midpoint coordinate along the dominant axis, s the step's sign), if jf lies in its range
+// [q.z, q.z + N - 1]. The rows come in bands of kBand, aligned to multiples of kBand: the
+// workgroup stages rows [t0 - 1, t0 + kBand + 1] of the copy whose rows run along the dominant
+// axis, over the columns its samples in the band can touch, into LDS (rows of kPitch floats,
+// filled with 4-element row loads) and interpolates from there. A band wider than kPitch is
+// sampled from global memory instead, with the same arithmetic. Bands in which every lane of
+// the workgroup has a sample on every row skip the per-sample range test.
+//
+// Staged cells outside the copy (band corners beyond its edge) hold whatever the range-checked
+// loads return; no sample of a ray reads them, because every tap of a sample lies inside the
+// copy's zero border.
+//
+// Summation: samples on even and odd rows go to two partial sums, which are added to the total
+// at the end of every band. Bands and parities are absolute row numbers, so a ray's sum does
+// not depend on which rays share its workgroup (and so not on the launch shape). Positions are
+// the same FP32 expressions as the adjoint's.
+constexpr int kBand = 32;              // rows per band and per partial sum
+constexpr int kBandRows = kBand + 3;   // staged rows: one above the band, two below
+// LDS row pitch in floats: a multiple of 4 (16-byte row stores) and below 255, so that one
+// address reaches a tap pair and the pair below it with two ds_read2_b32 immediate offsets.
+// 128 rays at up to 1.1 texels per cell plus a band's slope and the tap margins fit.
+constexpr int kPitch = 192;
+constexpr int kFwdRays = 128;          // rays (threads) per workgroup of the forward
+constexpr int kMaxWaves = kFwdRays / 64;
 
+__device__ __forceinline__ int wave_min(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Workgroup-wide (min a, max b, min c); every thread of the workgroup calls it.
+__device__ __forceinline__ int3 block_reduce(int a, int b, int c, int3 *red, int nwaves) {
+    a = wave_min(a);
+    b = wave_max(b);
+    c = wave_min(c);
+    const int w = threadIdx.x >> 6;
+    __syncthreads(); // previous readers of red are done
+    if ((threadIdx.x & 63) == 0) red[w] = make_int3(a, b, c);
+    __syncthreads();
+    int3 r = red[0];
+    for (int i = 1; i < nwaves; ++i) {
+        r.x = min(r.x, red[i].x);
+        r.y = max(r.y, red[i].y);
+        r.z = min(r.z, red[i].z);
+    }
+    return make_int3(__builtin_amdgcn_readfirstlane(r.x), __builtin_amdgcn_readfirstlane(r.y),
+                     __builtin_amdgcn_readfirstlane(r.z));
+}
+
+__device__ __forceinline__ int floor_div(int a, int b) {
+    return a >= 0 ? a / b : -((-a + b - 1) / b);
+}
+
+// Bilinear sample from the staged band at element `idx` (lower tap: row i, column c): the pair
+// (c, c + 1) runs along the non-dominant axis, rows i and i + 1 along the dominant one; fa / fb
+// are the fractions along those axes.
+__device__ __forceinline__ float band_sample(const float *band, unsigned idx, float fa,
+                                             float fb) {
+    const float a0 = band[idx], a1 = band[idx + 1];
+    const float b0 = band[idx + kPitch], b1 = band[idx + kPitch + 1];
+    const float u0 = fmaf(fa, a1 - a0, a0);
+    const float u1 = fmaf(fa, b1 - b0, b0);
+    return fmaf(fb, u1 - u0, u0);
+}
+
+// One pass (YDOM: rays whose dominant axis is y, row-major copy; else x, transposed copy).
+// All threads of the workgroup call it together; returns the lane's sum, 0 if not in the pass.
+//   rstride      elements per row of the copy (rows run along the dominant axis)
+//   cofs         element offset of the copy's centre (centred coordinate 0, 0)
 template <class SI, bool YDOM>
-__device__ __forceinline__ float march2d(rsrc_t R, float4 e, float4 q, int rstride, int cofs) {
+__device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool mine, int rstride, int cofs,
+                            float *band, int3 *red, int nthreads) {
     const int n = __float_as_int(q.y);
+    const bool act = mine && n > 0;
+    if (!__syncthreads_or(act)) return 0.0f;
+    const int nwaves = nthreads >> 6;
+    const float md = YDOM ? e.y : e.x;   // dominant coordinate of the midpoint
+    const float sd = YDOM ? e.w : e.z;   // dominant step, +-1
+    const float mc = YDOM ? e.x : e.y;   // non-dominant coordinate of the midpoint
+    const float vc = YDOM ? e.z : e.w;   // non-dominant step
+    const int r0 = (int)floorf(md);
+    const int s = sd > 0.0f ? 1 : -1;
+    const float sf = (float)s;
+    const int jlo = (int)q.z, jhi = jlo + n - 1;
+    const int ta = s > 0 ? r0 + jlo : r0 - jhi; // first row of this lane
+    const int tz = ta + n - 1;                  // last row
+    const int3 tr = block_reduce(act ? ta : INT_MAX, act ? tz : INT_MIN, 0, red, nwaves);
+    const unsigned span = (unsigned)(n - 1);
+    const float jmin = (float)jlo, jmax = (float)(act ? jhi : jlo);
     const int soff1 = rstride * (int)sizeof(typename SI::raw);
-    float acc = 0.0f, jf = q.z;
-    int j = 0;
-    while (j < n) {
-        const int jend = min(j + kSumBlock, n);
-        float s0 = 0.0f, s1 = 0.0f;
-        for (; j + 1 < jend; j += 2, jf += 2.0f) {
-            const float jg = jf + 1.0f;
-            s0 += sample2d<SI, YDOM>(R, fmaf(jf, e.z, e.x), fmaf(jf, e.w, e.y), rstride, soff1,
-                                     cofs);
-            s1 += sample2d<SI, YDOM>(R, fmaf(jg, e.z, e.x), fmaf(jg, e.w, e.y), rstride, soff1,
-                                     cofs);
+    const unsigned last = (unsigned)((kBandRows - 1) * kPitch - 2); // highest lower tap index
+    float acc = 0.0f;
+    for (int t0 = floor_div(tr.x, kBand) * kBand; t0 <= tr.y; t0 += kBand) {
+        const int b_lo = max(tr.x, t0) & ~1, b_hi = min(tr.y, t0 + kBand - 1);
+        // columns the lane's samples in this band touch (the non-dominant coordinate is linear
+        // in the row, so its extremes are at the lane's first and last row of the band), and
+        // whether the lane has a sample on every row [b_lo, b_hi + 1]
+        const int f = max(ta, b_lo), l = min(tz, b_hi);
+        int cmin = INT_MAX, cmax = INT_MIN;
+        if (act && f <= l) {
+            const int c1 = floor_i32(fmaf((float)(s * (f - r0)), vc, mc));
+            const int c2 = floor_i32(fmaf((float)(s * (l - r0)), vc, mc));
+            cmin = min(c1, c2);
+            cmax = max(c1, c2);
         }
-        if (j < jend) {
-            s0 += sample2d<SI, YDOM>(R, fmaf(jf, e.z, e.x), fmaf(jf, e.w, e.y), rstride, soff1,
-                                     cofs);
-            ++j;
-            jf += 1.0f;
+        const int full_lane = act && ta <= b_lo && tz >= ((b_hi | 1)) ? 1 : 0;
+        const int3 cr = block_reduce(cmin, cmax, full_lane, red, nwaves);
+        const int c_lo = cr.x - 1, W = cr.y + 3 - c_lo; // columns [cr.x - 1, cr.y + 2]
+        const int r_lo = t0 - 1;
+        float s0 = 0.0f, s1 = 0.0f;
+        float jf = (float)(s * (b_lo - r0));
+        if (W <= kPitch) {
+            // stage rows [r_lo, r_lo + kBandRows) x columns [c_lo, c_lo + 4 Q) as FP32
+            const int Q = (W + 3) >> 2; // >= 1 (W >= 4)
+            // i / Q as a multiply-high, exact for i < 2^16 (Q = 1: the identity)
+            const unsigned magic = Q == 1 ? 0u : 0xffffffffu / (unsigned)Q + 1u;
+            const int g0 = r_lo * rstride + c_lo + cofs;
+            const int items = kBandRows * Q;
+            // four items per thread and trip, loads first: an index past the end repeats the
+            // last item (same value to the same cell)
+            for (int i0 = threadIdx.x; i0 < items; i0 += 4 * nthreads) {
+                int row[4], c4[4];
+                Quad v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i = min(i0 + u * nthreads, items - 1);
+                    row[u] = Q == 1 ? i : (int)__umulhi((unsigned)i, magic);
+                    c4[u] = (i - row[u] * Q) << 2;
+                    v[u] = buf_quad<SI>(R, g0 + row[u] * rstride + c4[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    *reinterpret_cast<float4 *>(band + row[u] * kPitch + c4[u]) =
+                        make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
+            }
+            __syncthreads();
+            const int base = r_lo * kPitch + c_lo;
+            if (cr.z) { // every lane has a sample on every row of the band
+                for (int t = b_lo; t <= b_hi; t += 2, jf += 2.0f * sf) {
+                    const float jb = jf + sf;
+                    const float xa = fmaf(jf, e.z, e.x), ya = fmaf(jf, e.w, e.y);
+                    const float xb = fmaf(jb, e.z, e.x), yb = fmaf(jb, e.w, e.y);
+                    const float da = YDOM ? ya : xa, ca = YDOM ? xa : ya;
+                    const float db = YDOM ? yb : xb, cb = YDOM ? xb : yb;
+                    // (the min changes nothing here; its bound lets the compiler fold the second
+                    // row's offset into the LDS instruction)
+                    const unsigned ia = min(
+                        (unsigned)(__mul24(floor_i32(da), kPitch) + floor_i32(ca) - base), last);
+                    const unsigned ib = min(
+                        (unsigned)(__mul24(floor_i32(db), kPitch) + floor_i32(cb) - base), last);
+                    s0 += band_sample(band, ia, __builtin_amdgcn_fractf(ca),
+                                      __builtin_amdgcn_fractf(da));
+                    s1 += band_sample(band, ib, __builtin_amdgcn_fractf(cb),
+                                      __builtin_amdgcn_fractf(db));
+                }
+            } else {
+                for (int t = b_lo; t <= b_hi; t += 2, jf += 2.0f * sf) {
+                    const float ja = __builtin_amdgcn_fmed3f(jf, jmin, jmax);
+                    const float jb = __builtin_amdgcn_fmed3f(jf + sf, jmin, jmax);
+                    const float xa = fmaf(ja, e.z, e.x), ya = fmaf(ja, e.w, e.y);
+                    const float xb = fmaf(jb, e.z, e.x), yb = fmaf(jb, e.w, e.y);
+                    const float da = YDOM ? ya : xa, ca = YDOM ? xa : ya;
+                    const float db = YDOM ? yb : xb, cb = YDOM ? xb : yb;
+                    const unsigned ia = min(
+                        (unsigned)(__mul24(floor_i32(da), kPitch) + floor_i32(ca) - base), last);
+                    const unsigned ib = min(
+                        (unsigned)(__mul24(floor_i32(db), kPitch) + floor_i32(cb) - base), last);
+                    const float va = band_sample(band, ia, __builtin_amdgcn_fractf(ca),
+                                                 __builtin_amdgcn_fractf(da));
+                    const float vb = band_sample(band, ib, __builtin_amdgcn_fractf(cb),
+                                                 __builtin_amdgcn_fractf(db));
+                    s0 += act && (unsigned)(t - ta) <= span ? va : 0.0f;
+                    s1 += act && (unsigned)(t + 1 - ta) <= span ? vb : 0.0f;
+                }
+            }
+            __syncthreads(); // the band is read before the next one is staged
+        } else {
+            for (int t = b_lo; t <= b_hi; t += 2, jf += 2.0f * sf) {
+                const float ja = __builtin_amdgcn_fmed3f(jf, jmin, jmax);
+                const float jb = __builtin_amdgcn_fmed3f(jf + sf, jmin, jmax);
+                const float va = sample2d<SI, YDOM>(R, fmaf(ja, e.z, e.x), fmaf(ja, e.w, e.y),
+                                                    rstride, soff1, cofs);
+                const float vb = sample2d<SI, YDOM>(R, fmaf(jb, e.z, e.x), fmaf(jb, e.w, e.y),
+                                                    rstride, soff1, cofs);
+                s0 += act && (unsigned)(t - ta) <= span ? va : 0.0f;
+                s1 += act && (unsigned)(t + 1 - ta) <= span ? vb : 0.0f;
+            }
         }
         acc += s0 + s1;
     }
     return acc;
 }
 
-// One thread per ray (a, k); a wavefront holds 64 adjacent cells of one angle.
+// One workgroup = nthreads <= kFwdRays adjacent detector cells of one angle, one thread per
+// ray; LDS holds one band of kBandRows x kPitch floats (26.9 KB).
 // sino[b][a][k] = scale * sum_{j<N} bilinear(img, e + j v)   (forward_2d.hpp:63-78)
 template <class SI, class SS>
-__global__ __launch_bounds__(kMaxBlock) void k_fwd2d(const typename SI::raw *__restrict__ scratch,
-                                               int Wp, int Hp, const float4 *__restrict__ ev,
-                                               const float4 *__restrict__ ns,
-                                               typename SS::raw *__restrict__ sino, int D,
-                                               int A) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y * blockDim.y + threadIdx.y;
-    if (k >= D || a >= A) return;
+__global__ __launch_bounds__(kFwdRays) void k_fwd2d(const typename SI::raw *__restrict__ scratch,
+                                              int Wp, int Hp, const float4 *__restrict__ ev,
+                                              const float4 *__restrict__ ns,
+                                              typename SS::raw *__restrict__ sino, int D, int A) {
+    __shared__ __align__(16) float band[kBandRows * kPitch];
+    __shared__ int3 red[kMaxWaves];
+    const int nthreads = blockDim.x;
+    const int k = blockIdx.x * nthreads + threadIdx.x;
+    const int a = blockIdx.y;
+    const bool valid = k < D;
     const size_t b = blockIdx.z;
     const size_t r = (size_t)a * D + k;
-    const float4 e = ev[r];
-    const float4 q = ns[r];
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 e = valid ? ev[r] : zero;
+    const float4 q = valid ? ns[r] : zero; // N = 0
     const int one = Wp * Hp;
     const rsrc_t R = make_rsrc(scratch + b * 2 * (size_t)one, bytes_of<SI>(2 * (uint64_t)one));
     const int c0x = Wp / 2, c0y = Hp / 2;
-    float acc;
-    if (fabsf(e.w) == 1.0f) // y is the dominant axis: row-major copy
-        acc = march2d<SI, true>(R, e, q, Wp, c0y * Wp + c0x);
-    else                    // x dominant: transposed copy, second half of the block
-        acc = march2d<SI, false>(R, e, q, Hp, one + c0x * Hp + c0y);
-    sino[b * (size_t)A * D + r] = from_f32<SS>(acc * q.x);
+    const bool ydom = fabsf(e.w) == 1.0f; // y dominant: row-major copy; else the transposed one
+    const float acc =
+        march_band<SI, true>(R, e, q, valid && ydom, Wp, c0y * Wp + c0x, band, red, nthreads) +
+        march_band<SI, false>(R, e, q, valid && !ydom, Hp, one + c0x * Hp + c0y, band, red,
+                              nthreads);
+    if (valid) sino[b * (size_t)A * D + r] = from_f32<SS>(acc * q.x);
 }
 
 // ---------------------------------------------------------------- backprojection ---------
@@ -570,9 +746,9 @@ CMI_API cmi_status cmi_forward_2d(const cmi_plan2d *plan, const void *volume, in
                            (unsigned)batch);
         hipLaunchKernelGGL((k_stage2d<SI>), sg, sb, 0, s, (const RI *)volume, (RI *)scratch, nx,
                            ny, Wp, Hp);
-        dim3 fb(plan->block_x, plan->block_y),
-            fg((unsigned)((D + plan->block_x - 1) / plan->block_x),
-               (unsigned)((A + plan->block_y - 1) / plan->block_y), (unsigned)batch);
+        // block_x * block_y (at most kFwdRays) adjacent cells of one angle per workgroup
+        const unsigned nt = std::min<unsigned>(plan->block_x * plan->block_y, kFwdRays);
+        dim3 fb(nt), fg((unsigned)((D + nt - 1) / nt), (unsigned)A, (unsigned)batch);
         hipLaunchKernelGGL((k_fwd2d<SI, SS>), fg, fb, 0, s, (const RI *)scratch, Wp, Hp,
                            plan->d_ev, plan->d_ns, (typename SS::raw *)sinogram, D, A);
         CMI_HIP_TRY(hipGetLastError());

@@ -35,15 +35,16 @@ constexpr int kRing = 4;       // launches whose job tables may be in flight at 
 constexpr int kThreads = 256;  // threads per workgroup
 constexpr int kTile2X = 64, kTile2Y = 4;   // fan-beam tile: pixels
 constexpr int kTile3X = 16, kTile3Y = 16;  // cone-beam tile: columns of kZB voxels
+constexpr int kRunX = 64, kRunZ = 4;       // cone-beam tile of y runs: 64 x kYB x 4 voxels
 
 // Device copy of a job with the derived values the kernel needs.
 struct DevJob {
-    int32_t kind, mode;
+    int32_t kind, mode, columns;
     const void *sino;
     const float *table;
     void *volume;
     int32_t nx, ny, nz, nu, nv, nangles;
-    int32_t tiles_x, tiles_y;  // tiles per row / per slab of the volume
+    int32_t tiles_x, tiles_y;  // tiles along x / along y of the volume
     uint32_t first;            // first launch tile of this job
     uint32_t tile_begin;       // first tile of the job's own numbering
     uint32_t sino_bytes;       // size of the staged projections, for the range check
@@ -54,6 +55,9 @@ struct DevJob {
 uint64_t tiles_of(const cmi_bp_job &j) {
     if (j.kind == CMI_JOB_FAN2D)
         return (uint64_t)((j.nx + kTile2X - 1) / kTile2X) * ((j.ny + kTile2Y - 1) / kTile2Y);
+    if (j.columns == 1)
+        return (uint64_t)((j.nx + kRunX - 1) / kRunX) * ((j.ny + kYB - 1) / kYB) *
+               ((j.nz + kRunZ - 1) / kRunZ);
     return (uint64_t)((j.nx + kTile3X - 1) / kTile3X) * ((j.ny + kTile3Y - 1) / kTile3Y) *
            ((j.nz + kZB - 1) / kZB);
 }
@@ -136,6 +140,38 @@ __device__ __forceinline__ void serve_tile_3d(const DevJob &J, uint32_t t) {
     }
 }
 
+// Cone-beam tile of y runs: the run of kYB voxels k_bp3d_y gives thread (ix, iz).
+template <class SV, class SS>
+__device__ __forceinline__ void serve_tile_3d_y(const DevJob &J, uint32_t t) {
+    const uint32_t per_slab = (uint32_t)J.tiles_x * (uint32_t)J.tiles_y;
+    const int zq = (int)(t / per_slab);
+    const uint32_t r = t % per_slab;
+    const int ix = (int)(r % (uint32_t)J.tiles_x) * kRunX + (int)(threadIdx.x % kRunX);
+    const int iy0 = (int)(r / (uint32_t)J.tiles_x) * kYB;
+    const int iz = zq * kRunZ + (int)(threadIdx.x / kRunX);
+    if (ix >= J.nx || iz >= J.nz) return;
+    const rsrc_t R = make_rsrc(J.sino, J.sino_bytes);
+    float acc[kYB];
+#pragma unroll
+    for (int y = 0; y < kYB; ++y) acc[y] = 0.0f;
+    const float hx = (float)ix - J.cx, hy = (float)iy0 - J.cy, hz = (float)iz - J.cz;
+    if (J.mode == CMI_BP3D_FDK)
+        bp3d_yrun<SS, true, true>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+    else if (J.mode == CMI_BP3D_MATCHED)
+        bp3d_yrun<SS, true, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+    else
+        bp3d_yrun<SS, false, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+    auto *vol = static_cast<typename SV::raw *>(J.volume);
+#pragma unroll
+    for (int y = 0; y < kYB; ++y) {
+        const int iy = iy0 + y;
+        if (iy < J.ny) {
+            const size_t i = ((size_t)iz * J.ny + iy) * (size_t)J.nx + ix;
+            vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc[y]);
+        }
+    }
+}
+
 template <class SV, class SS>
 __global__ __launch_bounds__(kThreads) void k_serve_bp(const DevJob *__restrict__ jobs,
                                                        int njobs, uint32_t total,
@@ -156,6 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_serve_bp(const DevJob *__restrict_
         const DevJob &J = jobs[lo];
         const uint32_t lt = t - J.first + J.tile_begin;
         if (J.kind == CMI_JOB_FAN2D) serve_tile_2d<SV, SS>(J, lt);
+        else if (J.columns == 1) serve_tile_3d_y<SV, SS>(J, lt);
         else serve_tile_3d<SV, SS>(J, lt);
     }
 }
@@ -193,6 +230,9 @@ cmi_status check_job(const cmi_bp_job &j, uint32_t i, int sino_dtype) {
     if (j.nx > 16384 || j.ny > 16384 || j.nz > 16384 || j.nu > 65536 || j.nv > 65536 ||
         j.nangles > 65535)
         return fail(CMI_ERR_INVALID_ARGUMENT, "serve: job %u exceeds the size limits", i);
+    if (!fan && j.columns != 0 && j.columns != 1)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "serve: job %u has unknown columns %d", i,
+                    j.columns);
     if (fan && (j.nz != 1 || j.nv != 1))
         return fail(CMI_ERR_INVALID_ARGUMENT, "serve: fan-beam job %u needs nz = nv = 1", i);
     if (fan && !(std::isfinite(j.sx) && std::isfinite(j.sy) && std::isfinite(j.ox) &&
@@ -310,6 +350,13 @@ CMI_API cmi_status cmi_server_create(int device, uint32_t max_jobs, cmi_server *
     return CMI_OK;
 }
 
+CMI_API cmi_status cmi_server_get_capacity(const cmi_server *p, uint32_t *workgroups) {
+    clear_error();
+    if (!p || !workgroups) return fail(CMI_ERR_INVALID_ARGUMENT, "server_get_capacity: null");
+    *workgroups = p->grid;
+    return CMI_OK;
+}
+
 CMI_API cmi_status cmi_server_destroy(cmi_server *p) {
     clear_error();
     if (!p) return CMI_OK;
@@ -360,14 +407,23 @@ CMI_API cmi_status cmi_serve_backproject(cmi_server *p, const cmi_bp_job *jobs, 
         DevJob d{};
         d.kind = j.kind;
         d.mode = j.mode;
+        d.columns = j.kind == CMI_JOB_FAN2D ? 0 : j.columns;
         d.sino = j.sino;
         d.table = static_cast<const float *>(j.table);
         d.volume = j.volume;
         d.nx = (int)j.nx, d.ny = (int)j.ny, d.nz = (int)j.nz;
         d.nu = (int)j.nu, d.nv = (int)j.nv, d.nangles = (int)j.nangles;
         const bool fan = j.kind == CMI_JOB_FAN2D;
-        d.tiles_x = fan ? (d.nx + kTile2X - 1) / kTile2X : (d.nx + kTile3X - 1) / kTile3X;
-        d.tiles_y = fan ? (d.ny + kTile2Y - 1) / kTile2Y : (d.ny + kTile3Y - 1) / kTile3Y;
+        if (fan) {
+            d.tiles_x = (d.nx + kTile2X - 1) / kTile2X;
+            d.tiles_y = (d.ny + kTile2Y - 1) / kTile2Y;
+        } else if (d.columns == 1) {
+            d.tiles_x = (d.nx + kRunX - 1) / kRunX;
+            d.tiles_y = (d.ny + kYB - 1) / kYB;
+        } else {
+            d.tiles_x = (d.nx + kTile3X - 1) / kTile3X;
+            d.tiles_y = (d.ny + kTile3Y - 1) / kTile3Y;
+        }
         d.first = first;
         d.tile_begin = j.tile_begin;
         const uint64_t elems = (uint64_t)j.nangles * (j.nu + 2 * kPadDet) *

@@ -10,7 +10,8 @@ along detector rows with the band-limited ramp h of unit cell spacing (Kak and S
 -1/(pi k)^2, h[k even] = 0) times a frequency window, B_fbp the 2D backprojector with the
 squared magnification weight and B_fdk the geometry-matched 3D backprojector with the
 (DSO / depth)^2 weight. The convolution runs through an FFT of length >= 2 D (zero padded, so
-it is linear, not circular). The result is in the image's density units.
+it is linear, not circular; the length has only the prime factors 2, 3 and 5). The result is
+in the image's density units.
 
 Windows follow examples/fbp.py:27-48 over the normalised frequency |W| in [0, 1] (1 = Nyquist):
 ram-lak 1, shepp-logan sinc(W / (2 s)), cosine cos(W pi / (2 s)) (the reference multiplies by
@@ -50,11 +51,25 @@ def _window(W: torch.Tensor, name, s: float) -> torch.Tensor:
     return w * (a <= s)
 
 
+def fft_length(n_det: int) -> int:
+    """Smallest even L >= 2 n_det whose only prime factors are 2, 3 and 5 (sizes rocFFT runs
+    with its fastest radices); zero padding to L makes the row convolution linear."""
+    L = 2 * n_det
+    while True:
+        m = L
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1 and L % 2 == 0:
+            return L
+        L += 1
+
+
 def ramp_response(n_det: int, window="ram-lak", frequency_scaling: float = 1.0,
                   device=None) -> torch.Tensor:
     """Real frequency response (rfft layout) of the windowed band-limited ramp for rows of
-    n_det cells, FFT length L = next power of two >= 2 n_det. Returns (L // 2 + 1,) float32."""
-    L = 1 << max(1, math.ceil(math.log2(2 * n_det)))
+    n_det cells, FFT length L = fft_length(n_det). Returns (L // 2 + 1,) float32."""
+    L = fft_length(n_det)
     k = torch.arange(L, dtype=torch.float64)
     m = torch.where(k <= L // 2, k, k - L)                   # signed index, FFT order
     h = torch.zeros(L, dtype=torch.float64)

@@ -12,8 +12,9 @@ Jobs (raw FP32 projections + a FanGeometry or ConeGeometry) arrive over time. Th
   backprojector (cmi_serve_backproject) over tiles of every prepared job, at most
   `round_cost` pixel-angle units, shortest remaining work first with a guaranteed share for
   the oldest job, so small jobs are not stuck behind large ones and large ones keep moving;
-  up to `max_rounds_in_flight` rounds are queued on the compute stream while new arrivals are
-  prepared on the prep stream, so filtering overlaps backprojection;
+  up to `max_rounds_in_flight` rounds are queued, alternating between compute streams so
+  that one launch's workgroups fill the compute units another launch frees as it drains,
+  while new arrivals are prepared on the prep stream, so filtering overlaps backprojection;
 * completes a job when the round holding its last tiles has finished, releasing its staged
   projections and keeping its volume as the result.
 
@@ -34,7 +35,7 @@ import torch
 
 from . import _lib
 from .geometry import ConeGeometry, FanGeometry
-from .reconstruction import cosine_weights_cone, cosine_weights_fan, ramp_response
+from .reconstruction import cosine_weights_cone, cosine_weights_fan, fft_length, ramp_response
 
 # Relative cost of one voxel-angle of the cone-beam backprojector to one pixel-angle of the
 # fan-beam one (trilinear-free but 2 x 2 taps and a perspective divide per sample).
@@ -73,19 +74,15 @@ class _Active:
     footprint: int
     staged: torch.Tensor
     table: torch.Tensor
+    ycols: bool = False
     keep: list = field(default_factory=list)
     prep_done: Optional[torch.cuda.Event] = None
     next_tile: int = 0
     tiles_done: int = 0
-    waited: bool = False
 
     @property
     def remaining_cost(self) -> float:
         return (self.ntiles - self.next_tile) * self.tile_cost
-
-
-def _fft_len(n: int) -> int:
-    return 1 << max(1, math.ceil(math.log2(2 * n)))
 
 
 def footprint(job: FBPJob, storage: torch.dtype):
@@ -98,12 +95,12 @@ def footprint(job: FBPJob, storage: torch.dtype):
     if isinstance(g, FanGeometry):
         A, D = g.nangles, g.det_shape
         ny, nx = (int(v) for v in g.vol_shape)
-        L, rows, U = _fft_len(D), A, D
+        L, rows, U = fft_length(D), A, D
         vox, staged, table = nx * ny, A * (D + 4) * s, 16 * A
     else:
         A = g.nangles
         V, U = (int(v) for v in g.det_shape)
-        L, rows = _fft_len(U), A * V
+        L, rows = fft_length(U), A * V
         vox, staged, table = int(np.prod(g.vol_shape)), A * (V + 4) * (U + 4) * s, 48 * A
     transient = rows * U * 4 + 2 * rows * (L // 2 + 1) * 8 + rows * L * 4
     if not job.projections.is_cuda:
@@ -115,7 +112,7 @@ class FBPServer:
     def __init__(self, device: int = 0, storage: torch.dtype = torch.float16,
                  hbm_budget: int = 32 << 30, max_active: int = 64,
                  round_cost: float = 4e9, oldest_share: float = 0.5,
-                 max_rounds_in_flight: int = 2):
+                 max_rounds_in_flight: int = 3, compute_streams: int = 2):
         if storage not in (torch.float16, torch.bfloat16, torch.float32):
             raise TypeError("storage must be float16, bfloat16 or float32")
         self.device = torch.device("cuda", device)
@@ -127,7 +124,9 @@ class FBPServer:
         self.max_rounds_in_flight = int(max_rounds_in_flight)
         self._native = _lib.Server(device, max_jobs=max_active)
         self._prep = torch.cuda.Stream(self.device)
-        self._compute = torch.cuda.Stream(self.device)
+        # launches alternate between streams: while one launch drains its last tiles, the
+        # next one's workgroups take the freed compute units
+        self._compute = [torch.cuda.Stream(self.device) for _ in range(max(1, compute_streams))]
         self._cache = {}
         self.rounds = 0
         self.launch_log: List[tuple] = []
@@ -152,8 +151,9 @@ class FBPServer:
             H = ramp_response(int(g.det_shape[1]), window, fs, dev)
             cw = cosine_weights_cone(g, dev)
             scale = (math.pi / g.nangles) * g.DSD / (float(g.det_spacing[1]) * g.DSO)
-            table = torch.from_numpy(g.plan.bp_matrices("fdk")).to(dev)
-            parts = (H, cw, None, scale, table)
+            m, yinv = g.plan.bp_matrices("fdk", with_layout=True)
+            table = torch.from_numpy(m).to(dev)
+            parts = (H, cw, None, scale, (table, yinv))
         if len(self._cache) > 256:
             self._cache.clear()
         self._cache[key] = (g, parts)
@@ -168,6 +168,9 @@ class FBPServer:
         with torch.cuda.stream(self._prep):
             H, cw, row_scale, scale, table = self._filter_parts(g, job.window,
                                                                 job.frequency_scaling)
+            ycols = False
+            if not fan:
+                table, ycols = table
             p = job.projections.to(device=dev, dtype=torch.float32, non_blocking=True)
             if fan:
                 A, D = g.nangles, g.det_shape
@@ -192,7 +195,10 @@ class FBPServer:
                 staged = torch.empty((A, V + 4, U + 4), dtype=self.storage, device=dev)
                 keep = [_lib.stage_projections(y, staged, scale=scale)]
                 volume = torch.zeros((nz, ny, nx), dtype=self.storage, device=dev)
-                ntiles = ((nx + 15) // 16) * ((ny + 15) // 16) * ((nz + 7) // 8)
+                if ycols:  # runs of 8 y voxels, tiles of 64 x 8 x 4
+                    ntiles = ((nx + 63) // 64) * ((ny + 7) // 8) * ((nz + 3) // 4)
+                else:      # columns of 8 z voxels, tiles of 16 x 16 x 8
+                    ntiles = ((nx + 15) // 16) * ((ny + 15) // 16) * ((nz + 7) // 8)
                 tile_cost = float(VOXELS_PER_TILE_3D * A * CONE_COST)
             ev = torch.cuda.Event()
             ev.record(self._prep)
@@ -201,7 +207,7 @@ class FBPServer:
         # after the staging kernel in stream order.
         return _Active(job=job, fan=fan, ntiles=ntiles, tile_cost=tile_cost,
                        footprint=footprint(job, self.storage)[0], staged=staged, table=table,
-                       keep=keep, prep_done=ev)
+                       ycols=ycols, keep=keep, prep_done=ev)
 
     def _descriptor(self, a: _Active, t0: int, t1: int) -> _lib.BPJob:
         g, j = a.job.geom, _lib.BPJob()
@@ -216,7 +222,7 @@ class FBPServer:
             j.ox, j.oy = float(g.vol_offset[1]), float(g.vol_offset[0])
             j.inv_du = 1.0 / float(g.det_spacing)
         else:
-            j.kind, j.mode = _lib.JOB_CONE3D, _lib.BP3D_FDK
+            j.kind, j.mode, j.columns = _lib.JOB_CONE3D, _lib.BP3D_FDK, int(a.ycols)
             nz, ny, nx = (int(v) for v in g.vol_shape)
             V, U = (int(v) for v in g.det_shape)
             j.nx, j.ny, j.nz = nx, ny, nz
@@ -237,7 +243,7 @@ class FBPServer:
                 raise MemoryError(f"job {j.id} needs {need} B, more than the server's budget "
                                   f"of {self.hbm_budget} B")
         active: List[_Active] = []
-        rounds = deque()  # (event, [(active, tiles)])
+        rounds = []  # (event, [(active, tiles)])
         used = 0
         done = 0
         t0 = clock()
@@ -255,9 +261,10 @@ class FBPServer:
                 self.max_active_seen = max(self.max_active_seen, len(active))
                 self.max_used_seen = max(self.max_used_seen, used)
                 progressed = True
-            # completed rounds
-            while rounds and rounds[0][0].query():
-                _, parts = rounds.popleft()
+            # completed rounds (launches on different streams may finish out of order)
+            for r in [r for r in rounds if r[0].query()]:
+                rounds.remove(r)
+                parts = r[1]
                 now = clock() - t0
                 for a, n in parts:
                     a.tiles_done += n
@@ -273,9 +280,10 @@ class FBPServer:
             if len(rounds) < self.max_rounds_in_flight:
                 parts = self._plan_round(active)
                 if parts:
-                    self._launch(parts)
+                    stream = self._compute[self.rounds % len(self._compute)]
+                    self._launch(parts, stream)
                     ev = torch.cuda.Event()
-                    ev.record(self._compute)
+                    ev.record(stream)
                     rounds.append((ev, [(a, t1 - t0_) for a, t0_, t1 in parts]))
                     progressed = True
             if not progressed:
@@ -327,14 +335,11 @@ class FBPServer:
         out.sort(key=lambda p: -p[0].tile_cost)
         return out
 
-    def _launch(self, parts):
+    def _launch(self, parts, stream):
         for a, _, _ in parts:
-            if not a.waited:
-                self._compute.wait_event(a.prep_done)
-                a.waited = True
+            stream.wait_event(a.prep_done)
         descs = [self._descriptor(a, t0, t1) for a, t0, t1 in parts]
-        self._native.backproject(descs, self.storage, self.storage,
-                                 stream=self._compute.cuda_stream)
+        self._native.backproject(descs, self.storage, self.storage, stream=stream.cuda_stream)
         self.rounds += 1
         self.launch_log.append(tuple((a.job.id, t0, t1) for a, t0, t1 in parts))
 

@@ -238,11 +238,14 @@ CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_
  * rebuild them per call: backward_2d.hpp:58-62, backward_3d.hpp:110-178).
  *   cmi_fan2d_bp_table: table[4a..4a+3] = (-sin a, cos a, DSD[a], DSO[a]) in FP32.
  *   cmi_cone3d_bp_matrices: 12 floats per angle, the 3x4 matrix of `mode` (cmi_bp3d_mode)
- *   that cmi_backward_3d uses for the same description.
+ *   that cmi_backward_3d uses for the same description; *y_invariant (if not null) is set to
+ *   1 when no angle's U or W row depends on y (circular orbits about the y axis, untilted
+ *   panel), the case in which the backprojectors run along y (see cmi_bp_job.columns).
  */
 CMI_API cmi_status cmi_fan2d_bp_table(uint64_t nangles, const double *angles, const double *DSO,
                                       const double *DSD, float *table);
-CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *desc, int mode, float *out);
+CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *desc, int mode, float *out,
+                                          int *y_invariant);
 
 /*
  * Copies filtered projections into the zero-bordered layout the serving backprojector reads,
@@ -264,20 +267,25 @@ typedef enum cmi_job_kind { CMI_JOB_FAN2D = 0, CMI_JOB_CONE3D = 1 } cmi_job_kind
 /*
  * One backprojection job of a multi-job launch: volume += B sinogram over the tile range
  * [tile_begin, tile_end) of the job's volume. Fan-beam tiles are 64 x 4 pixels (row-major
- * over ceil(nx / 64) x ceil(ny / 4)); cone-beam tiles are 16 x 16 x 8 voxels (x fastest, then
- * y, then z blocks). The arithmetic of every voxel equals cmi_backward_2d_weighted /
- * cmi_backward_3d for the same tables, so a job's result does not depend on the other jobs of
- * the launch or on how its tiles are split between launches.
+ * over ceil(nx / 64) x ceil(ny / 4)). Cone-beam tiles are 16 x 16 x 8 voxels (x fastest, then
+ * y, then z blocks) with columns = 0, or 64 x 8 x 4 (x, y, z; x fastest, then y, then z
+ * blocks) with columns = 1, which the caller may set only for y_invariant matrices. The
+ * arithmetic of every voxel equals cmi_backward_2d_weighted / cmi_backward_3d for the same
+ * tables (cmi_backward_3d runs along y exactly when the matrices are y_invariant), so a job's
+ * result does not depend on the other jobs of the launch or on how its tiles are split
+ * between launches.
  *   sino    staged projections: [nangles][nu + 4] (fan) or [nangles][nv + 4][nu + 4] (cone)
  *   table   device copy of cmi_fan2d_bp_table / cmi_cone3d_bp_matrices output
  *   volume  [ny][nx] / [nz][ny][nx], accumulated into
  *   mode    fan: cmi_bp2d_weighting; cone: cmi_bp3d_mode (the mode the matrices were made for)
+ *   columns cone only: 0 = z columns (any geometry), 1 = y runs (y_invariant matrices only)
  *   fan only: sx, sy pixel spacing, ox, oy volume offset, inv_du = 1 / detector spacing
  * No counterpart in the reference, which runs one operator call per image.
  */
 typedef struct cmi_bp_job {
     int32_t kind;
     int32_t mode;
+    int32_t columns;
     const void *sino;
     const void *table;
     void *volume;
@@ -296,6 +304,10 @@ typedef struct cmi_server cmi_server;
  * and tile counters. */
 CMI_API cmi_status cmi_server_create(int device, uint32_t max_jobs, cmi_server **server);
 CMI_API cmi_status cmi_server_destroy(cmi_server *server);
+
+/* Workgroups of the serving kernel the device holds resident at once (the persistent grid
+ * of a launch with at least that many tiles). */
+CMI_API cmi_status cmi_server_get_capacity(const cmi_server *server, uint32_t *workgroups);
 
 /*
  * One persistent launch over `njobs` jobs whose volumes are stored as vol_dtype and sinograms

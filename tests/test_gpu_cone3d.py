@@ -201,3 +201,20 @@ def test_c3_dot_product(big, gpu):
     r = abs(float(lhs - rhs)) / abs(float(lhs))
     print(f"c3 dot test: {r:.3e}")
     assert r <= 1e-5
+
+
+def test_circular_orbit_odd_shapes(oracle, gpu):
+    """Circular orbit (the backprojectors run along y): volume sizes that are no multiple of
+    the run length or tile, anisotropic voxels, offsets, a panel turned about y."""
+    rec = cone(30, 40, 34, 16, 70.0, 110.0, du=(1.1, 0.9), vol_spacing=(0.9, 1.2, 1.0),
+               vol_offset=(0.7, -0.4, 0.3), det_offset=(1.2, -0.8), det_rotation=(0.03, 0.0, 0.0),
+               cor=0.5, nz=19, ny=21)
+    geom = as_cone_geometry(rec)
+    y = np.random.default_rng(12).random((16, 34, 40))
+    for mode in ("reference", "matched", "fdk"):
+        ref = oracle.bp3d(rec, y, matched=mode == "matched", fdk=mode == "fdk")
+        check_close(host(cm.backward(dev(y, gpu), geom, mode=mode)), ref,
+                    f"circular odd-shape bp3d {mode}")
+    yb = np.stack([y, 0.5 * y])
+    out = host(cm.backward(dev(yb, gpu), geom, mode="fdk"))
+    check_close(out[1], 0.5 * oracle.bp3d(rec, y, fdk=True), "circular odd-shape bp3d batch 1")

@@ -165,3 +165,68 @@ def test_native_job_validation(gpu):
     with pytest.raises(AttributeError):
         srv.backproject([bad], torch.float16, torch.float16)
     torch.cuda.synchronize()
+
+
+def _c5_geometry(kind, n, storage):
+    A = 2 * n
+    angles = np.linspace(0, 2 * np.pi, A, endpoint=False)
+    if kind == "fan":
+        return cm.FanGeometry(DSD=3.0 * n, DSO=2.0 * n, angles=angles, vol_shape=(n, n),
+                              det_shape=int(math.ceil(math.sqrt(2) * n)), vol_prec=storage,
+                              sino_prec=storage)
+    p = (3 * n) // 2
+    return cm.ConeGeometry(DSD=3.0 * n, DSO=2.0 * n, angles=angles, vol_shape=(n, n, n),
+                           det_shape=(p, p), vol_prec=storage, sino_prec=storage)
+
+
+def _fp64_subset(oracle, served, raw, g):
+    """FP64 pipeline on a subset: two image rows (fan) or the central slab (cone), filtering
+    only the panel rows that the slab projects onto (found from the FDK matrices: a
+    linear-fractional map takes its extremes over the slab at the slab's corners)."""
+    n = int(g.vol_shape[0])
+    if isinstance(g, cm.FanGeometry):
+        q = F.fbp_filter(raw.double().cpu(), g).numpy()
+        rows = (n // 4, n // 2)
+        ref = np.stack([oracle.bp2d(g, q, rows=(r, r + 1), wpow=2)[r] for r in rows])
+        return served.double().cpu().numpy()[list(rows)], ref
+    V, U = (int(v) for v in g.det_shape)
+    z = n // 2
+    M = g.plan.bp_matrices("fdk").astype(np.float64)
+    c = (n - 1) / 2.0
+    corners = np.array([[x, y, z - c, 1.0] for x in (-c, c) for y in (-c, c)])
+    P = np.einsum("aij,kj->aki", M, corners)
+    v = P[..., 1] / P[..., 2]
+    v0, v1 = max(0, int(math.floor(v.min())) - 2), min(V, int(math.ceil(v.max())) + 3)
+    H = F.ramp_response(U).double()
+    cw = F.cosine_weights_cone(g).double()
+    scale = (math.pi / g.nangles) * g.DSD / (float(g.det_spacing[1]) * g.DSO)
+    q = np.zeros((g.nangles, V, U))
+    q[:, v0:v1] = (F.filter_rows(raw[:, v0:v1].double().cpu() * cw[v0:v1], H) * scale).numpy()
+    ref = oracle.bp3d(g, q, slabs=(z, z + 1), fdk=True)[z]
+    return served[z].double().cpu().numpy(), ref
+
+
+def test_every_job_class_of_the_mix_against_fp64(oracle, gpu):
+    """One job of each class of the serving mix (fan n = 128..2048, cone n = 128..384, FP16
+    storage), served together; rel-L2 against the FP64 pipeline on a subset <= 1e-3."""
+    classes = [("fan", n) for n in (128, 256, 512, 1024, 2048)] + \
+              [("cone", n) for n in (128, 256, 384)]
+    jobs, raws = [], []
+    for i, (kind, n) in enumerate(classes):
+        g = _c5_geometry(kind, n, torch.float16)
+        g32 = _c5_geometry(kind, n, 32)
+        if kind == "fan":
+            x = torch.tensor(phantom.ellipses_2d(n, seed=i), dtype=torch.float32, device="cuda")
+        else:
+            x = torch.tensor(phantom.ellipsoids_3d(n, seed=i), device="cuda")
+        raws.append(cm.forward(x, g32))
+        jobs.append(FBPJob(geom=g, projections=raws[-1], tag=f"{kind}{n}"))
+    torch.cuda.synchronize()
+    FBPServer(hbm_budget=64 << 30).run(jobs)
+    worst = 0.0
+    for j, raw in zip(jobs, raws):
+        got, ref = _fp64_subset(oracle, j.volume, raw, j.geom)
+        r = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        print(f"{j.tag}: FP16 storage rel-L2 vs FP64 {r:.3e}", flush=True)
+        worst = max(worst, r)
+    assert worst <= 1e-3

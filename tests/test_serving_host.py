@@ -120,3 +120,20 @@ def test_fan2d_bp_table_matches_plan_formula_for_python_floats():
     du = 0.7
     assert np.float32(1.0 / du) == C.c_float(1.0 / du).value
     assert math.isclose(_lib.fan2d_bp_table([0.5], [10.0], [15.0])[0, 2], 15.0)
+
+
+def test_y_invariance_flag():
+    """Circular orbits about the y axis leave U and W free of y in every mode, also with the
+    panel turned about y (pitch); a rolled panel or an out-of-plane angle does not."""
+    circ = cone(10, 14, 12, 6, 30.0, 50.0, det_offset=(0.5, -0.7), cor=0.4)
+    for mode in ("reference", "matched", "fdk"):
+        M, yinv = _plan3d(circ).bp_matrices(mode, with_layout=True)
+        assert yinv, mode
+        assert np.all(M[:, 0, 1] == 0) and np.all(M[:, 2, 1] == 0)
+    pitched = cone(10, 14, 12, 6, 30.0, 50.0, det_rotation=(0.02, 0.0, 0.0))
+    assert _plan3d(pitched).bp_matrices("fdk", with_layout=True)[1]
+    rolled = cone(10, 14, 12, 6, 30.0, 50.0, det_rotation=(0.0, 0.02, 0.0))
+    assert not _plan3d(rolled).bp_matrices("fdk", with_layout=True)[1]
+    ang = np.vstack([np.linspace(0, 6, 6), np.full(6, 0.1), np.zeros(6)])
+    oblique = cone(10, 14, 12, 6, 30.0, 50.0, angles=ang)
+    assert not _plan3d(oblique).bp_matrices("matched", with_layout=True)[1]
