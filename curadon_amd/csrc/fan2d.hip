@@ -11,9 +11,12 @@
 //                ns   [A*D] float4               (world step length, N as int bits, -N/2, 0)
 // Sample j of a ray sits at m + (j - N/2) v. Coordinates are "centred padded": padded-copy
 // texel coordinate minus 0.5 (so floor() is the lower tap) minus the integer centre c0 of the
-// padded copy. Anchoring at the ray's midpoint near the image centre keeps |m| and |x| small,
-// so the FP32 rounding of the anchor and of each position stays far below what the same
-// values would carry at texel coordinates up to n, and it is not shared by all samples.
+// padded copy. Anchoring at the ray's midpoint near the image centre keeps |m| small. The step
+// is +-1 along the ray's dominant axis, so with jf = j - N/2, md the midpoint's dominant
+// coordinate and s the step's sign, sample jf's dominant lower tap is the integer
+// floor(md) + s jf with weight fract(md) (exact, one row per sample); its other coordinate is
+// c = fma(jf, v_other, m_other) in FP32, lower tap floor(c), weight fract(c). The forward and
+// the adjoint both use exactly this, so they are each other's transpose.
 //   per call     for each image b of the batch, one block of 2 * (nx+4) * (ny+4) elements:
 //                pad  [ny+4][nx+4]               image with a 2-texel zero border
 //                padT [nx+4][ny+4]               the same, transposed
@@ -137,19 +140,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_sino2d(const typename S::ra
 
 // ---------------------------------------------------------------- forward ------------------
 
-// Bilinear sample at centred padded coordinate (x, y) from a copy whose rows run along the
-// dominant axis: floor() gives the lower tap relative to the copy's centre, fract() its
-// weight, as the linear texture filter does. `cofs` is the element offset of the copy's centre
-// inside the image's block, `rstride` the copy's row length in elements.
-template <class SI, bool YDOM>
-__device__ __forceinline__ float sample2d(rsrc_t R, float x, float y, int rstride, int soff1,
-                                         int cofs) {
-    const int ix = floor_i32(x), iy = floor_i32(y);
-    const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
-    const int o = (YDOM ? __mul24(iy, rstride) + ix : __mul24(ix, rstride) + iy) + cofs;
-    const Pair p0 = buf_pair<SI>(R, o, 0);     // YDOM: (ix, iy),(ix+1, iy)  else (ix, iy),(ix, iy+1)
-    const Pair p1 = buf_pair<SI>(R, o, soff1); // YDOM: row iy+1             else column ix+1
-    const float fa = YDOM ? fx : fy, fb = YDOM ? fy : fx;
+// Bilinear sample from global memory: dominant row `row` (lower tap) with weight fb, other
+// coordinate c, in a copy whose rows run along the dominant axis (`rstride` elements per row,
+// `soff1` bytes to the next row, `cofs` the element offset of the copy's centre).
+template <class SI>
+__device__ __forceinline__ float sample_row(rsrc_t R, int row, float fb, float c, int rstride,
+                                           int soff1, int cofs) {
+    const int o = __mul24(row, rstride) + floor_i32(c) + cofs;
+    const float fa = __builtin_amdgcn_fractf(c);
+    const Pair p0 = buf_pair<SI>(R, o, 0);     // row, taps c and c + 1
+    const Pair p1 = buf_pair<SI>(R, o, soff1); // row + 1
     const float u0 = fmaf(fa, p0.hi - p0.lo, p0.lo);
     const float u1 = fmaf(fa, p1.hi - p1.lo, p1.lo);
     return fmaf(fb, u1 - u0, u0);
@@ -157,29 +157,26 @@ __device__ __forceinline__ float sample2d(rsrc_t R, float x, float y, int rstrid
 
 // Row-synchronous march through LDS bands.
 //
-// Sample j of a ray sits at m + (j - N/2) v, and the step v is +-1 along the ray's dominant
-// axis (forward_2d.hpp:63-78 normalise the step by its largest component), so a ray has one
-// sample per dominant row. A workgroup holds adjacent detector cells of one angle and walks the
-// dominant rows tau in increasing order: in row tau every lane whose dominant axis is the
-// pass's takes its sample on that row, jf = s (tau - floor(md)) relative to its midpoint (md
-// its midpoint coordinate along the dominant axis, s the step's sign), if jf lies in its range
-// [q.z, q.z + N - 1]. The rows come in bands of kBand, aligned to multiples of kBand: the
-// workgroup stages rows [t0 - 1, t0 + kBand + 1] of the copy whose rows run along the dominant
-// axis, over the columns its samples in the band can touch, into LDS (rows of kPitch floats,
-// filled with 4-element row loads) and interpolates from there. A band wider than kPitch is
-// sampled from global memory instead, with the same arithmetic. Bands in which every lane of
-// the workgroup has a sample on every row skip the per-sample range test.
+// A ray has one sample per dominant row: sample jf sits on row floor(md) + s jf. A workgroup
+// holds adjacent detector cells of one angle and walks the dominant rows tau in increasing
+// order: on row tau every lane whose dominant axis is the pass's takes its sample
+// jf = s (tau - floor(md)), if jf lies in its range [q.z, q.z + N - 1], so all lanes read the
+// same two rows. The rows come in bands of kBand, aligned to multiples of kBand: the workgroup
+// stages rows [t0, t0 + kBand] of the copy whose rows run along the dominant axis, over the
+// columns its samples in the band touch, into LDS (rows of kPitch floats, filled with
+// 4-element row loads) and interpolates from there. A band wider than kPitch is sampled from
+// global memory instead, with the same arithmetic. Bands in which every lane of the workgroup
+// has a sample on every row skip the per-sample range test.
 //
 // Staged cells outside the copy (band corners beyond its edge) hold whatever the range-checked
-// loads return; no sample of a ray reads them, because every tap of a sample lies inside the
-// copy's zero border.
+// loads return; no sample reads them, because every tap of a sample lies inside the copy's zero
+// border.
 //
 // Summation: samples on even and odd rows go to two partial sums, which are added to the total
 // at the end of every band. Bands and parities are absolute row numbers, so a ray's sum does
-// not depend on which rays share its workgroup (and so not on the launch shape). Positions are
-// the same FP32 expressions as the adjoint's.
+// not depend on which rays share its workgroup (and so not on the launch shape).
 constexpr int kBand = 32;              // rows per band and per partial sum
-constexpr int kBandRows = kBand + 3;   // staged rows: one above the band, two below
+constexpr int kBandRows = kBand + 1;   // staged rows: the band and the row below it
 // LDS row pitch in floats: a multiple of 4 (16-byte row stores) and below 255, so that one
 // address reaches a tap pair and the pair below it with two ds_read2_b32 immediate offsets.
 // 128 rays at up to 1.1 texels per cell plus a band's slope and the tap margins fit.
@@ -219,15 +216,12 @@ __device__ __forceinline__ int floor_div(int a, int b) {
     return a >= 0 ? a / b : -((-a + b - 1) / b);
 }
 
-// Bilinear sample from the staged band at element `idx` (lower tap: row i, column c): the pair
-// (c, c + 1) runs along the non-dominant axis, rows i and i + 1 along the dominant one; fa / fb
-// are the fractions along those axes.
-__device__ __forceinline__ float band_sample(const float *band, unsigned idx, float fa,
-                                             float fb) {
-    const float a0 = band[idx], a1 = band[idx + 1];
-    const float b0 = band[idx + kPitch], b1 = band[idx + kPitch + 1];
-    const float u0 = fmaf(fa, a1 - a0, a0);
-    const float u1 = fmaf(fa, b1 - b0, b0);
+// Bilinear sample from the staged band: `p` points at the lower tap (row, column) of a row of
+// kPitch floats; the pair runs along the non-dominant axis with fraction fa, the rows along
+// the dominant one with fraction fb.
+__device__ __forceinline__ float band_sample(const float *p, float fa, float fb) {
+    const float u0 = fmaf(fa, p[1] - p[0], p[0]);
+    const float u1 = fmaf(fa, p[kPitch + 1] - p[kPitch], p[kPitch]);
     return fmaf(fb, u1 - u0, u0);
 }
 
@@ -236,17 +230,18 @@ __device__ __forceinline__ float band_sample(const float *band, unsigned idx, fl
 //   rstride      elements per row of the copy (rows run along the dominant axis)
 //   cofs         element offset of the copy's centre (centred coordinate 0, 0)
 template <class SI, bool YDOM>
-__device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool mine, int rstride, int cofs,
-                            float *band, int3 *red, int nthreads) {
+__device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool mine, int rstride,
+                                            int cofs, float *band, int3 *red, int nthreads) {
     const int n = __float_as_int(q.y);
     const bool act = mine && n > 0;
     if (!__syncthreads_or(act)) return 0.0f;
     const int nwaves = nthreads >> 6;
     const float md = YDOM ? e.y : e.x;   // dominant coordinate of the midpoint
     const float sd = YDOM ? e.w : e.z;   // dominant step, +-1
-    const float mc = YDOM ? e.x : e.y;   // non-dominant coordinate of the midpoint
-    const float vc = YDOM ? e.z : e.w;   // non-dominant step
+    const float mc = YDOM ? e.x : e.y;   // other coordinate of the midpoint
+    const float vc = YDOM ? e.z : e.w;   // other step
     const int r0 = (int)floorf(md);
+    const float fb = md - (float)r0;     // dominant weight of every sample (exact)
     const int s = sd > 0.0f ? 1 : -1;
     const float sf = (float)s;
     const int jlo = (int)q.z, jhi = jlo + n - 1;
@@ -254,15 +249,13 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
     const int tz = ta + n - 1;                  // last row
     const int3 tr = block_reduce(act ? ta : INT_MAX, act ? tz : INT_MIN, 0, red, nwaves);
     const unsigned span = (unsigned)(n - 1);
-    const float jmin = (float)jlo, jmax = (float)(act ? jhi : jlo);
     const int soff1 = rstride * (int)sizeof(typename SI::raw);
-    const unsigned last = (unsigned)((kBandRows - 1) * kPitch - 2); // highest lower tap index
     float acc = 0.0f;
     for (int t0 = floor_div(tr.x, kBand) * kBand; t0 <= tr.y; t0 += kBand) {
         const int b_lo = max(tr.x, t0) & ~1, b_hi = min(tr.y, t0 + kBand - 1);
-        // columns the lane's samples in this band touch (the non-dominant coordinate is linear
-        // in the row, so its extremes are at the lane's first and last row of the band), and
-        // whether the lane has a sample on every row [b_lo, b_hi + 1]
+        // columns the lane's samples in this band touch (c is monotonic in jf, so its
+        // extremes are at the lane's first and last row of the band), and whether the lane
+        // has a sample on every row [b_lo, b_hi | 1]
         const int f = max(ta, b_lo), l = min(tz, b_hi);
         int cmin = INT_MAX, cmax = INT_MIN;
         if (act && f <= l) {
@@ -271,18 +264,17 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
             cmin = min(c1, c2);
             cmax = max(c1, c2);
         }
-        const int full_lane = act && ta <= b_lo && tz >= ((b_hi | 1)) ? 1 : 0;
+        const int full_lane = act && ta <= b_lo && tz >= (b_hi | 1) ? 1 : 0;
         const int3 cr = block_reduce(cmin, cmax, full_lane, red, nwaves);
-        const int c_lo = cr.x - 1, W = cr.y + 3 - c_lo; // columns [cr.x - 1, cr.y + 2]
-        const int r_lo = t0 - 1;
+        const int c_lo = cr.x, W = cr.y + 2 - c_lo; // columns [cr.x, cr.y + 1]
         float s0 = 0.0f, s1 = 0.0f;
         float jf = (float)(s * (b_lo - r0));
         if (W <= kPitch) {
-            // stage rows [r_lo, r_lo + kBandRows) x columns [c_lo, c_lo + 4 Q) as FP32
-            const int Q = (W + 3) >> 2; // >= 1 (W >= 4)
+            // stage rows [t0, t0 + kBandRows) x columns [c_lo, c_lo + 4 Q) as FP32
+            const int Q = (W + 3) >> 2; // >= 1
             // i / Q as a multiply-high, exact for i < 2^16 (Q = 1: the identity)
             const unsigned magic = Q == 1 ? 0u : 0xffffffffu / (unsigned)Q + 1u;
-            const int g0 = r_lo * rstride + c_lo + cofs;
+            const int g0 = t0 * rstride + c_lo + cofs;
             const int items = kBandRows * Q;
             // four items per thread and trip, loads first: an index past the end repeats the
             // last item (same value to the same cell)
@@ -302,54 +294,44 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
                         make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
             }
             __syncthreads();
-            const int base = r_lo * kPitch + c_lo;
+            // lane pointer of row t = band + (t - t0) * kPitch - c_lo + floor(c); the even
+            // and odd rows of a pair advance separately so that each reaches its row below by
+            // an immediate offset
+            const float *rowa = band + (b_lo - t0) * kPitch - c_lo;
+            const float *rowb = rowa + kPitch;
             if (cr.z) { // every lane has a sample on every row of the band
-                for (int t = b_lo; t <= b_hi; t += 2, jf += 2.0f * sf) {
-                    const float jb = jf + sf;
-                    const float xa = fmaf(jf, e.z, e.x), ya = fmaf(jf, e.w, e.y);
-                    const float xb = fmaf(jb, e.z, e.x), yb = fmaf(jb, e.w, e.y);
-                    const float da = YDOM ? ya : xa, ca = YDOM ? xa : ya;
-                    const float db = YDOM ? yb : xb, cb = YDOM ? xb : yb;
-                    // (the min changes nothing here; its bound lets the compiler fold the second
-                    // row's offset into the LDS instruction)
-                    const unsigned ia = min(
-                        (unsigned)(__mul24(floor_i32(da), kPitch) + floor_i32(ca) - base), last);
-                    const unsigned ib = min(
-                        (unsigned)(__mul24(floor_i32(db), kPitch) + floor_i32(cb) - base), last);
-                    s0 += band_sample(band, ia, __builtin_amdgcn_fractf(ca),
-                                      __builtin_amdgcn_fractf(da));
-                    s1 += band_sample(band, ib, __builtin_amdgcn_fractf(cb),
-                                      __builtin_amdgcn_fractf(db));
+                for (int t = b_lo; t <= b_hi;
+                     t += 2, jf += 2.0f * sf, rowa += 2 * kPitch, rowb += 2 * kPitch) {
+                    const float ca = fmaf(jf, vc, mc), cb = fmaf(jf + sf, vc, mc);
+                    s0 += band_sample(rowa + floor_i32(ca), __builtin_amdgcn_fractf(ca), fb);
+                    s1 += band_sample(rowb + floor_i32(cb), __builtin_amdgcn_fractf(cb), fb);
                 }
             } else {
-                for (int t = b_lo; t <= b_hi; t += 2, jf += 2.0f * sf) {
-                    const float ja = __builtin_amdgcn_fmed3f(jf, jmin, jmax);
-                    const float jb = __builtin_amdgcn_fmed3f(jf + sf, jmin, jmax);
-                    const float xa = fmaf(ja, e.z, e.x), ya = fmaf(ja, e.w, e.y);
-                    const float xb = fmaf(jb, e.z, e.x), yb = fmaf(jb, e.w, e.y);
-                    const float da = YDOM ? ya : xa, ca = YDOM ? xa : ya;
-                    const float db = YDOM ? yb : xb, cb = YDOM ? xb : yb;
-                    const unsigned ia = min(
-                        (unsigned)(__mul24(floor_i32(da), kPitch) + floor_i32(ca) - base), last);
-                    const unsigned ib = min(
-                        (unsigned)(__mul24(floor_i32(db), kPitch) + floor_i32(cb) - base), last);
-                    const float va = band_sample(band, ia, __builtin_amdgcn_fractf(ca),
-                                                 __builtin_amdgcn_fractf(da));
-                    const float vb = band_sample(band, ib, __builtin_amdgcn_fractf(cb),
-                                                 __builtin_amdgcn_fractf(db));
+                const int cw = W - 2; // highest lower-tap column offset
+                for (int t = b_lo; t <= b_hi;
+                     t += 2, jf += 2.0f * sf, rowa += 2 * kPitch, rowb += 2 * kPitch) {
+                    const float ca = fmaf(jf, vc, mc), cb = fmaf(jf + sf, vc, mc);
+                    // lanes without a sample on the row read a clamped cell, masked below
+                    const int ia = min(max(floor_i32(ca) - c_lo, 0), cw) + c_lo;
+                    const int ib = min(max(floor_i32(cb) - c_lo, 0), cw) + c_lo;
+                    const float va = band_sample(rowa + ia, __builtin_amdgcn_fractf(ca), fb);
+                    const float vb = band_sample(rowb + ib, __builtin_amdgcn_fractf(cb), fb);
                     s0 += act && (unsigned)(t - ta) <= span ? va : 0.0f;
                     s1 += act && (unsigned)(t + 1 - ta) <= span ? vb : 0.0f;
                 }
             }
             __syncthreads(); // the band is read before the next one is staged
         } else {
+            // lanes without a sample on the row sample their nearest own sample, masked below
+            const float jmin = (float)jlo, jmax = (float)(act ? jhi : jlo);
             for (int t = b_lo; t <= b_hi; t += 2, jf += 2.0f * sf) {
                 const float ja = __builtin_amdgcn_fmed3f(jf, jmin, jmax);
                 const float jb = __builtin_amdgcn_fmed3f(jf + sf, jmin, jmax);
-                const float va = sample2d<SI, YDOM>(R, fmaf(ja, e.z, e.x), fmaf(ja, e.w, e.y),
-                                                    rstride, soff1, cofs);
-                const float vb = sample2d<SI, YDOM>(R, fmaf(jb, e.z, e.x), fmaf(jb, e.w, e.y),
-                                                    rstride, soff1, cofs);
+                const int rowa = r0 + s * (int)ja, rowb = r0 + s * (int)jb;
+                const float va = sample_row<SI>(R, rowa, fb, fmaf(ja, vc, mc), rstride, soff1,
+                                                cofs);
+                const float vb = sample_row<SI>(R, rowb, fb, fmaf(jb, vc, mc), rstride, soff1,
+                                                cofs);
                 s0 += act && (unsigned)(t - ta) <= span ? va : 0.0f;
                 s1 += act && (unsigned)(t + 1 - ta) <= span ? vb : 0.0f;
             }
@@ -418,9 +400,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp2d(typename SV::raw *__restrict
 
 // Exact transpose of k_fwd2d in gather form: texel (tx, ty) collects, for every angle, the
 // samples of the rays whose bilinear footprint contains it. Candidate rays come from projecting
-// the footprint box [tx-.5, tx+1.5) x [ty-.5, ty+1.5) onto the detector; candidate samples
-// from the ray's dominant axis; each candidate is then re-evaluated with the forward kernel's
-// own FP32 expressions, so the weights are the forward's.
+// the footprint box [tx-.5, tx+1.5) x [ty-.5, ty+1.5) onto the detector; along a ray's
+// dominant axis exactly two samples can touch the texel, and their weights are evaluated with
+// the forward's own expressions.
 template <class SV, class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_adj2d(typename SV::raw *__restrict__ vol,
                                                const typename SS::raw *__restrict__ sino,
@@ -463,25 +445,29 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d(typename SV::raw *__restric
             if (n == 0) continue;
             const float4 e = ev[r];
             const bool ydom = fabsf(e.w) == 1.0f;
-            // samples whose dominant coordinate can floor to p-1 or p; jf = j - N/2
-            const float md = ydom ? e.y : e.x, vd = ydom ? e.w : e.z;
-            const float td = ydom ? fpy : fpx;
-            const float lo = (td - md) * vd - 1.0f;
-            const float jlo = fmaxf(floorf(lo - 1e-3f), q.z);
-            const float jhi = fminf(floorf(lo + 2.0f + 1e-3f), q.z + (float)(n - 1));
-            if (jlo > jhi) continue;
-            const float yv = buf_load<SS>(R, (int)r) * q.x;
+            // the forward's samples (fan2d.hip header): sample jf has dominant lower tap
+            // floor(md) + s jf with weight fract(md), so the texel's dominant index pd is
+            // touched by jf = s (pd - r0) (weight 1 - fb) and jf = s (pd - 1 - r0) (weight fb)
+            const float md = ydom ? e.y : e.x, sd = ydom ? e.w : e.z;
+            const float mc = ydom ? e.x : e.y, vc = ydom ? e.z : e.w;
+            const int pd = ydom ? py : px, pc = ydom ? px : py;
+            const int r0 = (int)floorf(md);
+            const float fb = md - (float)r0;
+            const int s = sd > 0.0f ? 1 : -1;
+            const int jlo = (int)q.z, jhi = jlo + n - 1;
             float part = 0.0f;
-            for (float jf = jlo; jf <= jhi; jf += 1.0f) {
-                // the forward march's own expressions (march2d)
-                const float x = fmaf(jf, e.z, e.x);
-                const float y = fmaf(jf, e.w, e.y);
-                const int ix = (int)floorf(x), iy = (int)floorf(y);
-                const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
-                const float wx = ix == px ? 1.0f - fx : (ix == px - 1 ? fx : 0.0f);
-                const float wy = iy == py ? 1.0f - fy : (iy == py - 1 ? fy : 0.0f);
-                part = fmaf(wx, wy, part);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int j = s * (pd - h - r0);
+                if (j < jlo || j > jhi) continue;
+                const float c = fmaf((float)j, vc, mc);
+                const int ic = floor_i32(c);
+                const float fc = __builtin_amdgcn_fractf(c);
+                const float wc = ic == pc ? 1.0f - fc : (ic == pc - 1 ? fc : 0.0f);
+                part = fmaf(h == 0 ? 1.0f - fb : fb, wc, part);
             }
+            if (part == 0.0f) continue;
+            const float yv = buf_load<SS>(R, (int)r) * q.x;
             acc = fmaf(part, yv, acc);
         }
     }
