@@ -86,41 +86,167 @@ __device__ __forceinline__ void bp3d_column(rsrc_t R, const float *__restrict__ 
     }
 }
 
-// Sum over angles for kYB consecutive voxels of one (x, z) row along y, for matrices whose U and
-// W rows do not depend on y (M[a][1] == M[a][9] == 0: circular source orbits about the y axis
-// with an untilted panel). The perspective divide, the u tap and weight and the distance
-// weight are then shared by the whole run; only V advances along y. (hx, hy, hz) is the centred
-// index of the run's first voxel. Semantics as bp3d_column.
+// ------------------------------------------------------------------------------------------
+// LDS-staged y runs, for matrices whose U and W rows do not depend on y (M[a][1] == M[a][9] == 0:
+// circular orbits about the y axis, panel turned at most about y). Then a run of voxels along y
+// shares its perspective divide, u tap, u weight and distance weight, and v advances by a
+// constant step.
+//
+// A workgroup of 256 threads owns a tile of kTileX x kYB x kTileZ voxels (x, y, z); thread t
+// takes the run of kYB y voxels at (x0 + t % kTileX, z0 + t / kTileX). Angles go in chunks of
+// kChunk: every wavefront computes the chunk's panel footprints from the tile's 8 corners (8
+// lanes per angle; U/W and V/W are linear-fractional, so over a box where W > 0 their extremes
+// are at corners), the workgroup stages the footprints (rows of at most kFootW columns, at most
+// kFootRows rows, from the zero-bordered panel) into LDS and the runs sample from there. A chunk
+// with a footprint that does not fit (or a corner with W <= 0) is sampled from global memory
+// instead, with the same arithmetic.
+//
+// Per voxel: u = clamp(U/W), v = clamp(V0/W + y dv) with dv = M[a][5]/W, lower taps floor(u),
+// floor(v) in the staged panel, bilinear weights fract(u), fract(v); MATCHED drops W <= 0, FDK
+// weights by 1/W^2.
+constexpr int kTileX = 16, kTileZ = 16;
+constexpr int kFootW = 64;                        // staged columns per row
+constexpr int kFootCols = 68;                     // LDS row pitch: 16-byte rows, banks shifted
+constexpr int kFootRows = 44;
+constexpr int kFootSlot = kFootCols * kFootRows;  // floats per staged angle
+constexpr int kChunk = 4;                         // angles per chunk: 4 x 12 KB of LDS
+constexpr int kFootFloats = kChunk * kFootSlot;
+
+// Per-angle set-up of a run: u tap column (element offset of the lower tap's column relative to
+// `base`), u weight, first v, v step and the sample weight.
+struct RunAngle {
+    int col;
+    float fu, v, dv, w;
+};
+
+template <bool MATCHED, bool FDK>
+__device__ __forceinline__ RunAngle run_angle(const float *m, float hx, float hy, float hz,
+                                              float ulim) {
+    const float Uh = fmaf(m[0], hx, fmaf(m[2], hz, m[3]));
+    const float Wh = fmaf(m[8], hx, fmaf(m[10], hz, m[11]));
+    const float Vh = fmaf(m[4], hx, fmaf(m[5], hy, fmaf(m[6], hz, m[7])));
+    const float rw = __builtin_amdgcn_rcpf(Wh);
+    const float u = __builtin_amdgcn_fmed3f(Uh * rw, -2.0f, ulim);
+    RunAngle r;
+    r.col = floor_i32(u);
+    r.fu = __builtin_amdgcn_fractf(u);
+    r.v = Vh * rw;
+    r.dv = m[5] * rw;
+    r.w = FDK ? rw * rw : 1.0f;
+    if (MATCHED) r.w = Wh > 0.0f ? r.w : 0.0f;
+    return r;
+}
+
+// Staged footprints: one workgroup-shared array (47.9 KB) for every instantiation a kernel
+// uses, so a kernel that dispatches between modes allocates it once.
+static __shared__ __align__(16) float foot_lds[kFootFloats];
+
+// (hx, hy, hz): centred index of this thread's run start; (tx, ty, tz): of the tile origin.
+// All 256 threads of the workgroup call it together (lanes without voxels too).
 template <class SS, bool MATCHED, bool FDK>
-__device__ __forceinline__ void bp3d_yrun(rsrc_t R, const float *__restrict__ M, int A, int U,
-                                         int V, float hx, float hy, float hz,
-                                         float (&acc)[kYB]) {
+__device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, const float *__restrict__ M, int A, int U,
+                                             int V, float hx, float hy, float hz, float tx,
+                                             float ty, float tz, float (&acc)[kYB]) {
     const int Up = U + 2 * kPadDet, Vp = V + 2 * kPadDet;
     const int panel = Up * Vp;
     const int sRow = Up * (int)sizeof(typename SS::raw);
     const float ulim = (float)U, vlim = (float)V;
-    int base = kPadDet * Up + kPadDet;
-    for (int a = 0; a < A; ++a, base += panel) {
-        const float *m = M + 12 * a; // wave-uniform: scalar loads
-        const float Uh = fmaf(m[0], hx, fmaf(m[2], hz, m[3]));
-        const float Wh = fmaf(m[8], hx, fmaf(m[10], hz, m[11]));
-        float Vh = fmaf(m[4], hx, fmaf(m[5], hy, fmaf(m[6], hz, m[7])));
-        const float rw = __builtin_amdgcn_rcpf(Wh);
-        const float u = __builtin_amdgcn_fmed3f(Uh * rw, -2.0f, ulim);
-        const float fu = __builtin_amdgcn_fractf(u);
-        const int col = base + (int)floorf(u);
-        float w = FDK ? rw * rw : 1.0f;
-        if (MATCHED) w = Wh > 0.0f ? w : 0.0f;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, k8 = lane >> 3, c8 = lane & 7;
+    // the tile's corner this lane evaluates
+    const float cx = (c8 & 1) ? tx + (float)(kTileX - 1) : tx;
+    const float cy = (c8 & 2) ? ty + (float)(kYB - 1) : ty;
+    const float cz = (c8 & 4) ? tz + (float)(kTileZ - 1) : tz;
+    for (int a0 = 0; a0 < A; a0 += kChunk) {
+        const int na = min(kChunk, A - a0);
+        // footprints: lanes 8k .. 8k+7 take angle a0 + k
+        float umin = 0.0f, umax = 0.0f, vmin = 0.0f, vmax = 0.0f;
+        int ok = 1;
+        if (k8 < na) {
+            const float *m = M + 12 * (a0 + k8);
+            const float Uh = fmaf(m[0], cx, fmaf(m[2], cz, m[3]));
+            const float Wh = fmaf(m[8], cx, fmaf(m[10], cz, m[11]));
+            const float Vh = fmaf(m[4], cx, fmaf(m[5], cy, fmaf(m[6], cz, m[7])));
+            const float rw = __builtin_amdgcn_rcpf(Wh);
+            umin = umax = __builtin_amdgcn_fmed3f(Uh * rw, -2.0f, ulim);
+            vmin = vmax = __builtin_amdgcn_fmed3f(Vh * rw, -2.0f, vlim);
+            ok = Wh > 0.0f ? 1 : 0;
+        }
 #pragma unroll
-        for (int y = 0; y < kYB; ++y) {
-            const float v = __builtin_amdgcn_fmed3f(Vh * rw, -2.0f, vlim);
-            const float fv = __builtin_amdgcn_fractf(v);
-            const int o = col + __mul24((int)floorf(v), Up);
-            const float r0 = lerp_pair(buf_pair<SS>(R, o, 0), fu);
-            const float r1 = lerp_pair(buf_pair<SS>(R, o, sRow), fu);
-            const float s = fmaf(fv, r1 - r0, r0);
-            acc[y] = (MATCHED || FDK) ? fmaf(s, w, acc[y]) : acc[y] + s;
-            Vh += m[5];
+        for (int off = 1; off < 8; off <<= 1) {
+            umin = fminf(umin, __shfl_xor(umin, off, 64));
+            umax = fmaxf(umax, __shfl_xor(umax, off, 64));
+            vmin = fminf(vmin, __shfl_xor(vmin, off, 64));
+            vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+            ok = min(ok, __shfl_xor(ok, off, 64));
+        }
+        // slot k: padded-panel origin (su0, sv0) and row count; lower taps lie within
+        // [floor(min) - 1, floor(max) + 1] (one-texel margin for the runs' accumulated v and
+        // rounding) and the upper tap adds one more row / column
+        int su0[kChunk], sv0[kChunk], srows[kChunk];
+        bool fits = true;
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const float kumin = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(umin), 8 * k));
+            const float kumax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(umax), 8 * k));
+            const float kvmin = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vmin), 8 * k));
+            const float kvmax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vmax), 8 * k));
+            const int kok = __builtin_amdgcn_readlane(ok, 8 * k);
+            const int ulo = (int)floorf(kumin) - 1 + kPadDet, uhi = (int)floorf(kumax) + 2 + kPadDet;
+            const int vlo = (int)floorf(kvmin) - 1 + kPadDet, vhi = (int)floorf(kvmax) + 2 + kPadDet;
+            su0[k] = max(ulo, 0);
+            sv0[k] = max(vlo, 0);
+            srows[k] = min(vhi, Vp - 1) - sv0[k] + 1;
+            if (k < na)
+                fits = fits && kok && uhi - ulo + 1 <= kFootW && vhi - vlo + 1 <= kFootRows;
+        }
+        if (!fits) { // from global memory
+            for (int k = 0; k < na; ++k) {
+                const float *m = M + 12 * (a0 + k); // wave-uniform: scalar loads
+                RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
+                const int col = (a0 + k) * panel + kPadDet * Up + kPadDet + r.col;
+#pragma unroll
+                for (int y = 0; y < kYB; ++y, r.v += r.dv) {
+                    const float v = __builtin_amdgcn_fmed3f(r.v, -2.0f, vlim);
+                    const int o = col + __mul24(floor_i32(v), Up);
+                    const float r0 = lerp_pair(buf_pair<SS>(R, o, 0), r.fu);
+                    const float r1 = lerp_pair(buf_pair<SS>(R, o, sRow), r.fu);
+                    const float s = fmaf(__builtin_amdgcn_fractf(v), r1 - r0, r0);
+                    acc[y] = (MATCHED || FDK) ? fmaf(s, r.w, acc[y]) : acc[y] + s;
+                }
+            }
+            continue;
+        }
+        __syncthreads(); // the previous chunk's samples are read
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            if (k >= na) break;
+            const int g0 = (a0 + k) * panel + sv0[k] * Up + su0[k];
+            for (int i = tid; i < srows[k] * (kFootW / 4); i += 256) {
+                const int row = i >> 4, q = (i & 15) << 2;
+                const Quad v = buf_quad<SS>(R, g0 + row * Up + q);
+                *reinterpret_cast<float4 *>(&foot_lds[k * kFootSlot + row * kFootCols + q]) =
+                    make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            if (k >= na) break;
+            const float *m = M + 12 * (a0 + k); // wave-uniform: scalar loads
+            RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
+            // LDS element of padded (row pv, column pu): slot + (pv - sv0) * kFootCols + pu - su0
+            const int ik = k * kFootSlot + r.col + kPadDet - su0[k] - (sv0[k] - kPadDet) * kFootCols;
+#pragma unroll
+            for (int y = 0; y < kYB; ++y, r.v += r.dv) {
+                const float v = __builtin_amdgcn_fmed3f(r.v, -2.0f, vlim);
+                const int i = ik + __mul24(floor_i32(v), kFootCols);
+                const float r0 = fmaf(r.fu, foot_lds[i + 1] - foot_lds[i], foot_lds[i]);
+                const float b0 = foot_lds[i + kFootCols], b1 = foot_lds[i + kFootCols + 1];
+                const float r1 = fmaf(r.fu, b1 - b0, b0);
+                const float s = fmaf(__builtin_amdgcn_fractf(v), r1 - r0, r0);
+                acc[y] = (MATCHED || FDK) ? fmaf(s, r.w, acc[y]) : acc[y] + s;
+            }
         }
     }
 }

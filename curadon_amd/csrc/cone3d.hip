@@ -281,30 +281,29 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp3d(typename SV::raw *__restrict
     }
 }
 
-// One thread per (ix, iz) and kYB consecutive y, for matrices whose U and W rows do not depend
-// on y (bp3d_yrun); a wavefront covers 64 consecutive x of one (z, y-run), so each y step writes
-// 64 consecutive voxels. vol += sum over angles.
-constexpr int kYRunX = 64, kYRunZ = 4;
-
+// For matrices whose U and W rows do not depend on y: a workgroup of 256 threads owns a tile of
+// kTileX x kYB x kTileZ voxels, each thread a run of kYB y voxels, and the angles' panel
+// footprints are staged in LDS (bp3d_yrun_lds). vol += sum over angles.
 template <class SV, class SS, bool MATCHED, bool FDK>
 __global__ __launch_bounds__(kMaxBlock) void k_bp3d_y(typename SV::raw *__restrict__ vol,
                                                       const typename SS::raw *__restrict__ ppad,
                                                       const float *__restrict__ M, int nx, int ny,
                                                       int nz, int U, int V, int A, float cx,
                                                       float cy, float cz) {
-    const int ix = blockIdx.x * kYRunX + threadIdx.x;
-    const int nzq = (nz + kYRunZ - 1) / kYRunZ;
-    const int iz = (blockIdx.z % nzq) * kYRunZ + threadIdx.y;
-    const size_t b = blockIdx.z / nzq;
-    const int iy0 = blockIdx.y * kYB;
-    if (ix >= nx || iz >= nz) return;
+    const int nzt = (nz + kTileZ - 1) / kTileZ;
+    const int ix0 = blockIdx.x * kTileX, iy0 = blockIdx.y * kYB;
+    const int iz0 = (blockIdx.z % nzt) * kTileZ;
+    const size_t b = blockIdx.z / nzt;
+    const int ix = ix0 + (int)(threadIdx.x % kTileX), iz = iz0 + (int)(threadIdx.x / kTileX);
     const uint32_t stack = (uint32_t)(U + 2 * kPad) * (V + 2 * kPad) * A;
     const rsrc_t R = make_rsrc(ppad + b * (size_t)stack, bytes_of<SS>(stack));
     float acc[kYB];
 #pragma unroll
     for (int y = 0; y < kYB; ++y) acc[y] = 0.0f;
-    bp3d_yrun<SS, MATCHED, FDK>(R, M, A, U, V, (float)ix - cx, (float)iy0 - cy, (float)iz - cz,
-                                acc);
+    bp3d_yrun_lds<SS, MATCHED, FDK>(R, M, A, U, V, (float)ix - cx, (float)iy0 - cy,
+                                    (float)iz - cz, (float)ix0 - cx, (float)iy0 - cy,
+                                    (float)iz0 - cz, acc);
+    if (ix >= nx || iz >= nz) return;
 #pragma unroll
     for (int y = 0; y < kYB; ++y) {
         const int iy = iy0 + y;
@@ -698,9 +697,9 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
         hipLaunchKernelGGL((k_stage_panel<SS>), pg, pb, 0, s, (const RS *)sinogram,
                            (RS *)scratch, U, V, A);
         if (plan->ycols[mode]) {
-            const int nzq = (nz + kYRunZ - 1) / kYRunZ;
-            dim3 bb(kYRunX, kYRunZ), bg((unsigned)((nx + kYRunX - 1) / kYRunX),
-                                        (unsigned)((ny + kYB - 1) / kYB), (unsigned)(nzq * batch));
+            const int nzt = (nz + kTileZ - 1) / kTileZ;
+            dim3 bb(kTileX * kTileZ), bg((unsigned)((nx + kTileX - 1) / kTileX),
+                                         (unsigned)((ny + kYB - 1) / kYB), (unsigned)(nzt * batch));
             auto kern = mode == CMI_BP3D_MATCHED ? k_bp3d_y<SV, SS, true, false>
                         : mode == CMI_BP3D_FDK   ? k_bp3d_y<SV, SS, true, true>
                                                  : k_bp3d_y<SV, SS, false, false>;

@@ -35,7 +35,7 @@ constexpr int kRing = 4;       // launches whose job tables may be in flight at 
 constexpr int kThreads = 256;  // threads per workgroup
 constexpr int kTile2X = 64, kTile2Y = 4;   // fan-beam tile: pixels
 constexpr int kTile3X = 16, kTile3Y = 16;  // cone-beam tile: columns of kZB voxels
-constexpr int kRunX = 64, kRunZ = 4;       // cone-beam tile of y runs: 64 x kYB x 4 voxels
+// cone-beam tile of y runs: kTileX x kYB x kTileZ voxels (bp_tiles.hpp)
 
 // Device copy of a job with the derived values the kernel needs.
 struct DevJob {
@@ -56,8 +56,8 @@ uint64_t tiles_of(const cmi_bp_job &j) {
     if (j.kind == CMI_JOB_FAN2D)
         return (uint64_t)((j.nx + kTile2X - 1) / kTile2X) * ((j.ny + kTile2Y - 1) / kTile2Y);
     if (j.columns == 1)
-        return (uint64_t)((j.nx + kRunX - 1) / kRunX) * ((j.ny + kYB - 1) / kYB) *
-               ((j.nz + kRunZ - 1) / kRunZ);
+        return (uint64_t)((j.nx + kTileX - 1) / kTileX) * ((j.ny + kYB - 1) / kYB) *
+               ((j.nz + kTileZ - 1) / kTileZ);
     return (uint64_t)((j.nx + kTile3X - 1) / kTile3X) * ((j.ny + kTile3Y - 1) / kTile3Y) *
            ((j.nz + kZB - 1) / kZB);
 }
@@ -140,27 +140,33 @@ __device__ __forceinline__ void serve_tile_3d(const DevJob &J, uint32_t t) {
     }
 }
 
-// Cone-beam tile of y runs: the run of kYB voxels k_bp3d_y gives thread (ix, iz).
+// Cone-beam tile of y runs: the tile and runs k_bp3d_y gives block (tx, ty, tz). Every thread
+// of the workgroup calls this (the LDS staging inside synchronises the workgroup).
 template <class SV, class SS>
 __device__ __forceinline__ void serve_tile_3d_y(const DevJob &J, uint32_t t) {
     const uint32_t per_slab = (uint32_t)J.tiles_x * (uint32_t)J.tiles_y;
-    const int zq = (int)(t / per_slab);
+    const int zt = (int)(t / per_slab);
     const uint32_t r = t % per_slab;
-    const int ix = (int)(r % (uint32_t)J.tiles_x) * kRunX + (int)(threadIdx.x % kRunX);
+    const int ix0 = (int)(r % (uint32_t)J.tiles_x) * kTileX;
     const int iy0 = (int)(r / (uint32_t)J.tiles_x) * kYB;
-    const int iz = zq * kRunZ + (int)(threadIdx.x / kRunX);
-    if (ix >= J.nx || iz >= J.nz) return;
+    const int iz0 = zt * kTileZ;
+    const int ix = ix0 + (int)(threadIdx.x % kTileX), iz = iz0 + (int)(threadIdx.x / kTileX);
     const rsrc_t R = make_rsrc(J.sino, J.sino_bytes);
     float acc[kYB];
 #pragma unroll
     for (int y = 0; y < kYB; ++y) acc[y] = 0.0f;
     const float hx = (float)ix - J.cx, hy = (float)iy0 - J.cy, hz = (float)iz - J.cz;
+    const float tx = (float)ix0 - J.cx, tz = (float)iz0 - J.cz;
     if (J.mode == CMI_BP3D_FDK)
-        bp3d_yrun<SS, true, true>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+        bp3d_yrun_lds<SS, true, true>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
+                                      acc);
     else if (J.mode == CMI_BP3D_MATCHED)
-        bp3d_yrun<SS, true, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+        bp3d_yrun_lds<SS, true, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
+                                       acc);
     else
-        bp3d_yrun<SS, false, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+        bp3d_yrun_lds<SS, false, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
+                                        acc);
+    if (ix >= J.nx || iz >= J.nz) return;
     auto *vol = static_cast<typename SV::raw *>(J.volume);
 #pragma unroll
     for (int y = 0; y < kYB; ++y) {
@@ -418,7 +424,7 @@ CMI_API cmi_status cmi_serve_backproject(cmi_server *p, const cmi_bp_job *jobs, 
             d.tiles_x = (d.nx + kTile2X - 1) / kTile2X;
             d.tiles_y = (d.ny + kTile2Y - 1) / kTile2Y;
         } else if (d.columns == 1) {
-            d.tiles_x = (d.nx + kRunX - 1) / kRunX;
+            d.tiles_x = (d.nx + kTileX - 1) / kTileX;
             d.tiles_y = (d.ny + kYB - 1) / kYB;
         } else {
             d.tiles_x = (d.nx + kTile3X - 1) / kTile3X;

@@ -195,8 +195,8 @@ class FBPServer:
                 staged = torch.empty((A, V + 4, U + 4), dtype=self.storage, device=dev)
                 keep = [_lib.stage_projections(y, staged, scale=scale)]
                 volume = torch.zeros((nz, ny, nx), dtype=self.storage, device=dev)
-                if ycols:  # runs of 8 y voxels, tiles of 64 x 8 x 4
-                    ntiles = ((nx + 63) // 64) * ((ny + 7) // 8) * ((nz + 3) // 4)
+                if ycols:  # runs of 8 y voxels, tiles of 16 x 8 x 16
+                    ntiles = ((nx + 15) // 16) * ((ny + 7) // 8) * ((nz + 15) // 16)
                 else:      # columns of 8 z voxels, tiles of 16 x 16 x 8
                     ntiles = ((nx + 15) // 16) * ((ny + 15) // 16) * ((nz + 7) // 8)
                 tile_cost = float(VOXELS_PER_TILE_3D * A * CONE_COST)

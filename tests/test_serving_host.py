@@ -137,3 +137,8 @@ def test_y_invariance_flag():
     ang = np.vstack([np.linspace(0, 6, 6), np.full(6, 0.1), np.zeros(6)])
     oblique = cone(10, 14, 12, 6, 30.0, 50.0, angles=ang)
     assert not _plan3d(oblique).bp_matrices("matched", with_layout=True)[1]
+
+
+def test_job_tiles_y_runs():
+    j = _lib.BPJob(kind=_lib.JOB_CONE3D, columns=1, nx=33, ny=16, nz=17)
+    assert _lib.bp_job_tiles(j) == 3 * 2 * 2
