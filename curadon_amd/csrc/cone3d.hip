@@ -8,7 +8,7 @@
 //                mfdk [A][12] float          matched matrix scaled so that W = (depth / DSO)
 //   per call     forward:  [B][nz+4][ny+4][nx+4] volume with a 2-voxel zero border, x fastest
 //                backward: [B][A][V+4][U+4]      panels with a 2-pixel zero border
-//                adjoint:  [B][nz][ny][nx] FP32 accumulator when the volume is stored in 16 bits
+//                adjoint:  per-ray march table [2][A][V][U] float4 (kernel k_ray_table3d)
 //
 // Forward rays take their discrete quantities (hit, step count) from an FP64 set-up in the
 // kernel, identical to the host oracle's, then march in FP32 from the ray's midpoint in
@@ -129,19 +129,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_panel(const typename S::raw
     ppad[(ab * Vp + vp) * (size_t)Up + up] = val;
 }
 
-__global__ __launch_bounds__(kMaxBlock) void k_fill_zero(float *__restrict__ p, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
-}
-
-// vol[i] += acc[i] in the volume's storage type.
-template <class S>
-__global__ __launch_bounds__(kMaxBlock) void k_add_into(typename S::raw *__restrict__ vol,
-                                                        const float *__restrict__ acc, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) vol[i] = from_f32<S>(to_f32<S>(vol[i]) + acc[i]);
-}
-
 // ---------------------------------------------------------------- forward -----------------
 
 // Trilinear sample at centred padded coordinates from the x-fastest padded volume.
@@ -211,39 +198,152 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
 
 // ---------------------------------------------------------------- adjoint -----------------
 
-// Transposed march: every sample of ray (iu, iv, a) adds y * w to the voxels its trilinear
-// footprint reads, at the forward march's FP32 positions. Float atomics make the summation
-// order, and so the last bits, run-dependent.
-template <class SS>
-__global__ __launch_bounds__(kMaxBlock) void k_adj3d(float *__restrict__ acc,
-                                                     const typename SS::raw *__restrict__ sino,
-                                                     const FwdGeom *__restrict__ geo, int nx,
-                                                     int ny, int nz, int U, int V, int A) {
+// Per-ray march table for the adjoint: rm = (m.xyz, j0), rs = (st.xyz, n as int bits), the same
+// FP32 parameters k_fwd3d derives with march_params; n = 0 for rays that miss the volume.
+__global__ __launch_bounds__(kMaxBlock) void k_ray_table3d(const FwdGeom *__restrict__ geo, int nx,
+                                                          int ny, int nz, int U, int V,
+                                                          float4 *__restrict__ rm,
+                                                          float4 *__restrict__ rs) {
     const int iu = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iv = blockIdx.y * blockDim.y + threadIdx.y;
-    const int a = blockIdx.z % A;
-    const size_t b = blockIdx.z / A;
-    if (iu >= U || iv >= V) return;
-    const float y = to_f32<SS>(sino[((b * A + a) * V + iv) * (size_t)U + iu]);
-    if (y == 0.0f) return;
+    const int iv = blockIdx.y;
+    const int a = blockIdx.z;
+    if (iu >= U) return;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
     const March3 p = march_params(geo[a], nx, ny, nz, iu, iv, Wp, Hp, Dp);
+    const size_t r = ((size_t)a * V + iv) * U + iu;
+    rm[r] = make_float4(p.m[0], p.m[1], p.m[2], p.j0);
+    rs[r] = make_float4(p.st[0], p.st[1], p.st[2], __int_as_float(p.n));
+}
+
+// Matched adjoint, tile-driven: a workgroup owns a kAT^3 voxel tile with FP32 accumulators in
+// LDS. For each angle it projects the tile's box, grown by one voxel on each side (the samples
+// whose trilinear footprint reaches the tile), through the geometry-matched matrices onto the
+// panel; the rays through the covered pixels (plus one pixel of margin) are the candidates. Each
+// candidate walks only its samples inside the grown box, at the forward's FP32 positions, and
+// adds y * w to the tile voxels its footprint covers (LDS float atomics: the last bits depend on
+// the order in which they land). The tile is added to the volume once, in its storage type.
+constexpr int kAT = 16;
+
+template <class SV, class SS>
+__global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restrict__ vol,
+                                                     const typename SS::raw *__restrict__ sino,
+                                                     const float4 *__restrict__ rm,
+                                                     const float4 *__restrict__ rs,
+                                                     const float *__restrict__ M, int nx, int ny,
+                                                     int nz, int U, int V, int A) {
+    __shared__ float tile[kAT * kAT * kAT];
+    const int tid = threadIdx.x;
+    const int ntz = (nz + kAT - 1) / kAT;
+    const int x0 = blockIdx.x * kAT, y0 = blockIdx.y * kAT, z0 = (blockIdx.z % ntz) * kAT;
+    const size_t b = blockIdx.z / ntz;
+    for (int i = tid; i < kAT * kAT * kAT; i += kMaxBlock) tile[i] = 0.0f;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
+    // centred padded coordinate p -> voxel index floor(p) + c
     const int cx = Wp / 2 - kPad, cy = Hp / 2 - kPad, cz = Dp / 2 - kPad;
-    float *vb = acc + b * (size_t)nx * ny * nz;
-    float jf = p.j0;
-    for (int j = 0; j < p.n; ++j, jf += 1.0f) {
-        const float x = fmaf(jf, p.st[0], p.m[0]), yy = fmaf(jf, p.st[1], p.m[1]),
-                    z = fmaf(jf, p.st[2], p.m[2]);
-        const int ix = (int)floorf(x) + cx, iy = (int)floorf(yy) + cy, iz = (int)floorf(z) + cz;
-        const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(yy),
-                    fz = __builtin_amdgcn_fractf(z);
+    // samples reaching the tile: floor(p) + c in [t0 - 1, t0 + kAT - 1] on each axis
+    const float lox = (float)(x0 - 1 - cx), loy = (float)(y0 - 1 - cy), loz = (float)(z0 - 1 - cz);
+    const float hix = lox + (float)(kAT + 1), hiy = loy + (float)(kAT + 1),
+                hiz = loz + (float)(kAT + 1);
+    // the grown box's corner this lane projects (centred voxel index of the matrices)
+    const int c8 = tid & 7;
+    const float qx = (float)((c8 & 1) ? x0 + kAT : x0 - 1) - (float)((nx - 1) / 2.0);
+    const float qy = (float)((c8 & 2) ? y0 + kAT : y0 - 1) - (float)((ny - 1) / 2.0);
+    const float qz = (float)((c8 & 4) ? z0 + kAT : z0 - 1) - (float)((nz - 1) / 2.0);
+    const typename SS::raw *sb = sino + b * (size_t)A * V * U;
+    __syncthreads();
+    for (int a = 0; a < A; ++a) {
+        const float *m = M + 12 * a; // wave-uniform: scalar loads
+        const float Uh = fmaf(m[0], qx, fmaf(m[1], qy, fmaf(m[2], qz, m[3])));
+        const float Vh = fmaf(m[4], qx, fmaf(m[5], qy, fmaf(m[6], qz, m[7])));
+        const float Wh = fmaf(m[8], qx, fmaf(m[9], qy, fmaf(m[10], qz, m[11])));
+        const float rw = __builtin_amdgcn_rcpf(Wh);
+        float umin = Uh * rw, umax = umin, vmin = Vh * rw, vmax = vmin;
+        int ok = Wh > 0.0f ? 1 : 0;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int X = ix + (c & 1), Y = iy + ((c >> 1) & 1), Z = iz + (c >> 2);
-            if (X < 0 || X >= nx || Y < 0 || Y >= ny || Z < 0 || Z >= nz) continue;
-            const float w = ((c & 1) ? fx : 1.0f - fx) * (((c >> 1) & 1) ? fy : 1.0f - fy) *
-                            ((c >> 2) ? fz : 1.0f - fz);
-            atomicAdd(vb + ((size_t)Z * ny + Y) * nx + X, y * w);
+        for (int off = 1; off < 8; off <<= 1) {
+            umin = fminf(umin, __shfl_xor(umin, off, 64));
+            umax = fmaxf(umax, __shfl_xor(umax, off, 64));
+            vmin = fminf(vmin, __shfl_xor(vmin, off, 64));
+            vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+            ok = min(ok, __shfl_xor(ok, off, 64));
+        }
+        // every lane group holds the same values; take lane 0's
+        int iu0 = 0, iu1 = U - 1, iv0 = 0, iv1 = V - 1;
+        if (__builtin_amdgcn_readfirstlane(ok)) {
+            const float u0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(umin)));
+            const float u1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(umax)));
+            const float v0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vmin)));
+            const float v1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vmax)));
+            if (u1 < -2.0f || u0 > (float)(U + 1) || v1 < -2.0f || v0 > (float)(V + 1)) continue;
+            iu0 = max(0, (int)ceilf(u0) - 1);
+            iu1 = min(U - 1, (int)floorf(u1) + 1);
+            iv0 = max(0, (int)ceilf(v0) - 1);
+            iv1 = min(V - 1, (int)floorf(v1) + 1);
+        }
+        const int niu = iu1 - iu0 + 1, niv = iv1 - iv0 + 1;
+        if (niu <= 0 || niv <= 0) continue;
+        const int nc = niu * niv;
+        const float rn = 1.0f / (float)niu; // c / niu by (c + .5) rn, corrected below
+        for (int c = tid; c < nc; c += kMaxBlock) {
+            int jv = (int)(((float)c + 0.5f) * rn);
+            jv += (c - jv * niu >= niu) ? 1 : 0; // one correction step keeps the division exact
+            jv -= (c - jv * niu < 0) ? 1 : 0;    // whatever the rounding of rn
+            const int iu = iu0 + (c - jv * niu), iv = iv0 + jv;
+            const size_t r = ((size_t)a * V + iv) * U + iu;
+            const float4 s4 = rs[r];
+            const int n = __float_as_int(s4.w);
+            if (n <= 0) continue;
+            const float y = to_f32<SS>(sb[r]);
+            if (y == 0.0f) continue;
+            const float4 m4 = rm[r];
+            // jf range with p in [lo, hi) on every axis, one sample of margin each side
+            float jlo = m4.w, jhi = m4.w + (float)(n - 1);
+            const float st[3] = {s4.x, s4.y, s4.z}, mm[3] = {m4.x, m4.y, m4.z};
+            const float lo[3] = {lox, loy, loz}, hi[3] = {hix, hiy, hiz};
+            bool empty = false;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (st[d] != 0.0f) {
+                    const float r0 = __builtin_amdgcn_rcpf(st[d]);
+                    const float t1 = (lo[d] - mm[d]) * r0, t2 = (hi[d] - mm[d]) * r0;
+                    jlo = fmaxf(jlo, fminf(t1, t2) - 1.0f);
+                    jhi = fminf(jhi, fmaxf(t1, t2) + 1.0f);
+                } else if (mm[d] < lo[d] || mm[d] >= hi[d]) {
+                    empty = true;
+                }
+            }
+            if (empty) continue;
+            const float jend = fminf(floorf(jhi), m4.w + (float)(n - 1));
+            for (float jf = fmaxf(ceilf(jlo), m4.w); jf <= jend; jf += 1.0f) {
+                // the forward's own positions (k_fwd3d)
+                const float px = fmaf(jf, s4.x, m4.x), py = fmaf(jf, s4.y, m4.y),
+                            pz = fmaf(jf, s4.z, m4.z);
+                const int lx = floor_i32(px) + cx - x0, ly = floor_i32(py) + cy - y0,
+                          lz = floor_i32(pz) + cz - z0;
+                if ((unsigned)(lx + 1) > (unsigned)kAT || (unsigned)(ly + 1) > (unsigned)kAT ||
+                    (unsigned)(lz + 1) > (unsigned)kAT)
+                    continue;
+                const float fx = __builtin_amdgcn_fractf(px), fy = __builtin_amdgcn_fractf(py),
+                            fz = __builtin_amdgcn_fractf(pz);
+                const float wx[2] = {y * (1.0f - fx), y * fx};
+                const float wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int X = lx + (k & 1), Y = ly + ((k >> 1) & 1), Z = lz + (k >> 2);
+                    if ((unsigned)X < (unsigned)kAT && (unsigned)Y < (unsigned)kAT &&
+                        (unsigned)Z < (unsigned)kAT)
+                        atomicAdd(&tile[(Z * kAT + Y) * kAT + X],
+                                  wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < kAT * kAT * kAT; i += kMaxBlock) {
+        const int X = x0 + (i % kAT), Y = y0 + (i / kAT) % kAT, Z = z0 + i / (kAT * kAT);
+        if (X < nx && Y < ny && Z < nz) {
+            const size_t o = ((b * nz + Z) * ny + Y) * (size_t)nx + X;
+            vol[o] = from_f32<SV>(to_f32<SV>(vol[o]) + tile[i]);
         }
     }
 }
@@ -735,31 +835,28 @@ CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_
     const cone3d::Desc &g = plan->g;
     const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, U = (int)g.nu, V = (int)g.nv,
               A = (int)plan->A;
-    const size_t nvox = (size_t)nx * ny * nz * batch;
-    float *acc = nullptr;
-    if (vol_dtype != CMI_F32) CMI_HIP_TRY(hipMallocAsync((void **)&acc, nvox * sizeof(float), s));
+    const size_t rays = (size_t)A * U * V;
+    float4 *table = nullptr; // [2][rays]
+    CMI_HIP_TRY(hipMallocAsync((void **)&table, 2 * rays * sizeof(float4), s));
     const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
         using SV = decltype(sv);
         using SS = decltype(ss);
-        float *target = acc ? acc : (float *)volume;
-        const unsigned nb = (unsigned)((nvox + 255) / 256);
-        if (acc) hipLaunchKernelGGL(k_fill_zero, dim3(nb), dim3(256), 0, s, acc, nvox);
-        dim3 fb(16, 16), fg((unsigned)((U + 15) / 16), (unsigned)((V + 15) / 16),
-                            (unsigned)(A * batch));
-        hipLaunchKernelGGL((k_adj3d<SS>), fg, fb, 0, s, target,
-                           (const typename SS::raw *)sinogram, plan->d_fgeo, nx, ny, nz, U, V, A);
-        if (acc)
-            hipLaunchKernelGGL((k_add_into<SV>), dim3(nb), dim3(256), 0, s,
-                               (typename SV::raw *)volume, (const float *)acc, nvox);
+        dim3 tb(256), tg((unsigned)((U + 255) / 256), (unsigned)V, (unsigned)A);
+        hipLaunchKernelGGL(k_ray_table3d, tg, tb, 0, s, plan->d_fgeo, nx, ny, nz, U, V, table,
+                           table + rays);
+        const int ntz = (nz + kAT - 1) / kAT;
+        dim3 ab(kMaxBlock), ag((unsigned)((nx + kAT - 1) / kAT), (unsigned)((ny + kAT - 1) / kAT),
+                               (unsigned)(ntz * batch));
+        hipLaunchKernelGGL((k_adj3d<SV, SS>), ag, ab, 0, s, (typename SV::raw *)volume,
+                           (const typename SS::raw *)sinogram, (const float4 *)table,
+                           (const float4 *)(table + rays), plan->d_mmat, nx, ny, nz, U, V, A);
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });
-    if (acc) {
-        const hipError_t fe = hipFreeAsync(acc, s);
-        if (st != CMI_OK) return st;
-        CMI_HIP_TRY(fe);
-    }
-    return st;
+    const hipError_t fe = hipFreeAsync(table, s);
+    if (st != CMI_OK) return st;
+    CMI_HIP_TRY(fe);
+    return CMI_OK;
 }
 
 } // extern "C"
