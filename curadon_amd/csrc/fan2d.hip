@@ -193,23 +193,24 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// Workgroup-wide (min a, max b, min c); every thread of the workgroup calls it.
-__device__ __forceinline__ int3 block_reduce(int a, int b, int c, int3 *red, int nwaves) {
-    a = wave_min(a);
-    b = wave_max(b);
-    c = wave_min(c);
+// Workgroup-wide minima of N values (callers negate to take maxima); every thread of the
+// workgroup calls it. `red` holds N ints per wavefront.
+template <int N>
+__device__ __forceinline__ void block_min(int (&v)[N], int *red, int nwaves) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = wave_min(v[i]);
     const int w = threadIdx.x >> 6;
     __syncthreads(); // previous readers of red are done
-    if ((threadIdx.x & 63) == 0) red[w] = make_int3(a, b, c);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < N; ++i) red[w * N + i] = v[i];
     __syncthreads();
-    int3 r = red[0];
-    for (int i = 1; i < nwaves; ++i) {
-        r.x = min(r.x, red[i].x);
-        r.y = max(r.y, red[i].y);
-        r.z = min(r.z, red[i].z);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        int r = red[i];
+        for (int k = 1; k < nwaves; ++k) r = min(r, red[k * N + i]);
+        v[i] = __builtin_amdgcn_readfirstlane(r);
     }
-    return make_int3(__builtin_amdgcn_readfirstlane(r.x), __builtin_amdgcn_readfirstlane(r.y),
-                     __builtin_amdgcn_readfirstlane(r.z));
 }
 
 __device__ __forceinline__ int floor_div(int a, int b) {
@@ -229,12 +230,29 @@ __device__ __forceinline__ float band_sample(const float *p, float fa, float fb)
 // All threads of the workgroup call it together; returns the lane's sum, 0 if not in the pass.
 //   rstride      elements per row of the copy (rows run along the dominant axis)
 //   cofs         element offset of the copy's centre (centred coordinate 0, 0)
+// Dominant row range and column of a ray at row t: sample jf = s (t - floor(md)), column
+// coordinate fma(jf, vc, mc). Used for the workgroup's extreme rays.
+struct RayRow {
+    float mc, vc;
+    int r0, s;
+};
+
+template <bool YDOM>
+__device__ __forceinline__ RayRow ray_row(float4 e) {
+    const float md = YDOM ? e.y : e.x;
+    return {YDOM ? e.x : e.y, YDOM ? e.z : e.w, (int)floorf(md), (YDOM ? e.w : e.z) > 0.0f ? 1 : -1};
+}
+
+__device__ __forceinline__ int ray_col(const RayRow &r, int t) {
+    return floor_i32(fmaf((float)(r.s * (t - r.r0)), r.vc, r.mc));
+}
+
 template <class SI, bool YDOM>
 __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool mine, int rstride,
-                                            int cofs, float *band, int3 *red, int nthreads) {
+                                            int cofs, float *band, int *red, float4 *ext,
+                                            int *flag, int nthreads) {
     const int n = __float_as_int(q.y);
     const bool act = mine && n > 0;
-    if (!__syncthreads_or(act)) return 0.0f;
     const int nwaves = nthreads >> 6;
     const float md = YDOM ? e.y : e.x;   // dominant coordinate of the midpoint
     const float sd = YDOM ? e.w : e.z;   // dominant step, +-1
@@ -247,28 +265,41 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
     const int jlo = (int)q.z, jhi = jlo + n - 1;
     const int ta = s > 0 ? r0 + jlo : r0 - jhi; // first row of this lane
     const int tz = ta + n - 1;                  // last row
-    const int3 tr = block_reduce(act ? ta : INT_MAX, act ? tz : INT_MIN, 0, red, nwaves);
+    const int k = (int)threadIdx.x;
+    if (k == 0) flag[0] = flag[1] = 0; // ordered before every use by block_min's barriers
+    // pass-wide: rows [tr0, tr1], the latest first row and earliest last row of any lane, the
+    // first and last active lane, and whether every lane of the workgroup is in the pass
+    int v[7] = {act ? ta : INT_MAX, act ? -tz : INT_MAX, act ? -ta : INT_MAX, act ? tz : INT_MAX,
+                act ? k : INT_MAX,  act ? -k : INT_MAX,  act ? 1 : 0};
+    block_min<7>(v, red, nwaves);
+    if (v[0] == INT_MAX) return 0.0f; // no lane in this pass
+    const int tr0 = v[0], tr1 = -v[1], ta_max = -v[2], tz_min = v[3];
+    const bool all_in = v[6] == 1; // the minimum is 1 only if every lane is active
+    // Rays of one angle come from one source and do not cross inside the image, so on any row
+    // the columns of the workgroup's samples lie between those of its first and last active
+    // rays (extended beyond their ends where needed). Every band checks this and falls back to
+    // global memory if a lane's samples leave the staged columns.
+    if (k == v[4]) ext[0] = e;
+    if (k == -v[5]) ext[1] = e;
+    __syncthreads();
+    const RayRow e0 = ray_row<YDOM>(ext[0]), e1 = ray_row<YDOM>(ext[1]);
     const unsigned span = (unsigned)(n - 1);
     const int soff1 = rstride * (int)sizeof(typename SI::raw);
     float acc = 0.0f;
-    for (int t0 = floor_div(tr.x, kBand) * kBand; t0 <= tr.y; t0 += kBand) {
-        const int b_lo = max(tr.x, t0) & ~1, b_hi = min(tr.y, t0 + kBand - 1);
-        // columns the lane's samples in this band touch (c is monotonic in jf, so its
-        // extremes are at the lane's first and last row of the band), and whether the lane
-        // has a sample on every row [b_lo, b_hi | 1]
-        const int f = max(ta, b_lo), l = min(tz, b_hi);
-        int cmin = INT_MAX, cmax = INT_MIN;
-        if (act && f <= l) {
-            const int c1 = floor_i32(fmaf((float)(s * (f - r0)), vc, mc));
-            const int c2 = floor_i32(fmaf((float)(s * (l - r0)), vc, mc));
-            cmin = min(c1, c2);
-            cmax = max(c1, c2);
-        }
-        const int full_lane = act && ta <= b_lo && tz >= (b_hi | 1) ? 1 : 0;
-        const int3 cr = block_reduce(cmin, cmax, full_lane, red, nwaves);
-        const int c_lo = cr.x, W = cr.y + 2 - c_lo; // columns [cr.x, cr.y + 1]
+    int bi = 0;
+    for (int t0 = floor_div(tr0, kBand) * kBand; t0 <= tr1; t0 += kBand, ++bi) {
+        const int b_lo = max(tr0, t0) & ~1, b_hi = min(tr1, t0 + kBand - 1);
+        const int ea = ray_col(e0, b_lo), eb = ray_col(e0, b_hi);
+        const int ec = ray_col(e1, b_lo), ed = ray_col(e1, b_hi);
+        // lower taps within [c_lo, c_hi]: one column of margin for rounding
+        const int c_lo = min(min(ea, eb), min(ec, ed)) - 1;
+        const int c_hi = max(max(ea, eb), max(ec, ed)) + 1;
+        const int W = c_hi + 2 - c_lo; // staged columns [c_lo, c_hi + 1]
+        // whether every lane has a sample on every row [b_lo, b_hi | 1]
+        const bool full = all_in && ta_max <= b_lo && tz_min >= (b_hi | 1);
         float s0 = 0.0f, s1 = 0.0f;
         float jf = (float)(s * (b_lo - r0));
+        bool use_lds = false;
         if (W <= kPitch) {
             // stage rows [t0, t0 + kBandRows) x columns [c_lo, c_lo + 4 Q) as FP32
             const int Q = (W + 3) >> 2; // >= 1
@@ -293,35 +324,50 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
                     *reinterpret_cast<float4 *>(band + row[u] * kPitch + c4[u]) =
                         make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
             }
+            // every lane checks that its samples in the band stay in the staged columns (the
+            // column is monotonic in the row, so its first and last row decide)
+            const int f = max(ta, b_lo), l = min(tz, b_hi);
+            bool out = false;
+            if (act && f <= l) {
+                const int c1 = floor_i32(fmaf((float)(s * (f - r0)), vc, mc));
+                const int c2 = floor_i32(fmaf((float)(s * (l - r0)), vc, mc));
+                out = min(c1, c2) < c_lo || max(c1, c2) > c_hi;
+            }
+            if (__builtin_amdgcn_ballot_w64(out) != 0 && (threadIdx.x & 63) == 0)
+                flag[bi & 1] = 1;
             __syncthreads();
-            // lane pointer of row t = band + (t - t0) * kPitch - c_lo + floor(c); the even
-            // and odd rows of a pair advance separately so that each reaches its row below by
-            // an immediate offset
-            const float *rowa = band + (b_lo - t0) * kPitch - c_lo;
-            const float *rowb = rowa + kPitch;
-            if (cr.z) { // every lane has a sample on every row of the band
-                for (int t = b_lo; t <= b_hi;
-                     t += 2, jf += 2.0f * sf, rowa += 2 * kPitch, rowb += 2 * kPitch) {
-                    const float ca = fmaf(jf, vc, mc), cb = fmaf(jf + sf, vc, mc);
-                    s0 += band_sample(rowa + floor_i32(ca), __builtin_amdgcn_fractf(ca), fb);
-                    s1 += band_sample(rowb + floor_i32(cb), __builtin_amdgcn_fractf(cb), fb);
-                }
-            } else {
-                const int cw = W - 2; // highest lower-tap column offset
-                for (int t = b_lo; t <= b_hi;
-                     t += 2, jf += 2.0f * sf, rowa += 2 * kPitch, rowb += 2 * kPitch) {
-                    const float ca = fmaf(jf, vc, mc), cb = fmaf(jf + sf, vc, mc);
-                    // lanes without a sample on the row read a clamped cell, masked below
-                    const int ia = min(max(floor_i32(ca) - c_lo, 0), cw) + c_lo;
-                    const int ib = min(max(floor_i32(cb) - c_lo, 0), cw) + c_lo;
-                    const float va = band_sample(rowa + ia, __builtin_amdgcn_fractf(ca), fb);
-                    const float vb = band_sample(rowb + ib, __builtin_amdgcn_fractf(cb), fb);
-                    s0 += act && (unsigned)(t - ta) <= span ? va : 0.0f;
-                    s1 += act && (unsigned)(t + 1 - ta) <= span ? vb : 0.0f;
+            use_lds = flag[bi & 1] == 0;
+            // the other slot was last read in the previous band, and is next written in the
+            // following one, after this band's closing barrier
+            if (threadIdx.x == 0) flag[(bi + 1) & 1] = 0;
+            if (use_lds) {
+                const float *rowa = band + (b_lo - t0) * kPitch - c_lo;
+                const float *rowb = rowa + kPitch;
+                if (full) { // every lane has a sample on every row of the band
+                    for (int t = b_lo; t <= b_hi;
+                         t += 2, jf += 2.0f * sf, rowa += 2 * kPitch, rowb += 2 * kPitch) {
+                        const float ca = fmaf(jf, vc, mc), cb = fmaf(jf + sf, vc, mc);
+                        s0 += band_sample(rowa + floor_i32(ca), __builtin_amdgcn_fractf(ca), fb);
+                        s1 += band_sample(rowb + floor_i32(cb), __builtin_amdgcn_fractf(cb), fb);
+                    }
+                } else {
+                    const int cw = W - 2; // highest lower-tap column offset
+                    for (int t = b_lo; t <= b_hi;
+                         t += 2, jf += 2.0f * sf, rowa += 2 * kPitch, rowb += 2 * kPitch) {
+                        const float ca = fmaf(jf, vc, mc), cb = fmaf(jf + sf, vc, mc);
+                        // lanes without a sample on the row read a clamped cell, masked below
+                        const int ia = min(max(floor_i32(ca) - c_lo, 0), cw) + c_lo;
+                        const int ib = min(max(floor_i32(cb) - c_lo, 0), cw) + c_lo;
+                        const float va = band_sample(rowa + ia, __builtin_amdgcn_fractf(ca), fb);
+                        const float vb = band_sample(rowb + ib, __builtin_amdgcn_fractf(cb), fb);
+                        s0 += act && (unsigned)(t - ta) <= span ? va : 0.0f;
+                        s1 += act && (unsigned)(t + 1 - ta) <= span ? vb : 0.0f;
+                    }
                 }
             }
-            __syncthreads(); // the band is read before the next one is staged
-        } else {
+            __syncthreads(); // the band and its flag are read before the next band is staged
+        }
+        if (!use_lds) {
             // lanes without a sample on the row sample their nearest own sample, masked below
             const float jmin = (float)jlo, jmax = (float)(act ? jhi : jlo);
             for (int t = b_lo; t <= b_hi; t += 2, jf += 2.0f * sf) {
@@ -350,7 +396,9 @@ __global__ __launch_bounds__(kFwdRays) void k_fwd2d(const typename SI::raw *__re
                                               const float4 *__restrict__ ns,
                                               typename SS::raw *__restrict__ sino, int D, int A) {
     __shared__ __align__(16) float band[kBandRows * kPitch];
-    __shared__ int3 red[kMaxWaves];
+    __shared__ int red[7 * kMaxWaves];
+    __shared__ float4 ext[2];
+    __shared__ int flag[2];
     const int nthreads = blockDim.x;
     const int k = blockIdx.x * nthreads + threadIdx.x;
     const int a = blockIdx.y;
@@ -365,9 +413,10 @@ __global__ __launch_bounds__(kFwdRays) void k_fwd2d(const typename SI::raw *__re
     const int c0x = Wp / 2, c0y = Hp / 2;
     const bool ydom = fabsf(e.w) == 1.0f; // y dominant: row-major copy; else the transposed one
     const float acc =
-        march_band<SI, true>(R, e, q, valid && ydom, Wp, c0y * Wp + c0x, band, red, nthreads) +
-        march_band<SI, false>(R, e, q, valid && !ydom, Hp, one + c0x * Hp + c0y, band, red,
-                              nthreads);
+        march_band<SI, true>(R, e, q, valid && ydom, Wp, c0y * Wp + c0x, band, red, ext, flag,
+                             nthreads) +
+        march_band<SI, false>(R, e, q, valid && !ydom, Hp, one + c0x * Hp + c0y, band, red, ext,
+                              flag, nthreads);
     if (valid) sino[b * (size_t)A * D + r] = from_f32<SS>(acc * q.x);
 }
 
