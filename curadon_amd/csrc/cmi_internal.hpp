@@ -164,29 +164,35 @@ struct Quad {
     float v[4];
 };
 
+// buf_quad_at takes the byte offset of the first element.
 template <class S>
-__device__ __forceinline__ Quad buf_quad(rsrc_t r, int elem_offset);
+__device__ __forceinline__ Quad buf_quad_at(rsrc_t r, int byte_offset);
 
 template <>
-__device__ __forceinline__ Quad buf_quad<F32>(rsrc_t r, int elem_offset) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, elem_offset * 4, 0, 0);
+__device__ __forceinline__ Quad buf_quad_at<F32>(rsrc_t r, int byte_offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_offset, 0, 0);
     const unsigned a = v[0], b = v[1], c = v[2], d = v[3];
     return {{__uint_as_float(a), __uint_as_float(b), __uint_as_float(c), __uint_as_float(d)}};
 }
 template <>
-__device__ __forceinline__ Quad buf_quad<F16>(rsrc_t r, int elem_offset) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, elem_offset * 2, 0, 0);
+__device__ __forceinline__ Quad buf_quad_at<F16>(rsrc_t r, int byte_offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_offset, 0, 0);
     const unsigned a = v[0], b = v[1];
     auto h = [](unsigned short x) { return static_cast<float>(__builtin_bit_cast(_Float16, x)); };
     return {{h((unsigned short)(a & 0xffffu)), h((unsigned short)(a >> 16)),
              h((unsigned short)(b & 0xffffu)), h((unsigned short)(b >> 16))}};
 }
 template <>
-__device__ __forceinline__ Quad buf_quad<BF16>(rsrc_t r, int elem_offset) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, elem_offset * 2, 0, 0);
+__device__ __forceinline__ Quad buf_quad_at<BF16>(rsrc_t r, int byte_offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_offset, 0, 0);
     const unsigned a = v[0], b = v[1];
     return {{__uint_as_float(a << 16), __uint_as_float(a & 0xffff0000u), __uint_as_float(b << 16),
              __uint_as_float(b & 0xffff0000u)}};
+}
+
+template <class S>
+__device__ __forceinline__ Quad buf_quad(rsrc_t r, int elem_offset) {
+    return buf_quad_at<S>(r, elem_offset * (int)sizeof(typename S::raw));
 }
 
 // Byte size of n elements, for the range check of a resource.

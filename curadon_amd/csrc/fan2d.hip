@@ -140,20 +140,26 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_sino2d(const typename S::ra
 
 // ---------------------------------------------------------------- forward ------------------
 
-// Bilinear sample from global memory: dominant row `row` (lower tap) with weight fb, other
-// coordinate c, in a copy whose rows run along the dominant axis (`rstride` elements per row,
-// `soff1` bytes to the next row, `cofs` the element offset of the copy's centre).
-template <class SI>
-__device__ __forceinline__ float sample_row(rsrc_t R, int row, float fb, float c, int rstride,
-                                           int soff1, int cofs) {
-    const int o = __mul24(row, rstride) + floor_i32(c) + cofs;
-    const float fa = __builtin_amdgcn_fractf(c);
-    const Pair p0 = buf_pair<SI>(R, o, 0);     // row, taps c and c + 1
-    const Pair p1 = buf_pair<SI>(R, o, soff1); // row + 1
-    const float u0 = fmaf(fa, p0.hi - p0.lo, p0.lo);
-    const float u1 = fmaf(fa, p1.hi - p1.lo, p1.lo);
-    return fmaf(fb, u1 - u0, u0);
-}
+// A lane's sums over its samples of one band. A sample with lower tap (row t, column i) and
+// fraction fa along the row adds p0 += v(t, i), x0 += fa (v(t, i + 1) - v(t, i)), and p1, x1
+// likewise on row t + 1. With u0 = p0 + x0 and u1 = p1 + x1 (the sums of the row-wise linear
+// interpolants on the sample rows and on the rows below), the band's contribution is
+// u0 + fb (u1 - u0), the sum of the samples' bilinear interpolants, because every sample of a
+// ray has the same dominant weight fb.
+struct BandSums {
+    float p0 = 0.0f, x0 = 0.0f, p1 = 0.0f, x1 = 0.0f;
+    // a0, a1: v(t, i), v(t, i + 1); b0, b1: v(t + 1, i), v(t + 1, i + 1)
+    __device__ __forceinline__ void add(float a0, float a1, float b0, float b1, float fa) {
+        p0 += a0;
+        x0 = fmaf(fa, a1 - a0, x0);
+        p1 += b0;
+        x1 = fmaf(fa, b1 - b0, x1);
+    }
+    __device__ __forceinline__ float total(float fb) const {
+        const float u0 = p0 + x0, u1 = p1 + x1;
+        return fmaf(fb, u1 - u0, u0);
+    }
+};
 
 // Row-synchronous march through LDS bands.
 //
@@ -164,18 +170,20 @@ __device__ __forceinline__ float sample_row(rsrc_t R, int row, float fb, float c
 // same two rows. The rows come in bands of kBand, aligned to multiples of kBand: the workgroup
 // stages rows [t0, t0 + kBand] of the copy whose rows run along the dominant axis, over the
 // columns its samples in the band touch, into LDS (rows of kPitch floats, filled with
-// 4-element row loads) and interpolates from there. A band wider than kPitch is sampled from
-// global memory instead, with the same arithmetic. Bands in which every lane of the workgroup
-// has a sample on every row skip the per-sample range test.
+// 4-element row loads) and interpolates from there. The last two cells of every row are kept
+// at zero and are what lanes without a sample on a row read. A band wider than kPitch - 4
+// columns is sampled from global memory instead, with the same arithmetic. Bands in which
+// every lane of the workgroup has a sample on every row skip the per-sample range test.
 //
 // Staged cells outside the copy (band corners beyond its edge) hold whatever the range-checked
 // loads return; no sample reads them, because every tap of a sample lies inside the copy's zero
 // border.
 //
-// Summation: samples on even and odd rows go to two partial sums, which are added to the total
-// at the end of every band. Bands and parities are absolute row numbers, so a ray's sum does
-// not depend on which rays share its workgroup (and so not on the launch shape).
-constexpr int kBand = 32;              // rows per band and per partial sum
+// Summation: a lane adds its samples of a band row by row into its BandSums, and the band's
+// total into the ray's sum. Bands are absolute row numbers and both paths add the same values
+// in the same order, so a ray's sum does not depend on which rays share its workgroup (and so
+// not on the launch shape).
+constexpr int kBand = 32;              // rows per band
 constexpr int kBandRows = kBand + 1;   // staged rows: the band and the row below it
 // LDS row pitch in floats: a multiple of 4 (16-byte row stores) and below 255, so that one
 // address reaches a tap pair and the pair below it with two ds_read2_b32 immediate offsets.
@@ -215,15 +223,6 @@ __device__ __forceinline__ void block_min(int (&v)[N], int *red, int nwaves) {
 
 __device__ __forceinline__ int floor_div(int a, int b) {
     return a >= 0 ? a / b : -((-a + b - 1) / b);
-}
-
-// Bilinear sample from the staged band: `p` points at the lower tap (row, column) of a row of
-// kPitch floats; the pair runs along the non-dominant axis with fraction fa, the rows along
-// the dominant one with fraction fb.
-__device__ __forceinline__ float band_sample(const float *p, float fa, float fb) {
-    const float u0 = fmaf(fa, p[1] - p[0], p[0]);
-    const float u1 = fmaf(fa, p[kPitch + 1] - p[kPitch], p[kPitch]);
-    return fmaf(fb, u1 - u0, u0);
 }
 
 // One pass (YDOM: rays whose dominant axis is y, row-major copy; else x, transposed copy).
@@ -284,45 +283,64 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
     __syncthreads();
     const RayRow e0 = ray_row<YDOM>(ext[0]), e1 = ray_row<YDOM>(ext[1]);
     const unsigned span = (unsigned)(n - 1);
+    // a lane has a sample on row t iff (unsigned)t - ta_m <= span_m (never for inactive lanes:
+    // rows are far from the offset 2^31)
+    const unsigned ta_m = act ? (unsigned)ta : 0x80000000u, span_m = act ? span : 0u;
     const int soff1 = rstride * (int)sizeof(typename SI::raw);
     float acc = 0.0f;
     int bi = 0;
     for (int t0 = floor_div(tr0, kBand) * kBand; t0 <= tr1; t0 += kBand, ++bi) {
-        const int b_lo = max(tr0, t0) & ~1, b_hi = min(tr1, t0 + kBand - 1);
+        const int b_lo = max(tr0, t0), b_hi = min(tr1, t0 + kBand - 1);
         const int ea = ray_col(e0, b_lo), eb = ray_col(e0, b_hi);
         const int ec = ray_col(e1, b_lo), ed = ray_col(e1, b_hi);
-        // lower taps within [c_lo, c_hi]: one column of margin for rounding
-        const int c_lo = min(min(ea, eb), min(ec, ed)) - 1;
-        const int c_hi = max(max(ea, eb), max(ec, ed)) + 1;
-        const int W = c_hi + 2 - c_lo; // staged columns [c_lo, c_hi + 1]
-        // whether every lane has a sample on every row [b_lo, b_hi | 1]
-        const bool full = all_in && ta_max <= b_lo && tz_min >= (b_hi | 1);
-        float s0 = 0.0f, s1 = 0.0f;
+        // lower taps within [c_lo, c_hi]: one column of margin for rounding (the extreme rays
+        // are the same for every lane: the bounds live in scalar registers)
+        const int c_lo = __builtin_amdgcn_readfirstlane(min(min(ea, eb), min(ec, ed)) - 1);
+        const int c_hi = __builtin_amdgcn_readfirstlane(max(max(ea, eb), max(ec, ed)) + 1);
+        const int W = c_hi + 2 - c_lo; // columns [c_lo, c_hi + 1] hold the taps
+        // whether every lane has a sample on every row [b_lo, b_hi]
+        const bool full = all_in && ta_max <= b_lo && tz_min >= b_hi;
+        BandSums sums;
         float jf = (float)(s * (b_lo - r0));
         bool use_lds = false;
-        if (W <= kPitch) {
-            // stage rows [t0, t0 + kBandRows) x columns [c_lo, c_lo + 4 Q) as FP32
-            const int Q = (W + 3) >> 2; // >= 1
-            // i / Q as a multiply-high, exact for i < 2^16 (Q = 1: the identity)
-            const unsigned magic = Q == 1 ? 0u : 0xffffffffu / (unsigned)Q + 1u;
-            const int g0 = t0 * rstride + c_lo + cofs;
-            const int items = kBandRows * Q;
-            // four items per thread and trip, loads first: an index past the end repeats the
-            // last item (same value to the same cell)
-            for (int i0 = threadIdx.x; i0 < items; i0 += 4 * nthreads) {
-                int row[4], c4[4];
-                Quad v[4];
+        if (W <= kPitch - 4) {
+            // stage rows [t0, t0 + kBandRows) x columns [c_lo, c_lo + 4 Q) as FP32, with
+            // 4 Q <= kPitch - 4 (the zero cells stay untouched). Thread g Q + q (g < G groups
+            // of Q threads) takes quad q of rows g, g + G, ...; a row index past the band's last
+            // row is clamped to it (the same values to the same cells). Quotients come from a
+            // float reciprocal: exact, as every numerator is below 2^8 and every true quotient
+            // is at least 1/(2 Q) > 2^-8 away from the next integer.
+            const int Q = (W + 3) >> 2; // 1 .. 47, below nthreads (a multiple of 64)
+            const float rq = __builtin_amdgcn_rcpf((float)Q);
+            const int G = (int)(((float)nthreads + 0.5f) * rq);
+            const int g = (int)(((float)threadIdx.x + 0.5f) * rq);
+            if (g < G) {
+                const float rg = __builtin_amdgcn_rcpf((float)G);
+                const int K = (int)(((float)(kBandRows + G - 1) + 0.5f) * rg); // ceil(33 / G)
+                const int q4 = ((int)threadIdx.x - g * Q) << 2;
+                // byte offsets: global (this thread's quad in row g, and in the last row) and LDS
+                constexpr int esz = (int)sizeof(typename SI::raw);
+                const int gb = ((t0 + g) * rstride + c_lo + cofs + q4) * esz;
+                const int gb_max = ((t0 + kBandRows - 1) * rstride + c_lo + cofs + q4) * esz;
+                const int lb = (g * kPitch + q4) * 4;
+                const int lb_max = ((kBandRows - 1) * kPitch + q4) * 4;
+                const int gstep = G * rstride * esz, lstep = G * kPitch * 4;
+                // groups of eight rows in flight (K <= 17: three groups at most)
+                for (int k0 = 0; k0 < K; k0 += 8) {
+                    // uniform; opaque, so that the compiler keeps one base per group instead of
+                    // one induction variable per row
+                    int sg = k0 * gstep, sl = k0 * lstep;
+                    asm("" : "+s"(sg), "+s"(sl));
+                    Quad v[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int i = min(i0 + u * nthreads, items - 1);
-                    row[u] = Q == 1 ? i : (int)__umulhi((unsigned)i, magic);
-                    c4[u] = (i - row[u] * Q) << 2;
-                    v[u] = buf_quad<SI>(R, g0 + row[u] * rstride + c4[u]);
+                    for (int u = 0; u < 8; ++u)
+                        v[u] = buf_quad_at<SI>(R, min(gb + (sg + u * gstep), gb_max));
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        *reinterpret_cast<float4 *>(reinterpret_cast<char *>(band) +
+                                                    min(lb + (sl + u * lstep), lb_max)) =
+                            make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
                 }
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    *reinterpret_cast<float4 *>(band + row[u] * kPitch + c4[u]) =
-                        make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
             }
             // every lane checks that its samples in the band stay in the staged columns (the
             // column is monotonic in the row, so its first and last row decide)
@@ -341,64 +359,101 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
             // following one, after this band's closing barrier
             if (threadIdx.x == 0) flag[(bi + 1) & 1] = 0;
             if (use_lds) {
-                const float *rowa = band + (b_lo - t0) * kPitch - c_lo;
-                const float *rowb = rowa + kPitch;
+                // float offset of the band's first row, column 0, held opaque in a scalar
+                // register so that the compiler does not split a constant off every address
+                int rofs = (b_lo - t0) * kPitch - c_lo;
+                asm("" : "+s"(rofs));
+                const float *rowa = band + rofs;
+                int t = b_lo;
                 if (full) { // every lane has a sample on every row of the band
-                    for (int t = b_lo; t <= b_hi;
-                         t += 2, jf += 2.0f * sf, rowa += 2 * kPitch, rowb += 2 * kPitch) {
-                        const float ca = fmaf(jf, vc, mc), cb = fmaf(jf + sf, vc, mc);
-                        s0 += band_sample(rowa + floor_i32(ca), __builtin_amdgcn_fractf(ca), fb);
-                        s1 += band_sample(rowb + floor_i32(cb), __builtin_amdgcn_fractf(cb), fb);
+                    for (; t + 3 <= b_hi; t += 4, jf += 4.0f * sf, rowa += 4 * kPitch) {
+                        float c[4], x[4][4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            c[u] = fmaf(u == 0 ? jf : jf + (float)u * sf, vc, mc);
+                            const float *p = rowa + u * kPitch + floor_i32(c[u]);
+                            x[u][0] = p[0];
+                            x[u][1] = p[1];
+                            x[u][2] = p[kPitch];
+                            x[u][3] = p[kPitch + 1];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            sums.add(x[u][0], x[u][1], x[u][2], x[u][3],
+                                     __builtin_amdgcn_fractf(c[u]));
+                    }
+                    for (; t <= b_hi; ++t, jf += sf, rowa += kPitch) {
+                        const float c = fmaf(jf, vc, mc);
+                        const float *p = rowa + floor_i32(c);
+                        sums.add(p[0], p[1], p[kPitch], p[kPitch + 1], __builtin_amdgcn_fractf(c));
                     }
                 } else {
-                    const int cw = W - 2; // highest lower-tap column offset
-                    for (int t = b_lo; t <= b_hi;
-                         t += 2, jf += 2.0f * sf, rowa += 2 * kPitch, rowb += 2 * kPitch) {
-                        const float ca = fmaf(jf, vc, mc), cb = fmaf(jf + sf, vc, mc);
-                        // lanes without a sample on the row read a clamped cell, masked below
-                        const int ia = min(max(floor_i32(ca) - c_lo, 0), cw) + c_lo;
-                        const int ib = min(max(floor_i32(cb) - c_lo, 0), cw) + c_lo;
-                        const float va = band_sample(rowa + ia, __builtin_amdgcn_fractf(ca), fb);
-                        const float vb = band_sample(rowb + ib, __builtin_amdgcn_fractf(cb), fb);
-                        s0 += act && (unsigned)(t - ta) <= span ? va : 0.0f;
-                        s1 += act && (unsigned)(t + 1 - ta) <= span ? vb : 0.0f;
+                    const int zc = c_lo + kPitch - 2; // the zero cells
+                    for (; t + 1 <= b_hi; t += 2, jf += 2.0f * sf, rowa += 2 * kPitch) {
+                        float c[2], x[2][4];
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            c[u] = fmaf(u == 0 ? jf : jf + (float)u * sf, vc, mc);
+                            const int i = floor_i32(c[u]);
+                            const bool in = (unsigned)(t + u) - ta_m <= span_m;
+                            const float *p = rowa + u * kPitch + (in ? i : zc);
+                            x[u][0] = p[0];
+                            x[u][1] = p[1];
+                            x[u][2] = p[kPitch];
+                            x[u][3] = p[kPitch + 1];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 2; ++u)
+                            sums.add(x[u][0], x[u][1], x[u][2], x[u][3],
+                                     __builtin_amdgcn_fractf(c[u]));
+                    }
+                    if (t <= b_hi) {
+                        const float c = fmaf(jf, vc, mc);
+                        const int i = floor_i32(c);
+                        const bool in = (unsigned)t - ta_m <= span_m;
+                        const float *p = rowa + (in ? i : zc);
+                        sums.add(p[0], p[1], p[kPitch], p[kPitch + 1], __builtin_amdgcn_fractf(c));
                     }
                 }
             }
             __syncthreads(); // the band and its flag are read before the next band is staged
         }
         if (!use_lds) {
-            // lanes without a sample on the row sample their nearest own sample, masked below
+            // lanes without a sample on the row sample their nearest own sample and add zeros
             const float jmin = (float)jlo, jmax = (float)(act ? jhi : jlo);
-            for (int t = b_lo; t <= b_hi; t += 2, jf += 2.0f * sf) {
-                const float ja = __builtin_amdgcn_fmed3f(jf, jmin, jmax);
-                const float jb = __builtin_amdgcn_fmed3f(jf + sf, jmin, jmax);
-                const int rowa = r0 + s * (int)ja, rowb = r0 + s * (int)jb;
-                const float va = sample_row<SI>(R, rowa, fb, fmaf(ja, vc, mc), rstride, soff1,
-                                                cofs);
-                const float vb = sample_row<SI>(R, rowb, fb, fmaf(jb, vc, mc), rstride, soff1,
-                                                cofs);
-                s0 += act && (unsigned)(t - ta) <= span ? va : 0.0f;
-                s1 += act && (unsigned)(t + 1 - ta) <= span ? vb : 0.0f;
+            for (int t = b_lo; t <= b_hi; ++t, jf += sf) {
+                const float j = __builtin_amdgcn_fmed3f(jf, jmin, jmax);
+                const float c = fmaf(j, vc, mc);
+                const int o = __mul24(r0 + s * (int)j, rstride) + floor_i32(c) + cofs;
+                Pair a = buf_pair<SI>(R, o, 0);     // row, columns i and i + 1
+                Pair b = buf_pair<SI>(R, o, soff1); // the row below
+                if ((unsigned)t - ta_m > span_m) a = b = Pair{0.0f, 0.0f};
+                sums.add(a.lo, a.hi, b.lo, b.hi, __builtin_amdgcn_fractf(c));
             }
         }
-        acc += s0 + s1;
+        acc += sums.total(fb);
     }
     return acc;
 }
 
 // One workgroup = nthreads <= kFwdRays adjacent detector cells of one angle, one thread per
-// ray; LDS holds one band of kBandRows x kPitch floats (26.9 KB).
+// ray; LDS holds one band of kBandRows x kPitch floats (24.8 KB).
 // sino[b][a][k] = scale * sum_{j<N} bilinear(img, e + j v)   (forward_2d.hpp:63-78)
 template <class SI, class SS>
 __global__ __launch_bounds__(kFwdRays) void k_fwd2d(const typename SI::raw *__restrict__ scratch,
                                               int Wp, int Hp, const float4 *__restrict__ ev,
                                               const float4 *__restrict__ ns,
                                               typename SS::raw *__restrict__ sino, int D, int A) {
-    __shared__ __align__(16) float band[kBandRows * kPitch];
-    __shared__ int red[7 * kMaxWaves];
-    __shared__ float4 ext[2];
-    __shared__ int flag[2];
+    // one LDS object with the band first, so that cell addresses need no base offset
+    __shared__ struct {
+        float band[kBandRows * kPitch];
+        float4 ext[2];
+        int red[7 * kMaxWaves];
+        int flag[2];
+    } sh;
+    float *band = sh.band;
+    float4 *ext = sh.ext;
+    int *red = sh.red, *flag = sh.flag;
     const int nthreads = blockDim.x;
     const int k = blockIdx.x * nthreads + threadIdx.x;
     const int a = blockIdx.y;
@@ -412,6 +467,9 @@ __global__ __launch_bounds__(kFwdRays) void k_fwd2d(const typename SI::raw *__re
     const rsrc_t R = make_rsrc(scratch + b * 2 * (size_t)one, bytes_of<SI>(2 * (uint64_t)one));
     const int c0x = Wp / 2, c0y = Hp / 2;
     const bool ydom = fabsf(e.w) == 1.0f; // y dominant: row-major copy; else the transposed one
+    // the zero cells; ordered before their first read by the barriers at the start of a pass
+    for (int i = threadIdx.x; i < 2 * kBandRows; i += nthreads)
+        band[(i >> 1) * kPitch + kPitch - 2 + (i & 1)] = 0.0f;
     const float acc =
         march_band<SI, true>(R, e, q, valid && ydom, Wp, c0y * Wp + c0x, band, red, ext, flag,
                              nthreads) +
