@@ -183,7 +183,10 @@ struct BandSums {
 // total into the ray's sum. Bands are absolute row numbers and both paths add the same values
 // in the same order, so a ray's sum does not depend on which rays share its workgroup (and so
 // not on the launch shape).
-constexpr int kBand = 32;              // rows per band
+// rows per band: a band and the row below it make 32 staged rows, so that the common band
+// widths (25 to 32 quads, four groups of threads) stage eight rows per thread, one group of
+// loads in flight
+constexpr int kBand = 31;
 constexpr int kBandRows = kBand + 1;   // staged rows: the band and the row below it
 // LDS row pitch in floats: a multiple of 4 (16-byte row stores) and below 255, so that one
 // address reaches a tap pair and the pair below it with two ds_read2_b32 immediate offsets.
