@@ -6,7 +6,10 @@
 //                mref [A][12] float          reference-mode projection matrix (see bp_matrices)
 //                mmat [A][12] float          geometry-matched projection matrix
 //                mfdk [A][12] float          matched matrix scaled so that W = (depth / DSO)
-//   per call     forward:  [B][nz+4][ny+4][nx+4] volume with a 2-voxel zero border, x fastest
+//                fast [A] int                axis (0 x, 1 y, 2 z) the detector's v step follows most
+//                                            closely: the forward reads the copy in which it is fastest
+//   per call     forward:  [B][copies] volume with a 2-voxel zero border, one copy per fast axis
+//                          in use: x fastest [z][y][x], y fastest [z][x][y], z fastest [y][x][z]
 //                backward: [B][A][V+4][U+4]      panels with a 2-pixel zero border
 //                adjoint:  per-ray march table [2][A][V][U] float4 (kernel k_ray_table3d)
 //
@@ -35,6 +38,9 @@ struct cmi_plan3d {
     uint64_t total_samples = 0;
     uint64_t table_bytes = 0;
     FwdGeom *d_fgeo = nullptr;
+    int *d_fast = nullptr;      // [A] fast axis of each angle's volume copy
+    int copy_slot[3] = {-1, -1, -1}; // per fast axis: its copy's index, -1 if no angle uses it
+    int ncopies = 0;
     float *d_mref = nullptr;
     float *d_mmat = nullptr;
     float *d_mfdk = nullptr;
@@ -112,6 +118,39 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage3d(const typename S::raw *__
     pad[((b * Dp + zp) * Hp + yp) * (size_t)Wp + xp] = v;
 }
 
+// Copy with fast axis F = 1 (y) or 2 (z) and x as the middle axis: for each slow coordinate
+// (z for F = 1, y for F = 2), dst[slow][xp][fp] = pad(xp, yp, zp), transposed through LDS in
+// 32 x 32 tiles (reads along x, writes along the fast axis).
+template <class S, int F>
+__global__ __launch_bounds__(kMaxBlock) void k_stage3d_t(const typename S::raw *__restrict__ vol,
+                                                         typename S::raw *__restrict__ dst,
+                                                         int nx, int ny, int nz, int Wp, int Hp,
+                                                         int Dp) {
+    static_assert(F == 1 || F == 2, "transposed copies have y or z fastest");
+    using raw = typename S::raw;
+    __shared__ raw tile[32][33];
+    const int Ef = F == 1 ? Hp : Dp, Es = F == 1 ? Dp : Hp;
+    const int sp = blockIdx.z % Es;
+    const size_t b = blockIdx.z / Es;
+    const int xp0 = blockIdx.x * 32, fp0 = blockIdx.y * 32;
+    const raw *src = vol + b * (size_t)nx * ny * nz;
+    raw *out = dst + (size_t)sp * Wp * Ef;
+    for (int r = threadIdx.y; r < 32; r += 8) { // r: fast-axis offset, threadIdx.x: x offset
+        const int xp = xp0 + threadIdx.x, fp = fp0 + r;
+        const int yp = F == 1 ? fp : sp, zp = F == 1 ? sp : fp;
+        const int x = xp - kPad, y = yp - kPad, z = zp - kPad;
+        raw v = (raw)0.0f;
+        if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+            v = src[((size_t)z * ny + y) * nx + x];
+        tile[r][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) { // r: x offset, threadIdx.x: fast-axis offset
+        const int xp = xp0 + r, fp = fp0 + threadIdx.x;
+        if (xp < Wp && fp < Ef) out[(size_t)xp * Ef + fp] = tile[threadIdx.x][r];
+    }
+}
+
 // ppad[b][a][vp][up] = sino[b][a][vp-2][up-2], 0 in the border.
 template <class S>
 __global__ __launch_bounds__(kMaxBlock) void k_stage_panel(const typename S::raw *__restrict__ sino,
@@ -131,48 +170,66 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_panel(const typename S::raw
 
 // ---------------------------------------------------------------- forward -----------------
 
-// Trilinear sample at centred padded coordinates from the x-fastest padded volume.
-// `cofs` is the element offset of the padded volume's centre; sW, sH, sHW are the byte
-// strides of one row, one slice, one slice plus one row.
+// Trilinear sample at centred padded coordinates (f, m, l) along the copy's fast, middle and
+// slow axes. `cofs` is the element offset of the copy's centre; sM, sL, sML are the byte strides
+// of one step along the middle axis, the slow axis, and both.
 template <class SI>
-__device__ __forceinline__ float sample3d(rsrc_t R, float x, float y, float z, int Wp, int Hp,
-                                         int cofs, int sW, int sHW, int sHWW) {
-    const int ix = (int)floorf(x), iy = (int)floorf(y), iz = (int)floorf(z);
-    const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y),
-                fz = __builtin_amdgcn_fractf(z);
-    const int o = __mul24(__mul24(iz, Hp) + iy, Wp) + ix + cofs;
-    const Pair p00 = buf_pair<SI>(R, o, 0);    // (y,   z)
-    const Pair p10 = buf_pair<SI>(R, o, sW);   // (y+1, z)
-    const Pair p01 = buf_pair<SI>(R, o, sHW);  // (y,   z+1)
-    const Pair p11 = buf_pair<SI>(R, o, sHWW); // (y+1, z+1)
-    const float a00 = fmaf(fx, p00.hi - p00.lo, p00.lo);
-    const float a10 = fmaf(fx, p10.hi - p10.lo, p10.lo);
-    const float a01 = fmaf(fx, p01.hi - p01.lo, p01.lo);
-    const float a11 = fmaf(fx, p11.hi - p11.lo, p11.lo);
-    const float b0 = fmaf(fy, a10 - a00, a00);
-    const float b1 = fmaf(fy, a11 - a01, a01);
-    return fmaf(fz, b1 - b0, b0);
+__device__ __forceinline__ float sample3d(rsrc_t R, float f, float m, float l, int Ef, int Em,
+                                         int cofs, int sM, int sL, int sML) {
+    const int i_f = floor_i32(f), i_m = floor_i32(m), i_l = floor_i32(l);
+    const float ff = __builtin_amdgcn_fractf(f), fm = __builtin_amdgcn_fractf(m),
+                fl = __builtin_amdgcn_fractf(l);
+    const int o = __mul24(__mul24(i_l, Em) + i_m, Ef) + i_f + cofs;
+    const Pair p00 = buf_pair<SI>(R, o, 0);   // (m,     l)
+    const Pair p10 = buf_pair<SI>(R, o, sM);  // (m + 1, l)
+    const Pair p01 = buf_pair<SI>(R, o, sL);  // (m,     l + 1)
+    const Pair p11 = buf_pair<SI>(R, o, sML); // (m + 1, l + 1)
+    const float a00 = fmaf(ff, p00.hi - p00.lo, p00.lo);
+    const float a10 = fmaf(ff, p10.hi - p10.lo, p10.lo);
+    const float a01 = fmaf(ff, p01.hi - p01.lo, p01.lo);
+    const float a11 = fmaf(ff, p11.hi - p11.lo, p11.lo);
+    const float b0 = fmaf(fm, a10 - a00, a00);
+    const float b1 = fmaf(fm, a11 - a01, a01);
+    return fmaf(fl, b1 - b0, b0);
 }
 
-// One thread per ray (iu, iv) of projection a; a wavefront covers a 16 x 4 pixel tile.
+// One thread per ray (iu, iv) of projection a. A wavefront covers 4 x 16 detector pixels, the
+// 16 along v, and reads the volume copy whose fastest axis is the one v follows (fast[a]), so
+// the wavefront's samples of one step lie in a few runs of consecutive voxels whatever the
+// projection angle. A workgroup holds four wavefronts side by side along u (16 x 16 pixels).
 // sino[b][a][iv][iu] = sum_{i=0..nst} trilinear(vol, s + (tmin + i dt) d)  (forward_3d.hpp:87-93)
 template <class SI, class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__restrict__ pad,
-                                                     const FwdGeom *__restrict__ geo, int nx,
+                                                     const FwdGeom *__restrict__ geo,
+                                                     const int *__restrict__ fast, int slot0,
+                                                     int slot1, int slot2, int ncopies, int nx,
                                                      int ny, int nz, int U, int V, int A,
                                                      typename SS::raw *__restrict__ sino) {
-    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iv = blockIdx.y * blockDim.y + threadIdx.y;
+    const int lane = threadIdx.x; // blockDim (64, 4); consecutive lanes step along v
+    const int iu = blockIdx.x * 16 + threadIdx.y * 4 + (lane >> 4);
+    const int iv = blockIdx.y * 16 + (lane & 15);
     const int a = blockIdx.z % A;
     const size_t b = blockIdx.z / A;
     if (iu >= U || iv >= V) return;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
     const March3 p = march_params(geo[a], nx, ny, nz, iu, iv, Wp, Hp, Dp);
+    // axes (fast, middle, slow): x fastest (x, y, z), y fastest (y, x, z), z fastest (z, x, y)
+    const int F = fast[a];
+    const int slot = F == 0 ? slot0 : F == 1 ? slot1 : slot2;
+    const float mf = F == 0 ? p.m[0] : F == 1 ? p.m[1] : p.m[2];
+    const float mm = F == 0 ? p.m[1] : p.m[0];
+    const float ml = F == 2 ? p.m[1] : p.m[2];
+    const float tf = F == 0 ? p.st[0] : F == 1 ? p.st[1] : p.st[2];
+    const float tm = F == 0 ? p.st[1] : p.st[0];
+    const float tl = F == 2 ? p.st[1] : p.st[2];
+    const int Ef = F == 0 ? Wp : F == 1 ? Hp : Dp;
+    const int Em = F == 0 ? Hp : Wp;
+    const int El = F == 2 ? Hp : Dp;
     const uint32_t one = (uint32_t)Wp * Hp * Dp;
-    const rsrc_t R = make_rsrc(pad + b * one, bytes_of<SI>(one));
+    const rsrc_t R = make_rsrc(pad + (b * ncopies + slot) * one, bytes_of<SI>(one));
     const int es = (int)sizeof(typename SI::raw);
-    const int sW = Wp * es, sHW = Wp * Hp * es, sHWW = sHW + sW;
-    const int cofs = ((Dp / 2) * Hp + Hp / 2) * Wp + Wp / 2;
+    const int sM = Ef * es, sL = Ef * Em * es, sML = sL + sM;
+    const int cofs = ((El / 2) * Em + Em / 2) * Ef + Ef / 2;
     float acc = 0.0f, jf = p.j0;
     int j = 0;
     while (j < p.n) {
@@ -180,14 +237,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
         float s0 = 0.0f, s1 = 0.0f;
         for (; j + 1 < jend; j += 2, jf += 2.0f) {
             const float jg = jf + 1.0f;
-            s0 += sample3d<SI>(R, fmaf(jf, p.st[0], p.m[0]), fmaf(jf, p.st[1], p.m[1]),
-                               fmaf(jf, p.st[2], p.m[2]), Wp, Hp, cofs, sW, sHW, sHWW);
-            s1 += sample3d<SI>(R, fmaf(jg, p.st[0], p.m[0]), fmaf(jg, p.st[1], p.m[1]),
-                               fmaf(jg, p.st[2], p.m[2]), Wp, Hp, cofs, sW, sHW, sHWW);
+            s0 += sample3d<SI>(R, fmaf(jf, tf, mf), fmaf(jf, tm, mm), fmaf(jf, tl, ml), Ef, Em,
+                               cofs, sM, sL, sML);
+            s1 += sample3d<SI>(R, fmaf(jg, tf, mf), fmaf(jg, tm, mm), fmaf(jg, tl, ml), Ef, Em,
+                               cofs, sM, sL, sML);
         }
         if (j < jend) {
-            s0 += sample3d<SI>(R, fmaf(jf, p.st[0], p.m[0]), fmaf(jf, p.st[1], p.m[1]),
-                               fmaf(jf, p.st[2], p.m[2]), Wp, Hp, cofs, sW, sHW, sHWW);
+            s0 += sample3d<SI>(R, fmaf(jf, tf, mf), fmaf(jf, tm, mm), fmaf(jf, tl, ml), Ef, Em,
+                               cofs, sM, sL, sML);
             ++j;
             jf += 1.0f;
         }
@@ -629,7 +686,16 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
     const int64_t A = p->A;
 
     std::vector<FwdGeom> fgeo(A);
-    for (int64_t a = 0; a < A; ++a) fgeo[a] = cone3d::fwd_angle(g, d->phi[a], d->theta[a], d->psi[a]);
+    std::vector<int> fast(A);
+    for (int64_t a = 0; a < A; ++a) {
+        fgeo[a] = cone3d::fwd_angle(g, d->phi[a], d->theta[a], d->psi[a]);
+        const double *v = fgeo[a].dv;
+        const int f = std::fabs(v[1]) >= std::fabs(v[0]) && std::fabs(v[1]) >= std::fabs(v[2]) ? 1
+                      : std::fabs(v[2]) > std::fabs(v[0])                                   ? 2
+                                                                                            : 0;
+        fast[a] = f;
+        if (p->copy_slot[f] < 0) p->copy_slot[f] = p->ncopies++;
+    }
     std::vector<float> mref, mmat, mfdk;
     bp_matrices(g, d->phi, d->theta, d->psi, A, mref, mmat, mfdk);
     p->ycols[CMI_BP3D_REFERENCE] = y_invariant(mref, A);
@@ -644,16 +710,18 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
         ~Cleanup() {
             if (!armed) return;
             (void)hipFree(p->d_fgeo);
+            (void)hipFree(p->d_fast);
             (void)hipFree(p->d_mref);
             (void)hipFree(p->d_mmat);
             (void)hipFree(p->d_mfdk);
         }
     } cleanup{p.get()};
     CMI_HIP_TRY(hipMalloc(&p->d_fgeo, sizeof(FwdGeom) * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_fast, sizeof(int) * A));
     CMI_HIP_TRY(hipMalloc(&p->d_mref, sizeof(float) * 12 * A));
     CMI_HIP_TRY(hipMalloc(&p->d_mmat, sizeof(float) * 12 * A));
     CMI_HIP_TRY(hipMalloc(&p->d_mfdk, sizeof(float) * 12 * A));
-    p->table_bytes = (sizeof(FwdGeom) + 3 * 12 * sizeof(float)) * A;
+    p->table_bytes = (sizeof(FwdGeom) + sizeof(int) + 3 * 12 * sizeof(float)) * A;
     unsigned long long *d_total = nullptr;
     CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
     std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
@@ -664,6 +732,8 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
         s, [](hipStream_t q) { (void)hipStreamDestroy(q); });
     CMI_HIP_TRY(hipMemcpyAsync(p->d_fgeo, fgeo.data(), sizeof(FwdGeom) * A,
                                hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_fast, fast.data(), sizeof(int) * A, hipMemcpyHostToDevice,
+                               s));
     CMI_HIP_TRY(hipMemcpyAsync(p->d_mref, mref.data(), sizeof(float) * 12 * A,
                                hipMemcpyHostToDevice, s));
     CMI_HIP_TRY(hipMemcpyAsync(p->d_mmat, mmat.data(), sizeof(float) * 12 * A,
@@ -708,6 +778,7 @@ CMI_API cmi_status cmi_plan3d_destroy(cmi_plan3d *plan) {
     if (st == CMI_OK) {
         (void)hipDeviceSynchronize();
         (void)hipFree(plan->d_fgeo);
+        (void)hipFree(plan->d_fast);
         (void)hipFree(plan->d_mref);
         (void)hipFree(plan->d_mmat);
         (void)hipFree(plan->d_mfdk);
@@ -746,18 +817,39 @@ CMI_API cmi_status cmi_forward_3d(const cmi_plan3d *plan, const void *volume, in
               A = (int)plan->A;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
     void *scratch = nullptr;
-    CMI_HIP_TRY(hipMallocAsync(&scratch, (size_t)Wp * Hp * Dp * dtype_size(vol_dtype) * batch, s));
+    const size_t one = (size_t)Wp * Hp * Dp, nc = (size_t)plan->ncopies;
+    CMI_HIP_TRY(hipMallocAsync(&scratch, one * nc * dtype_size(vol_dtype) * batch, s));
     const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto si, auto ss) -> cmi_status {
         using SI = decltype(si);
         using SS = decltype(ss);
         using RI = typename SI::raw;
-        dim3 pb(256), pg((unsigned)((Wp + 255) / 256), (unsigned)Hp, (unsigned)(Dp * batch));
-        hipLaunchKernelGGL((k_stage3d<SI>), pg, pb, 0, s, (const RI *)volume, (RI *)scratch, nx,
-                           ny, nz, Wp, Hp, Dp);
-        dim3 fb(16, 16), fg((unsigned)((U + 15) / 16), (unsigned)((V + 15) / 16),
-                            (unsigned)(A * batch));
+        for (uint64_t b = 0; b < batch; ++b) {
+            const RI *src = (const RI *)volume + b * (size_t)nx * ny * nz;
+            RI *copies = (RI *)scratch + b * nc * one;
+            if (plan->copy_slot[0] >= 0) {
+                dim3 pb(256), pg((unsigned)((Wp + 255) / 256), (unsigned)Hp, (unsigned)Dp);
+                hipLaunchKernelGGL((k_stage3d<SI>), pg, pb, 0, s, src,
+                                   copies + plan->copy_slot[0] * one, nx, ny, nz, Wp, Hp, Dp);
+            }
+            if (plan->copy_slot[1] >= 0) {
+                dim3 tb(32, 8), tg((unsigned)((Wp + 31) / 32), (unsigned)((Hp + 31) / 32),
+                                   (unsigned)Dp);
+                hipLaunchKernelGGL((k_stage3d_t<SI, 1>), tg, tb, 0, s, src,
+                                   copies + plan->copy_slot[1] * one, nx, ny, nz, Wp, Hp, Dp);
+            }
+            if (plan->copy_slot[2] >= 0) {
+                dim3 tb(32, 8), tg((unsigned)((Wp + 31) / 32), (unsigned)((Dp + 31) / 32),
+                                   (unsigned)Hp);
+                hipLaunchKernelGGL((k_stage3d_t<SI, 2>), tg, tb, 0, s, src,
+                                   copies + plan->copy_slot[2] * one, nx, ny, nz, Wp, Hp, Dp);
+            }
+        }
+        dim3 fb(64, 4), fg((unsigned)((U + 15) / 16), (unsigned)((V + 15) / 16),
+                           (unsigned)(A * batch));
         hipLaunchKernelGGL((k_fwd3d<SI, SS>), fg, fb, 0, s, (const RI *)scratch, plan->d_fgeo,
-                           nx, ny, nz, U, V, A, (typename SS::raw *)sinogram);
+                           plan->d_fast, plan->copy_slot[0], plan->copy_slot[1],
+                           plan->copy_slot[2], plan->ncopies, nx, ny, nz, U, V, A,
+                           (typename SS::raw *)sinogram);
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });

@@ -105,6 +105,24 @@ def test_general_geometry(oracle, general, gpu):
     check_close(host(cm.adjoint(dev(y, gpu), geom)), oracle.adj3d(rec, y), "general adj3d")
 
 
+def test_tilted_orbits_use_every_volume_copy(oracle, gpu):
+    """The forward reads, per projection, the volume copy in which the axis closest to the
+    detector's v direction is fastest. theta = 0 keeps v along y; theta = pi/2 turns it into the
+    xz-plane, along x for phi near 0 and along z for phi near pi/2, so this plan stages all three
+    copies. Batched, so that every copy is staged per image."""
+    r = np.random.default_rng(11)
+    phi = np.concatenate([np.linspace(0, 2 * np.pi, 4, endpoint=False),
+                          r.uniform(-0.3, 0.3, 4), np.pi / 2 + r.uniform(-0.3, 0.3, 4)])
+    theta = np.concatenate([np.zeros(4), np.full(8, np.pi / 2) + r.uniform(-0.2, 0.2, 8)])
+    psi = r.uniform(-0.2, 0.2, 12)
+    rec = cone(20, 30, 26, 12, 50.0, 80.0, angles=np.vstack([phi, theta, psi]), nz=18, ny=22)
+    geom = as_cone_geometry(rec)
+    vols = r.random((2, 18, 22, 20))
+    got = host(cm.forward(dev(vols, gpu), geom))
+    for i in range(2):
+        check_close(got[i], oracle.fwd3d(rec, vols[i]), f"tilted fwd3d [{i}]")
+
+
 def test_tiny_golden_3d(gpu):
     import os
     d = np.load(os.path.join(os.path.dirname(__file__), "golden", "cone3d_tiny.npz"))
