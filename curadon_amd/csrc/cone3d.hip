@@ -272,28 +272,64 @@ __global__ __launch_bounds__(kMaxBlock) void k_ray_table3d(const FwdGeom *__rest
     rs[r] = make_float4(p.st[0], p.st[1], p.st[2], __int_as_float(p.n));
 }
 
-// Matched adjoint, tile-driven: a workgroup owns a kAT^3 voxel tile with FP32 accumulators in
-// LDS. For each angle it projects the tile's box, grown by one voxel on each side (the samples
+// Largest |y| of a projection stack, in two steps: kMaxParts workgroups each write the maximum
+// of their grid-stride share; the adjoint reduces the kMaxParts values.
+constexpr int kMaxParts = 256;
+
+template <class S>
+__global__ __launch_bounds__(kMaxBlock) void k_absmax_parts(const typename S::raw *__restrict__ y,
+                                                            size_t n, float *__restrict__ parts) {
+    __shared__ float red[kMaxBlock / 64];
+    float m = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * kMaxBlock + threadIdx.x; i < n;
+         i += (size_t)kMaxParts * kMaxBlock)
+        m = fmaxf(m, fabsf(to_f32<S>(y[i])));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kMaxBlock / 64; ++w) m = fmaxf(m, red[w]);
+        parts[blockIdx.x] = m;
+    }
+}
+
+// Matched adjoint, tile-driven: a workgroup owns a kAT^3 voxel tile with accumulators in LDS.
+// For each angle it projects the tile's box, grown by one voxel on each side (the samples
 // whose trilinear footprint reaches the tile), through the geometry-matched matrices onto the
 // panel; the rays through the covered pixels (plus one pixel of margin) are the candidates. Each
 // candidate walks only its samples inside the grown box, at the forward's FP32 positions, and
-// adds y * w to the tile voxels its footprint covers (LDS float atomics: the last bits depend on
-// the order in which they land). The tile is added to the volume once, in its storage type.
+// adds y * w to the tile voxels its footprint covers. The accumulators are 64-bit fixed point
+// in units of max|y| / 2^30: a contribution y * w is rounded to that grid (|y w| <= max|y|, so
+// it fits in 32 bits) and added with an integer LDS atomic, which gfx950 executes about 45
+// times faster than an FP32 LDS atomic add. Integer sums are exact, so the result does not
+// depend on the order of the additions; the rounding of each term is at most max|y| 2^-31.
+// The tile is added to the volume once, in its storage type.
 constexpr int kAT = 16;
+constexpr int kPh = 4; // candidate rays of one wavefront are kPh pixels apart in u and v
+// LDS tile pitches: odd row and slice pitches (17 and 273 floats, 273 = 17 mod 32) spread the
+// voxels that differ only in y or z over different banks, as the lanes of one atomic
+// instruction hit voxels at various depths along their rays
+constexpr int kATRow = kAT + 1, kATSlice = kAT * kATRow + 1;
 
 template <class SV, class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restrict__ vol,
                                                      const typename SS::raw *__restrict__ sino,
                                                      const float4 *__restrict__ rm,
                                                      const float4 *__restrict__ rs,
-                                                     const float *__restrict__ M, int nx, int ny,
-                                                     int nz, int U, int V, int A) {
-    __shared__ float tile[kAT * kAT * kAT];
+                                                     const float *__restrict__ M,
+                                                     const float *__restrict__ parts, int nx,
+                                                     int ny, int nz, int U, int V, int A) {
+    __shared__ unsigned long long tile[kAT * kATSlice];
     const int tid = threadIdx.x;
+    // fixed-point scale: max|y| maps to 2^30
+    float ymax = 0.0f;
+    for (int i = tid & 63; i < kMaxParts; i += 64) ymax = fmaxf(ymax, parts[i]);
+    for (int off = 32; off > 0; off >>= 1) ymax = fmaxf(ymax, __shfl_xor(ymax, off, 64));
+    const float yscale = ymax > 0.0f ? 1073741824.0f / ymax : 0.0f;
     const int ntz = (nz + kAT - 1) / kAT;
     const int x0 = blockIdx.x * kAT, y0 = blockIdx.y * kAT, z0 = (blockIdx.z % ntz) * kAT;
     const size_t b = blockIdx.z / ntz;
-    for (int i = tid; i < kAT * kAT * kAT; i += kMaxBlock) tile[i] = 0.0f;
+    for (int i = tid; i < kAT * kATSlice; i += kMaxBlock) tile[i] = 0ull;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
     // centred padded coordinate p -> voxel index floor(p) + c
     const int cx = Wp / 2 - kPad, cy = Hp / 2 - kPad, cz = Dp / 2 - kPad;
@@ -339,18 +375,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
         }
         const int niu = iu1 - iu0 + 1, niv = iv1 - iv0 + 1;
         if (niu <= 0 || niv <= 0) continue;
-        const int nc = niu * niv;
-        const float rn = 1.0f / (float)niu; // c / niu by (c + .5) rn, corrected below
+        // Candidates in phase order: phase (pu, pv) in [0, kPh)^2 holds the rays
+        // (pu + kPh qu, pv + kPh qv), so the consecutive candidates of a wavefront are kPh
+        // pixels apart in u and v and their samples' taps rarely meet in one atomic instruction
+        // (LDS serialises lanes that add to the same address). Padding the phases to equal
+        // size leaves holes past the rectangle's edge, which are skipped.
+        const int nqu = (niu + kPh - 1) / kPh, nqv = (niv + kPh - 1) / kPh;
+        const int nq = nqu * nqv, nc = kPh * kPh * nq;
         for (int c = tid; c < nc; c += kMaxBlock) {
-            int jv = (int)(((float)c + 0.5f) * rn);
-            jv += (c - jv * niu >= niu) ? 1 : 0; // one correction step keeps the division exact
-            jv -= (c - jv * niu < 0) ? 1 : 0;    // whatever the rounding of rn
-            const int iu = iu0 + (c - jv * niu), iv = iv0 + jv;
+            const int ph = c / nq, rq = c - ph * nq;
+            const int qv = rq / nqu, qu = rq - qv * nqu;
+            const int lu = (ph % kPh) + kPh * qu, lv = (ph / kPh) + kPh * qv;
+            if (lu >= niu || lv >= niv) continue;
+            const int iu = iu0 + lu, iv = iv0 + lv;
             const size_t r = ((size_t)a * V + iv) * U + iu;
             const float4 s4 = rs[r];
             const int n = __float_as_int(s4.w);
             if (n <= 0) continue;
-            const float y = to_f32<SS>(sb[r]);
+            const float y = to_f32<SS>(sb[r]) * yscale; // |y| <= 2^30
             if (y == 0.0f) continue;
             const float4 m4 = rm[r];
             // jf range with p in [lo, hi) on every axis, one sample of margin each side
@@ -388,19 +430,25 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
                 for (int k = 0; k < 8; ++k) {
                     const int X = lx + (k & 1), Y = ly + ((k >> 1) & 1), Z = lz + (k >> 2);
                     if ((unsigned)X < (unsigned)kAT && (unsigned)Y < (unsigned)kAT &&
-                        (unsigned)Z < (unsigned)kAT)
-                        atomicAdd(&tile[(Z * kAT + Y) * kAT + X],
-                                  wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2]);
+                        (unsigned)Z < (unsigned)kAT) {
+                        const int q = (int)__builtin_rintf(wx[k & 1] * wy[(k >> 1) & 1] *
+                                                           wz[k >> 2]);
+                        atomicAdd(&tile[Z * kATSlice + Y * kATRow + X],
+                                  (unsigned long long)(long long)q);
+                    }
                 }
             }
         }
     }
     __syncthreads();
     for (int i = tid; i < kAT * kAT * kAT; i += kMaxBlock) {
-        const int X = x0 + (i % kAT), Y = y0 + (i / kAT) % kAT, Z = z0 + i / (kAT * kAT);
+        const int lx = i % kAT, ly = (i / kAT) % kAT, lz = i / (kAT * kAT);
+        const int X = x0 + lx, Y = y0 + ly, Z = z0 + lz;
         if (X < nx && Y < ny && Z < nz) {
             const size_t o = ((b * nz + Z) * ny + Y) * (size_t)nx + X;
-            vol[o] = from_f32<SV>(to_f32<SV>(vol[o]) + tile[i]);
+            const long long q = (long long)tile[lz * kATSlice + ly * kATRow + lx];
+            const float add = (float)((double)q * ((double)ymax * 0x1p-30));
+            vol[o] = from_f32<SV>(to_f32<SV>(vol[o]) + add);
         }
     }
 }
@@ -928,20 +976,25 @@ CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_
     const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, U = (int)g.nu, V = (int)g.nv,
               A = (int)plan->A;
     const size_t rays = (size_t)A * U * V;
-    float4 *table = nullptr; // [2][rays]
-    CMI_HIP_TRY(hipMallocAsync((void **)&table, 2 * rays * sizeof(float4), s));
+    float4 *table = nullptr; // [2][rays], then kMaxParts floats of partial maxima of |y|
+    CMI_HIP_TRY(hipMallocAsync((void **)&table, 2 * rays * sizeof(float4) +
+                                                    kMaxParts * sizeof(float), s));
+    float *parts = reinterpret_cast<float *>(table + 2 * rays);
     const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
         using SV = decltype(sv);
         using SS = decltype(ss);
         dim3 tb(256), tg((unsigned)((U + 255) / 256), (unsigned)V, (unsigned)A);
         hipLaunchKernelGGL(k_ray_table3d, tg, tb, 0, s, plan->d_fgeo, nx, ny, nz, U, V, table,
                            table + rays);
+        hipLaunchKernelGGL((k_absmax_parts<SS>), dim3(kMaxParts), dim3(kMaxBlock), 0, s,
+                           (const typename SS::raw *)sinogram, rays * batch, parts);
         const int ntz = (nz + kAT - 1) / kAT;
         dim3 ab(kMaxBlock), ag((unsigned)((nx + kAT - 1) / kAT), (unsigned)((ny + kAT - 1) / kAT),
                                (unsigned)(ntz * batch));
         hipLaunchKernelGGL((k_adj3d<SV, SS>), ag, ab, 0, s, (typename SV::raw *)volume,
                            (const typename SS::raw *)sinogram, (const float4 *)table,
-                           (const float4 *)(table + rays), plan->d_mmat, nx, ny, nz, U, V, A);
+                           (const float4 *)(table + rays), plan->d_mmat, parts, nx, ny, nz,
+                           U, V, A);
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });
