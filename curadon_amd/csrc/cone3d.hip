@@ -272,14 +272,18 @@ __global__ __launch_bounds__(kMaxBlock) void k_ray_table3d(const FwdGeom *__rest
     rs[r] = make_float4(p.st[0], p.st[1], p.st[2], __int_as_float(p.n));
 }
 
-// Largest |y| of a projection stack, in two steps: kMaxParts workgroups each write the maximum
-// of their grid-stride share; the adjoint reduces the kMaxParts values.
+// Largest |y| of each projection stack of a batch, in two steps: per stack b, kMaxParts
+// workgroups each write the maximum of their grid-stride share to parts[b][*]; the adjoint
+// reduces the kMaxParts values. Each volume of a batch gets its own fixed-point unit, so that
+// its result does not depend on the other volumes.
 constexpr int kMaxParts = 256;
 
 template <class S>
 __global__ __launch_bounds__(kMaxBlock) void k_absmax_parts(const typename S::raw *__restrict__ y,
                                                             size_t n, float *__restrict__ parts) {
     __shared__ float red[kMaxBlock / 64];
+    y += (size_t)blockIdx.y * n;
+    parts += (size_t)blockIdx.y * kMaxParts;
     float m = 0.0f;
     for (size_t i = (size_t)blockIdx.x * kMaxBlock + threadIdx.x; i < n;
          i += (size_t)kMaxParts * kMaxBlock)
@@ -321,14 +325,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
                                                      int ny, int nz, int U, int V, int A) {
     __shared__ unsigned long long tile[kAT * kATSlice];
     const int tid = threadIdx.x;
-    // fixed-point scale: max|y| maps to 2^30
-    float ymax = 0.0f;
-    for (int i = tid & 63; i < kMaxParts; i += 64) ymax = fmaxf(ymax, parts[i]);
-    for (int off = 32; off > 0; off >>= 1) ymax = fmaxf(ymax, __shfl_xor(ymax, off, 64));
-    const float yscale = ymax > 0.0f ? 1073741824.0f / ymax : 0.0f;
     const int ntz = (nz + kAT - 1) / kAT;
     const int x0 = blockIdx.x * kAT, y0 = blockIdx.y * kAT, z0 = (blockIdx.z % ntz) * kAT;
     const size_t b = blockIdx.z / ntz;
+    // fixed-point scale: max|y| of this volume's projections maps to 2^30
+    float ymax = 0.0f;
+    for (int i = tid & 63; i < kMaxParts; i += 64) ymax = fmaxf(ymax, parts[b * kMaxParts + i]);
+    for (int off = 32; off > 0; off >>= 1) ymax = fmaxf(ymax, __shfl_xor(ymax, off, 64));
+    const float yscale = ymax > 0.0f ? 1073741824.0f / ymax : 0.0f;
     for (int i = tid; i < kAT * kATSlice; i += kMaxBlock) tile[i] = 0ull;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
     // centred padded coordinate p -> voxel index floor(p) + c
@@ -976,9 +980,9 @@ CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_
     const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, U = (int)g.nu, V = (int)g.nv,
               A = (int)plan->A;
     const size_t rays = (size_t)A * U * V;
-    float4 *table = nullptr; // [2][rays], then kMaxParts floats of partial maxima of |y|
+    float4 *table = nullptr; // [2][rays], then [batch][kMaxParts] partial maxima of |y|
     CMI_HIP_TRY(hipMallocAsync((void **)&table, 2 * rays * sizeof(float4) +
-                                                    kMaxParts * sizeof(float), s));
+                                                    kMaxParts * batch * sizeof(float), s));
     float *parts = reinterpret_cast<float *>(table + 2 * rays);
     const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
         using SV = decltype(sv);
@@ -986,8 +990,8 @@ CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_
         dim3 tb(256), tg((unsigned)((U + 255) / 256), (unsigned)V, (unsigned)A);
         hipLaunchKernelGGL(k_ray_table3d, tg, tb, 0, s, plan->d_fgeo, nx, ny, nz, U, V, table,
                            table + rays);
-        hipLaunchKernelGGL((k_absmax_parts<SS>), dim3(kMaxParts), dim3(kMaxBlock), 0, s,
-                           (const typename SS::raw *)sinogram, rays * batch, parts);
+        hipLaunchKernelGGL((k_absmax_parts<SS>), dim3(kMaxParts, (unsigned)batch),
+                           dim3(kMaxBlock), 0, s, (const typename SS::raw *)sinogram, rays, parts);
         const int ntz = (nz + kAT - 1) / kAT;
         dim3 ab(kMaxBlock), ag((unsigned)((nx + kAT - 1) / kAT), (unsigned)((ny + kAT - 1) / kAT),
                                (unsigned)(ntz * batch));

@@ -508,81 +508,176 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp2d(typename SV::raw *__restrict
 
 // ---------------------------------------------------------------- adjoint ------------------
 
-// Exact transpose of k_fwd2d in gather form: texel (tx, ty) collects, for every angle, the
-// samples of the rays whose bilinear footprint contains it. Candidate rays come from projecting
-// the footprint box [tx-.5, tx+1.5) x [ty-.5, ty+1.5) onto the detector; along a ray's
-// dominant axis exactly two samples can touch the texel, and their weights are evaluated with
-// the forward's own expressions.
-template <class SV, class SS>
-__global__ __launch_bounds__(kMaxBlock) void k_adj2d(typename SV::raw *__restrict__ vol,
-                                               const typename SS::raw *__restrict__ sino,
-                                               const float *__restrict__ ageo,
-                                               const float4 *__restrict__ ev,
-                                               const float4 *__restrict__ ns, int nx, int ny,
-                                               int D, int A) {
-    const int tx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ty = blockIdx.y * blockDim.y + threadIdx.y;
-    if (tx >= nx || ty >= ny) return;
-    const size_t b = blockIdx.z;
-    const uint32_t sino_elems = (uint32_t)A * D;
-    const rsrc_t R = make_rsrc(sino + b * sino_elems, bytes_of<SS>(sino_elems));
-    // footprint box in the plan's texel-centre frame
-    const float qx[2] = {(float)tx - 0.5f, (float)tx + 1.5f};
-    const float qy[2] = {(float)ty - 0.5f, (float)ty + 1.5f};
-    // the texel's lower-tap index in the ray table's centred padded frame
-    const int px = tx + kPad - (nx + 2 * kPad) / 2, py = ty + kPad - (ny + 2 * kPad) / 2;
-    const float fpx = (float)px, fpy = (float)py;
-    float acc = 0.0f;
-    for (int a = 0; a < A; ++a) {
+// Largest |y q.x| over the rays of each sinogram of a batch (q.x the ray's scale), in two
+// steps: per sinogram b, kAdjParts workgroups each write the maximum of their grid-stride
+// share to parts[b][*]; the adjoint reduces the kAdjParts values. Each image of a batch gets
+// its own fixed-point unit, so that its result does not depend on the other images.
+constexpr int kAdjParts = 256;
+
+template <class SS>
+__global__ __launch_bounds__(kMaxBlock) void k_adj2d_absmax(const typename SS::raw *__restrict__ sino,
+                                                           const float4 *__restrict__ ns,
+                                                           size_t rays, float *__restrict__ parts) {
+    __shared__ float red[kMaxBlock / 64];
+    const size_t b = blockIdx.y;
+    sino += b * rays;
+    parts += b * kAdjParts;
+    float m = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * kMaxBlock + threadIdx.x; i < rays;
+         i += (size_t)kAdjParts * kMaxBlock)
+        m = fmaxf(m, fabsf(to_f32<SS>(sino[i]) * ns[i].x));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kMaxBlock / 64; ++w) m = fmaxf(m, red[w]);
+        parts[blockIdx.x] = m;
+    }
+}
+
+__device__ __forceinline__ float adj2d_ymax(const float *__restrict__ parts) {
+    float m = 0.0f;
+    for (int i = threadIdx.x & 63; i < kAdjParts; i += 64) m = fmaxf(m, parts[i]);
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    return m;
+}
+
+// Matched adjoint (exact transpose of k_fwd2d), tile-driven. A workgroup owns a kT2 x kT2 texel
+// tile and a group of angles. Per angle, the candidate rays are those through the tile's
+// footprint box (every texel's [t - .5, t + 1.5) box, in the ray search frame, projected onto
+// the detector through its corners); they are staged in LDS, and the (ray, row) pairs of the
+// kT2 + 1 dominant rows whose samples reach the tile are spread over the workgroup. A sample on
+// row pd (lower tap) with column c adds y q.x times its forward weights to the texels
+// (pd, floor c), (pd, floor c + 1), (pd + 1, ...) (fan2d.hip header), in 64-bit fixed point
+// (units of max|y q.x| / 2^30, one term fits in 32 bits) with integer LDS atomics: gfx950 runs
+// them about 45 times faster than FP32 LDS atomic adds, and integer sums are exact, so the
+// result does not depend on the order of the additions. Each tile then adds its sums to a
+// 64-bit image with integer global atomics (several angle groups share a tile), and
+// k_adj2d_finish adds that image to the volume.
+constexpr int kT2 = 32;       // texel tile edge
+constexpr int kT2Rays = 128;  // candidate rays staged per chunk
+constexpr int kT2Pitch = kT2 + 1;
+
+template <class SS>
+__global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__restrict__ accimg,
+                                                         const typename SS::raw *__restrict__ sino,
+                                                         const float *__restrict__ ageo,
+                                                         const float4 *__restrict__ ev,
+                                                         const float4 *__restrict__ ns,
+                                                         const float *__restrict__ parts, int nx,
+                                                         int ny, int D, int A, int G) {
+    __shared__ unsigned long long tile[kT2 * kT2Pitch];
+    __shared__ float r_mc[kT2Rays], r_vc[kT2Rays], r_yv[kT2Rays], r_fb[kT2Rays];
+    __shared__ int r_r0[kT2Rays], r_sd[kT2Rays], r_jlo[kT2Rays], r_jhi[kT2Rays];
+    const int tid = threadIdx.x;
+    const int g = blockIdx.z % G;
+    const size_t b = blockIdx.z / G;
+    const int tx0 = blockIdx.x * kT2, ty0 = blockIdx.y * kT2;
+    const int a0 = (int)((int64_t)g * A / G), a1 = (int)((int64_t)(g + 1) * A / G);
+    for (int i = tid; i < kT2 * kT2Pitch; i += kMaxBlock) tile[i] = 0ull;
+    const float ymax = adj2d_ymax(parts + b * kAdjParts);
+    const float yscale = ymax > 0.0f ? 1073741824.0f / ymax : 0.0f;
+    // the tile's first texel in the ray table's centred padded frame
+    const int px0 = tx0 + kPad - (nx + 2 * kPad) / 2, py0 = ty0 + kPad - (ny + 2 * kPad) / 2;
+    // footprint box corner this lane projects (lanes 0..3 of every wavefront)
+    const float qx = (tid & 1) ? (float)tx0 + (float)kT2 + 0.5f : (float)tx0 - 0.5f;
+    const float qy = (tid & 2) ? (float)ty0 + (float)kT2 + 0.5f : (float)ty0 - 0.5f;
+    const typename SS::raw *sb = sino + b * (size_t)A * D;
+    for (int a = a0; a < a1; ++a) {
         const float *gp = ageo + 6 * a; // wave-uniform
-        const float sxw = gp[0], syw = gp[1], ox = gp[2], oy = gp[3], dx = gp[4], dy = gp[5];
-        float kmin = INFINITY, kmax = -INFINITY;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const float wx = qx[c & 1] - sxw, wy = qy[c >> 1] - syw;
-            const float num = wx * oy - wy * ox;
-            const float den = wx * dy - wy * dx;
-            const float kk = -num / den;
-            kmin = fminf(kmin, kk);
-            kmax = fmaxf(kmax, kk);
+        const float wx = qx - gp[0], wy = qy - gp[1];
+        float kk = -(wx * gp[3] - wy * gp[2]) / (wx * gp[5] - wy * gp[4]);
+        float kmin = kk, kmax = kk;
+        for (int off = 1; off < 4; off <<= 1) {
+            kmin = fminf(kmin, __shfl_xor(kmin, off, 64));
+            kmax = fmaxf(kmax, __shfl_xor(kmax, off, 64));
         }
+        kmin = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(kmin)));
+        kmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(kmax)));
+        if (!(kmax >= -1.0f) || !(kmin <= (float)D)) continue; // misses the detector (or NaN)
         const int k0 = max((int)floorf(kmin - 0.01f), 0);
         const int k1 = min((int)ceilf(kmax + 0.01f), D - 1);
-        for (int k = k0; k <= k1; ++k) {
-            const size_t r = (size_t)a * D + k;
-            const float4 q = ns[r];
-            const int n = __float_as_int(q.y);
-            if (n == 0) continue;
-            const float4 e = ev[r];
-            const bool ydom = fabsf(e.w) == 1.0f;
-            // the forward's samples (fan2d.hip header): sample jf has dominant lower tap
-            // floor(md) + s jf with weight fract(md), so the texel's dominant index pd is
-            // touched by jf = s (pd - r0) (weight 1 - fb) and jf = s (pd - 1 - r0) (weight fb)
-            const float md = ydom ? e.y : e.x, sd = ydom ? e.w : e.z;
-            const float mc = ydom ? e.x : e.y, vc = ydom ? e.z : e.w;
-            const int pd = ydom ? py : px, pc = ydom ? px : py;
-            const int r0 = (int)floorf(md);
-            const float fb = md - (float)r0;
-            const int s = sd > 0.0f ? 1 : -1;
-            const int jlo = (int)q.z, jhi = jlo + n - 1;
-            float part = 0.0f;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int j = s * (pd - h - r0);
-                if (j < jlo || j > jhi) continue;
-                const float c = fmaf((float)j, vc, mc);
-                const int ic = floor_i32(c);
-                const float fc = __builtin_amdgcn_fractf(c);
-                const float wc = ic == pc ? 1.0f - fc : (ic == pc - 1 ? fc : 0.0f);
-                part = fmaf(h == 0 ? 1.0f - fb : fb, wc, part);
+        for (int kc = k0; kc <= k1; kc += kT2Rays) {
+            const int nk = min(kT2Rays, k1 - kc + 1);
+            __syncthreads(); // the previous chunk's readers are done
+            if (tid < nk) {
+                const size_t r = (size_t)a * D + kc + tid;
+                const float4 q = ns[r];
+                const int n = __float_as_int(q.y);
+                const float4 e = ev[r];
+                const bool ydom = fabsf(e.w) == 1.0f;
+                const float md = ydom ? e.y : e.x, sd = ydom ? e.w : e.z;
+                const int r0 = (int)floorf(md);
+                const float yv = n > 0 ? to_f32<SS>(sb[r]) * q.x * yscale : 0.0f; // |yv| <= 2^30
+                r_mc[tid] = ydom ? e.x : e.y;
+                r_vc[tid] = ydom ? e.z : e.w;
+                r_yv[tid] = yv;
+                r_fb[tid] = md - (float)r0;
+                r_r0[tid] = r0;
+                r_sd[tid] = (sd > 0.0f ? 1 : 0) | (ydom ? 2 : 0);
+                r_jlo[tid] = yv != 0.0f ? (int)q.z : 1; // empty range for rays adding nothing
+                r_jhi[tid] = yv != 0.0f ? (int)q.z + n - 1 : 0;
             }
-            if (part == 0.0f) continue;
-            const float yv = buf_load<SS>(R, (int)r) * q.x;
-            acc = fmaf(part, yv, acc);
+            __syncthreads();
+            // items (ray, t): the sample whose dominant lower tap is pd = d0 - 1 + t
+            const int items = nk * (kT2 + 1);
+            for (int it = tid; it < items; it += kMaxBlock) {
+                const int ri = it / (kT2 + 1), t = it - ri * (kT2 + 1);
+                const int sdv = r_sd[ri];
+                const bool ydom = (sdv & 2) != 0;
+                const int s = (sdv & 1) ? 1 : -1;
+                const int d0 = ydom ? py0 : px0, c0 = ydom ? px0 : py0;
+                const int jf = s * (d0 - 1 + t - r_r0[ri]);
+                if (jf < r_jlo[ri] || jf > r_jhi[ri]) continue;
+                const float c = fmaf((float)jf, r_vc[ri], r_mc[ri]);
+                const int lc = floor_i32(c) - c0; // the lower tap's column in the tile
+                if (lc < -1 || lc >= kT2) continue;
+                const float fa = __builtin_amdgcn_fractf(c), fb = r_fb[ri], yv = r_yv[ri];
+                const float wa[2] = {yv * (1.0f - fb), yv * fb}; // rows t - 1, t
+                const float wc[2] = {1.0f - fa, fa};             // columns lc, lc + 1
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int ld = t - 1 + (k >> 1), lcc = lc + (k & 1);
+                    if ((unsigned)ld < (unsigned)kT2 && (unsigned)lcc < (unsigned)kT2) {
+                        const int q = (int)__builtin_rintf(wa[k >> 1] * wc[k & 1]);
+                        // tile rows run along y: (ld, lcc) is (y, x) for y-dominant rays
+                        const int ix = ydom ? lcc : ld, iy = ydom ? ld : lcc;
+                        atomicAdd(&tile[iy * kT2Pitch + ix], (unsigned long long)(long long)q);
+                    }
+                }
+            }
         }
     }
-    const size_t i = b * (size_t)nx * ny + (size_t)ty * nx + tx;
-    vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc);
+    __syncthreads();
+    unsigned long long *out = accimg + b * (size_t)nx * ny;
+    for (int i = tid; i < kT2 * kT2; i += kMaxBlock) {
+        const int lx = i % kT2, ly = i / kT2;
+        const int X = tx0 + lx, Y = ty0 + ly;
+        const unsigned long long v = tile[ly * kT2Pitch + lx];
+        if (X < nx && Y < ny && v != 0ull) atomicAdd(&out[(size_t)Y * nx + X], v);
+    }
+}
+
+// vol += accimg * max|y q.x| / 2^30.
+template <class SV>
+__global__ __launch_bounds__(kMaxBlock) void k_adj2d_finish(typename SV::raw *__restrict__ vol,
+                                                           unsigned long long *__restrict__ accimg,
+                                                           const float *__restrict__ parts,
+                                                           size_t img) {
+    const size_t b = blockIdx.y;
+    const float ymax = adj2d_ymax(parts + b * kAdjParts);
+    const double unit = (double)ymax * 0x1p-30;
+    const size_t j = (size_t)blockIdx.x * kMaxBlock + threadIdx.x;
+    if (j >= img) return;
+    const size_t i = b * img + j;
+    const long long q = (long long)accimg[i];
+    vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + (float)((double)q * unit));
+}
+
+__global__ __launch_bounds__(kMaxBlock) void k_fill_u64(unsigned long long *__restrict__ p,
+                                                        size_t n) {
+    const size_t i = (size_t)blockIdx.x * kMaxBlock + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 // ---------------------------------------------------------------- dispatch -----------------
@@ -908,16 +1003,38 @@ CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_
     CMI_TRY(guard.enter(plan->device));
     const hipStream_t s = (hipStream_t)stream;
     const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
-    return dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
+    const size_t img = (size_t)nx * ny * batch, rays = (size_t)A * D;
+    // scratch: the 64-bit sum image, then kAdjParts partial maxima
+    void *scratch = nullptr;
+    CMI_HIP_TRY(hipMallocAsync(&scratch, img * 8 + kAdjParts * batch * sizeof(float), s));
+    unsigned long long *accimg = (unsigned long long *)scratch;
+    float *parts = (float *)(accimg + img);
+    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
         using SV = decltype(sv);
         using SS = decltype(ss);
-        dim3 b(64, 4), g((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)batch);
-        hipLaunchKernelGGL((k_adj2d<SV, SS>), g, b, 0, s, (typename SV::raw *)volume,
-                           (const typename SS::raw *)sinogram, plan->d_ageo, plan->d_ev,
-                           plan->d_ns, nx, ny, D, A);
+        const unsigned nb = (unsigned)((img + kMaxBlock - 1) / kMaxBlock);
+        hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(kMaxBlock), 0, s, accimg, img);
+        hipLaunchKernelGGL((k_adj2d_absmax<SS>), dim3(kAdjParts, (unsigned)batch),
+                           dim3(kMaxBlock), 0, s, (const typename SS::raw *)sinogram, plan->d_ns,
+                           rays, parts);
+        // angle groups: enough workgroups to fill the GPU several times over
+        const unsigned tx = (unsigned)((nx + kT2 - 1) / kT2), ty = (unsigned)((ny + kT2 - 1) / kT2);
+        const uint64_t tiles = (uint64_t)tx * ty * batch;
+        const int G = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)A, (4096 + tiles - 1) / tiles));
+        hipLaunchKernelGGL((k_adj2d_tile<SS>), dim3(tx, ty, (unsigned)(G * batch)),
+                           dim3(kMaxBlock), 0, s, accimg, (const typename SS::raw *)sinogram,
+                           plan->d_ageo, plan->d_ev, plan->d_ns, parts, nx, ny, D, A, G);
+        const size_t one = (size_t)nx * ny;
+        hipLaunchKernelGGL((k_adj2d_finish<SV>),
+                           dim3((unsigned)((one + kMaxBlock - 1) / kMaxBlock), (unsigned)batch),
+                           dim3(kMaxBlock), 0, s, (typename SV::raw *)volume, accimg, parts, one);
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });
+    const hipError_t fe = hipFreeAsync(scratch, s);
+    if (st != CMI_OK) return st;
+    CMI_HIP_TRY(fe);
+    return CMI_OK;
 }
 
 } // extern "C"

@@ -139,10 +139,13 @@ def test_batched_equals_single_3d(small, gpu):
     xs = torch.rand(2, 32, 32, 32, device=gpu)
     sb = cm.forward(xs, geom)
     ys = torch.rand(2, 24, 40, 48, device=gpu)
+    ys[1] *= 3.0  # a different fixed-point unit for the adjoint of each stack
     vb = cm.backward(ys, geom)
+    ab = cm.adjoint(ys, geom)
     for i in range(2):
         assert torch.equal(sb[i], cm.forward(xs[i].contiguous(), geom))
         assert torch.equal(vb[i], cm.backward(ys[i].contiguous(), geom))
+        assert torch.equal(ab[i], cm.adjoint(ys[i].contiguous(), geom))
 
 
 def test_reference_binding_signature(oracle, small, gpu):

@@ -203,6 +203,7 @@ def test_batched_equals_single(gpu):
     for i in range(3):
         assert torch.equal(sb[i], cm.forward(xs[i].contiguous(), geom))
     ys = torch.rand(3, 64, 136, device=gpu)
+    ys[2] *= 5.0  # a different fixed-point unit for the adjoint of each sinogram
     vb = cm.backward(ys, geom)
     ab = cm.adjoint(ys, geom)
     for i in range(3):
