@@ -3,7 +3,9 @@
 Drop-in names (python/standalone/src/curadon/__init__.py:1-15): FanGeometry, ConeGeometry,
 forward, backward, TorchForward, TorchBackprojection, plan_2d, forward_2d, backward_2d,
 forward_3d, backward_3d. Additions: adjoint / adjoint_2d / TorchAdjoint (matched transpose),
-fbp / fbp_filter (filtered backprojection, FDK for cone beam).
+fbp / fbp_filter (filtered backprojection, FDK for cone beam). Stream and event pools under
+curadon's names (event, event_view, event_pool, stream, stream_view, stream_pool,
+get_next_stream, get_stream, get_next_event, get_event) come from curadon_amd.streams.
 Usage: ``import curadon_amd as curadon``.
 """
 from .geometry import ConeGeometry, FanGeometry
@@ -14,9 +16,13 @@ from .forward import forward
 from .backward import adjoint, backward
 from .diffable import TorchAdjoint, TorchBackprojection, TorchForward
 from .reconstruction import fbp, fbp_filter
+from .streams import (event, event_pool, event_view, get_event, get_next_event, get_next_stream,
+                      get_stream, stream, stream_pool, stream_view)
 
 __all__ = [
     "FanGeometry", "ConeGeometry", "forward", "backward", "adjoint", "TorchForward",
     "TorchBackprojection", "TorchAdjoint", "plan_2d", "forward_2d", "backward_2d",
     "adjoint_2d", "forward_3d", "backward_3d", "plan_3d", "CuradonError", "fbp", "fbp_filter",
+    "event", "event_view", "event_pool", "stream", "stream_view", "stream_pool",
+    "get_next_stream", "get_stream", "get_next_event", "get_event",
 ]
