@@ -114,6 +114,8 @@ SIGNATURES = {
     "cmi_version": (C.c_int, []),
     "cmi_plan2d_create": (_i32, [_i32, _i32, _u64p, _dp, _dp, _i32, _u64, C.c_double, C.c_double,
                                  _dp, _dp, _dp, _u64, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "cmi_plan2d_create_vec": (_i32, [_i32, _i32, _u64p, _dp, _dp, _i32, _u64, _dp, _u64,
+                                     C.POINTER(_vp)]),
     "cmi_plan2d_destroy": (_i32, [_vp]),
     "cmi_plan2d_set_block": (_i32, [_vp, C.c_uint32, C.c_uint32]),
     "cmi_plan2d_get_block": (_i32, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -260,6 +262,35 @@ class plan_2d:
             float(det_offset), DSO.ctypes.data_as(_dp), DSD.ctypes.data_as(_dp),
             angles.ctypes.data_as(_dp), A, float(det_pitch), float(COR), C.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_views(cls, device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec,
+                   det_count, vecs):
+        """Plan from ASTRA-like views ([A][6]: source x, y, detector centre x, y, cell step
+        u x, y in world coordinates; cmi_plan2d_create_vec)."""
+        self = cls.__new__(cls)
+        self._h = None
+        v = np.ascontiguousarray(np.asarray(vecs, dtype=np.float64))
+        if v.ndim != 2 or v.shape[1] != 6:
+            raise AttributeError("vecs must have shape (nangles, 6)")
+        vs = np.ascontiguousarray(np.asarray(vol_shape, dtype=np.uint64).reshape(-1))
+        if vs.size != 2:
+            raise AttributeError("vol_shape must be two values")
+        sp = _f64(vol_spacing, 2)
+        of = _f64(vol_offset, 2)
+        self.device = int(device)
+        self.vol_prec = precision_code(vol_prec)
+        self.det_prec = precision_code(det_prec)
+        self.vol_shape = (int(vs[0]), int(vs[1]))
+        self.det_count = int(det_count)
+        self.nangles = int(v.shape[0])
+        h = _vp()
+        check(_lib.cmi_plan2d_create_vec(
+            self.device, self.vol_prec, vs.ctypes.data_as(_u64p), sp.ctypes.data_as(_dp),
+            of.ctypes.data_as(_dp), self.det_prec, self.det_count, v.ctypes.data_as(_dp),
+            self.nangles, C.byref(h)))
+        self._h = h
+        return self
 
     @property
     def handle(self):

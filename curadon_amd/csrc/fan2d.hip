@@ -4,6 +4,11 @@
 // Data layout in HBM
 //   plan tables  geo  [A] AngleGeom (FP64)       source / cell-0 vector / cell step
 //                meta [A] float4                 (-sin, cos, DSD, DSO) for the backprojector
+//                                                (plans from angles / DSO / DSD)
+//                bpv  [A][3] float4              (n.x, n.y, n0, L), (e.x, e.y, e0, c0),
+//                                                (1/|u|, 0, 0, 0): the backprojector's view
+//                                                constants (plans from ASTRA-like views,
+//                                                geometry2d.hpp bp_view)
 //                ageo [A] float[6]               FP32 copy of geo for the adjoint's ray search
 //                ev   [A*D] float4               (mx, my, vx, vy) per ray: m is the sample
 //                                                j = N/2 in centred padded coordinates (see
@@ -51,6 +56,7 @@ struct cmi_plan2d {
     uint64_t table_bytes = 0;
     AngleGeom *d_geo = nullptr;
     float4 *d_meta = nullptr;
+    float4 *d_bpv = nullptr; // views (plans from vectors); d_meta is null then
     float *d_ageo = nullptr;
     float4 *d_ev = nullptr;
     float4 *d_ns = nullptr;
@@ -506,6 +512,42 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp2d(typename SV::raw *__restrict
     vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc * inv_du);
 }
 
+// The same for a plan built from ASTRA-like views (geometry2d.hpp bp_view): per view,
+// depth = n . x + n0, w = L / depth, cell coordinate w (e . x + e0) + c0, weight w^WPOW / |u|.
+// vol[b][y][x] += sum_a w_a^WPOW / |u_a| lerp(sino[b][a], cell_a).
+template <class SV, class SS, int WPOW>
+__global__ __launch_bounds__(kMaxBlock) void k_bp2d_vec(typename SV::raw *__restrict__ vol,
+                                                        const typename SS::raw *__restrict__ spad,
+                                                        const float4 *__restrict__ bpv, int nx,
+                                                        int ny, int D, int A, float sx, float sy,
+                                                        float ox, float oy) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= nx || iy >= ny) return;
+    const size_t b = blockIdx.z;
+    const int Dp = D + 2 * kPadDet;
+    const uint32_t sino_elems = (uint32_t)A * Dp;
+    const rsrc_t R = make_rsrc(spad + b * sino_elems, bytes_of<SS>(sino_elems));
+    const float x = ((float)ix - (float)nx / 2.0f) * sx + ox + 0.5f;
+    const float y = ((float)iy - (float)ny / 2.0f) * sy + oy + 0.5f;
+    const float hi = (float)D;
+    float acc = 0.0f;
+    int row = kPadDet;
+#pragma unroll 4
+    for (int a = 0; a < A; ++a, row += Dp) {
+        const float4 m0 = bpv[3 * a], m1 = bpv[3 * a + 1]; // wave-uniform: scalar loads
+        const float idu = bpv[3 * a + 2].x;
+        const float w = m0.w * __builtin_amdgcn_rcpf(fmaf(m0.x, x, fmaf(m0.y, y, m0.z)));
+        const float eta = fmaf(m1.x, x, fmaf(m1.y, y, m1.z));
+        const float u = __builtin_amdgcn_fmed3f(fmaf(w, eta, m1.w) - 0.5f, -2.0f, hi);
+        const float al = __builtin_amdgcn_fractf(u);
+        const Pair p = buf_pair<SS>(R, row + floor_i32(u), 0);
+        acc = fmaf(fmaf(al, p.hi - p.lo, p.lo), (WPOW == 2 ? w * w : w) * idu, acc);
+    }
+    const size_t i = b * (size_t)nx * ny + (size_t)iy * nx + ix;
+    vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc);
+}
+
 // ---------------------------------------------------------------- adjoint ------------------
 
 // Largest |y q.x| over the rays of each sinogram of a batch (q.x the ray's scale), in two
@@ -726,6 +768,111 @@ cmi_status check_call(const cmi_plan2d *p, const void *vol, int vol_dtype, const
     return CMI_OK;
 }
 
+// Checks and fields common to both plan constructors (Python order (y, x) -> (x, y), as
+// plan.cu:48-50).
+cmi_status check_plan_shape(int vol_prec, int det_prec, const uint64_t vol_shape[2],
+                            const double vol_spacing[2], uint64_t det_count, uint64_t nangles,
+                            const char *fn) {
+    if (!dtype_ok(vol_prec) || !dtype_ok(det_prec))
+        return fail(CMI_ERR_DTYPE, "%s: precision must be f32, f16 or bf16", fn);
+    if (vol_shape[0] == 0 || vol_shape[1] == 0 || det_count == 0 || nangles == 0)
+        return fail(CMI_ERR_INVALID_ARGUMENT,
+                    "%s: empty shape (vol %llu x %llu, det %llu, angles %llu)", fn,
+                    (unsigned long long)vol_shape[0], (unsigned long long)vol_shape[1],
+                    (unsigned long long)det_count, (unsigned long long)nangles);
+    if (vol_shape[0] > (1u << 20) || vol_shape[1] > (1u << 20) || det_count > (1u << 24) ||
+        nangles > (1u << 24) || det_count * nangles >= (1ull << 31))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "%s: shape too large", fn);
+    if (!(vol_spacing[0] > 0) || !(vol_spacing[1] > 0))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "%s: spacings must be positive", fn);
+    return CMI_OK;
+}
+
+void set_plan_shape(cmi_plan2d &p, int device, int vol_prec, const uint64_t vol_shape[2],
+                    const double vol_spacing[2], const double vol_offset[2], int det_prec,
+                    uint64_t det_count, uint64_t nangles) {
+    p.device = device;
+    p.vol_prec = vol_prec;
+    p.det_prec = det_prec;
+    p.g.nx = (int64_t)vol_shape[1];
+    p.g.ny = (int64_t)vol_shape[0];
+    p.g.sx = vol_spacing[1];
+    p.g.sy = vol_spacing[0];
+    p.g.ox = vol_offset[1];
+    p.g.oy = vol_offset[0];
+    p.g.D = (int64_t)det_count;
+    p.A = (int64_t)nangles;
+}
+
+// Shared tail of the plan constructors: per-angle tables to the device, then the FP64 ray
+// table (k_ray_table). `meta` (circular plans) or `bpv` (plans from views) is empty.
+cmi_status finish_plan2d(std::unique_ptr<cmi_plan2d> p, const std::vector<AngleGeom> &geo,
+                         const std::vector<float4> &meta, const std::vector<float4> &bpv,
+                         cmi_plan2d **plan) {
+    const int64_t A = p->A, D = p->g.D;
+    std::vector<float> ageo(6 * A);
+    for (int64_t a = 0; a < A; ++a)
+        for (int d = 0; d < 2; ++d) {
+            ageo[6 * a + d] = (float)geo[a].src[d];
+            ageo[6 * a + 2 + d] = (float)geo[a].org[d];
+            ageo[6 * a + 4 + d] = (float)geo[a].del[d];
+        }
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(p->device));
+    struct Cleanup {
+        cmi_plan2d *p;
+        bool armed = true;
+        ~Cleanup() {
+            if (!armed) return;
+            (void)hipFree(p->d_geo);
+            (void)hipFree(p->d_meta);
+            (void)hipFree(p->d_bpv);
+            (void)hipFree(p->d_ageo);
+            (void)hipFree(p->d_ev);
+            (void)hipFree(p->d_ns);
+        }
+    } cleanup{p.get()};
+    const size_t rays = (size_t)A * D;
+    CMI_HIP_TRY(hipMalloc(&p->d_geo, sizeof(AngleGeom) * A));
+    if (!meta.empty()) CMI_HIP_TRY(hipMalloc(&p->d_meta, sizeof(float4) * A));
+    if (!bpv.empty()) CMI_HIP_TRY(hipMalloc(&p->d_bpv, sizeof(float4) * bpv.size()));
+    CMI_HIP_TRY(hipMalloc(&p->d_ageo, sizeof(float) * 6 * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_ev, sizeof(float4) * rays));
+    CMI_HIP_TRY(hipMalloc(&p->d_ns, sizeof(float4) * rays));
+    p->table_bytes = sizeof(AngleGeom) * A + sizeof(float4) * (meta.size() + bpv.size()) +
+                     sizeof(float) * 6 * A + 2 * sizeof(float4) * rays;
+    unsigned long long *d_total = nullptr;
+    CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
+    std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
+        d_total, [](unsigned long long *q) { (void)hipFree(q); });
+    hipStream_t s;
+    CMI_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<ihipStream_t, void (*)(hipStream_t)> stream_guard(
+        s, [](hipStream_t q) { (void)hipStreamDestroy(q); });
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_geo, geo.data(), sizeof(AngleGeom) * A,
+                               hipMemcpyHostToDevice, s));
+    if (!meta.empty())
+        CMI_HIP_TRY(hipMemcpyAsync(p->d_meta, meta.data(), sizeof(float4) * A,
+                                   hipMemcpyHostToDevice, s));
+    if (!bpv.empty())
+        CMI_HIP_TRY(hipMemcpyAsync(p->d_bpv, bpv.data(), sizeof(float4) * bpv.size(),
+                                   hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_ageo, ageo.data(), sizeof(float) * 6 * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(unsigned long long), s));
+    dim3 blk(256), grd((unsigned)((D + 255) / 256), (unsigned)A);
+    hipLaunchKernelGGL(k_ray_table, grd, blk, 0, s, p->d_geo, (int)A, (int)D, p->g.nx, p->g.ny,
+                       p->g.sx, p->g.sy, p->d_ev, p->d_ns, d_total);
+    CMI_HIP_TRY(hipGetLastError());
+    unsigned long long total = 0;
+    CMI_HIP_TRY(hipMemcpyAsync(&total, d_total, sizeof total, hipMemcpyDeviceToHost, s));
+    CMI_HIP_TRY(hipStreamSynchronize(s));
+    p->total_samples = total;
+    cleanup.armed = false;
+    *plan = p.release();
+    return CMI_OK;
+}
+
 } // namespace
 
 // ===================================================================== C-ABI ===============
@@ -743,17 +890,9 @@ CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vo
     *plan = nullptr;
     if (!vol_shape || !vol_spacing || !vol_offset || !DSO || !DSD || !angles)
         return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create: null geometry array");
-    if (!dtype_ok(vol_prec) || !dtype_ok(det_prec))
-        return fail(CMI_ERR_DTYPE, "plan2d_create: precision must be f32, f16 or bf16");
-    if (vol_shape[0] == 0 || vol_shape[1] == 0 || det_count == 0 || nangles == 0)
-        return fail(CMI_ERR_INVALID_ARGUMENT,
-                    "plan2d_create: empty shape (vol %llu x %llu, det %llu, angles %llu)",
-                    (unsigned long long)vol_shape[0], (unsigned long long)vol_shape[1],
-                    (unsigned long long)det_count, (unsigned long long)nangles);
-    if (vol_shape[0] > (1u << 20) || vol_shape[1] > (1u << 20) || det_count > (1u << 24) ||
-        nangles > (1u << 24) || det_count * nangles >= (1ull << 31))
-        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create: shape too large");
-    if (!(vol_spacing[0] > 0) || !(vol_spacing[1] > 0) || !(det_spacing > 0))
+    CMI_TRY(check_plan_shape(vol_prec, det_prec, vol_shape, vol_spacing, det_count, nangles,
+                             "plan2d_create"));
+    if (!(det_spacing > 0))
         return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create: spacings must be positive");
     for (uint64_t a = 0; a < nangles; ++a)
         if (!std::isfinite(angles[a]) || !std::isfinite(DSO[a]) || !std::isfinite(DSD[a]))
@@ -761,85 +900,61 @@ CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vo
                         (unsigned long long)a);
 
     auto p = std::make_unique<cmi_plan2d>();
-    p->device = device;
-    p->vol_prec = vol_prec;
-    p->det_prec = det_prec;
-    // Python order (y, x) -> (x, y), as plan.cu:48-50
-    p->g.nx = (int64_t)vol_shape[1];
-    p->g.ny = (int64_t)vol_shape[0];
-    p->g.sx = vol_spacing[1];
-    p->g.sy = vol_spacing[0];
-    p->g.ox = vol_offset[1];
-    p->g.oy = vol_offset[0];
-    p->g.D = (int64_t)det_count;
+    set_plan_shape(*p, device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec, det_count,
+                   nangles);
     p->g.du = det_spacing;
     p->g.det_offset = det_offset;
-    p->A = (int64_t)nangles;
     p->pitch = det_pitch;
     p->cor = COR;
 
-    const int64_t A = p->A, D = p->g.D;
+    const int64_t A = p->A;
     std::vector<AngleGeom> geo(A);
     std::vector<float4> meta(A);
-    std::vector<float> ageo(6 * A);
     for (int64_t a = 0; a < A; ++a) {
         geo[a] = fan2d::plan_angle(p->g, angles[a], DSO[a], DSD[a]);
         meta[a] = bp2d_meta(angles[a], DSO[a], DSD[a]);
-        for (int d = 0; d < 2; ++d) {
-            ageo[6 * a + d] = (float)geo[a].src[d];
-            ageo[6 * a + 2 + d] = (float)geo[a].org[d];
-            ageo[6 * a + 4 + d] = (float)geo[a].del[d];
-        }
     }
+    return finish_plan2d(std::move(p), geo, meta, {}, plan);
+}
 
-    DeviceGuard guard;
-    CMI_TRY(guard.enter(device));
-    struct Cleanup {
-        cmi_plan2d *p;
-        bool armed = true;
-        ~Cleanup() {
-            if (!armed) return;
-            (void)hipFree(p->d_geo);
-            (void)hipFree(p->d_meta);
-            (void)hipFree(p->d_ageo);
-            (void)hipFree(p->d_ev);
-            (void)hipFree(p->d_ns);
-        }
-    } cleanup{p.get()};
-    const size_t rays = (size_t)A * D;
-    CMI_HIP_TRY(hipMalloc(&p->d_geo, sizeof(AngleGeom) * A));
-    CMI_HIP_TRY(hipMalloc(&p->d_meta, sizeof(float4) * A));
-    CMI_HIP_TRY(hipMalloc(&p->d_ageo, sizeof(float) * 6 * A));
-    CMI_HIP_TRY(hipMalloc(&p->d_ev, sizeof(float4) * rays));
-    CMI_HIP_TRY(hipMalloc(&p->d_ns, sizeof(float4) * rays));
-    p->table_bytes = sizeof(AngleGeom) * A + sizeof(float4) * A + sizeof(float) * 6 * A +
-                     2 * sizeof(float4) * rays;
-    unsigned long long *d_total = nullptr;
-    CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
-    std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
-        d_total, [](unsigned long long *q) { (void)hipFree(q); });
-    hipStream_t s;
-    CMI_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    std::unique_ptr<ihipStream_t, void (*)(hipStream_t)> stream_guard(
-        s, [](hipStream_t q) { (void)hipStreamDestroy(q); });
-    CMI_HIP_TRY(hipMemcpyAsync(p->d_geo, geo.data(), sizeof(AngleGeom) * A,
-                               hipMemcpyHostToDevice, s));
-    CMI_HIP_TRY(hipMemcpyAsync(p->d_meta, meta.data(), sizeof(float4) * A,
-                               hipMemcpyHostToDevice, s));
-    CMI_HIP_TRY(hipMemcpyAsync(p->d_ageo, ageo.data(), sizeof(float) * 6 * A,
-                               hipMemcpyHostToDevice, s));
-    CMI_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(unsigned long long), s));
-    dim3 blk(256), grd((unsigned)((D + 255) / 256), (unsigned)A);
-    hipLaunchKernelGGL(k_ray_table, grd, blk, 0, s, p->d_geo, (int)A, (int)D, p->g.nx, p->g.ny,
-                       p->g.sx, p->g.sy, p->d_ev, p->d_ns, d_total);
-    CMI_HIP_TRY(hipGetLastError());
-    unsigned long long total = 0;
-    CMI_HIP_TRY(hipMemcpyAsync(&total, d_total, sizeof total, hipMemcpyDeviceToHost, s));
-    CMI_HIP_TRY(hipStreamSynchronize(s));
-    p->total_samples = total;
-    cleanup.armed = false;
-    *plan = p.release();
-    return CMI_OK;
+CMI_API cmi_status cmi_plan2d_create_vec(int device, int vol_prec, const uint64_t vol_shape[2],
+                                         const double vol_spacing[2], const double vol_offset[2],
+                                         int det_prec, uint64_t det_count, const double *vecs,
+                                         uint64_t nangles, cmi_plan2d **plan) {
+    clear_error();
+    if (!plan) return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create_vec: output pointer is null");
+    *plan = nullptr;
+    if (!vol_shape || !vol_spacing || !vol_offset || !vecs)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create_vec: null geometry array");
+    CMI_TRY(check_plan_shape(vol_prec, det_prec, vol_shape, vol_spacing, det_count, nangles,
+                             "plan2d_create_vec"));
+    for (uint64_t a = 0; a < nangles; ++a) {
+        const double *v = vecs + 6 * a;
+        for (int i = 0; i < 6; ++i)
+            if (!std::isfinite(v[i]))
+                return fail(CMI_ERR_INVALID_ARGUMENT,
+                            "plan2d_create_vec: non-finite view %llu", (unsigned long long)a);
+        const fan2d::BpView bv = fan2d::bp_view(v, (int64_t)det_count);
+        if (!(v[4] * v[4] + v[5] * v[5] > 0.0) || !(bv.L > 0.0))
+            return fail(CMI_ERR_INVALID_ARGUMENT,
+                        "plan2d_create_vec: view %llu has a zero cell step or its source on "
+                        "the detector line", (unsigned long long)a);
+    }
+    auto p = std::make_unique<cmi_plan2d>();
+    set_plan_shape(*p, device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec, det_count,
+                   nangles);
+    p->g.du = std::sqrt(vecs[4] * vecs[4] + vecs[5] * vecs[5]); // view 0's, for get_info
+    const int64_t A = p->A;
+    std::vector<AngleGeom> geo(A);
+    std::vector<float4> bpv(3 * A);
+    for (int64_t a = 0; a < A; ++a) {
+        geo[a] = fan2d::plan_angle_vec(p->g, vecs + 6 * a);
+        const fan2d::BpView bv = fan2d::bp_view(vecs + 6 * a, p->g.D);
+        bpv[3 * a] = make_float4((float)bv.n[0], (float)bv.n[1], (float)bv.n0, (float)bv.L);
+        bpv[3 * a + 1] = make_float4((float)bv.e[0], (float)bv.e[1], (float)bv.e0, (float)bv.c0);
+        bpv[3 * a + 2] = make_float4((float)bv.inv_du, 0.0f, 0.0f, 0.0f);
+    }
+    return finish_plan2d(std::move(p), geo, {}, bpv, plan);
 }
 
 CMI_API cmi_status cmi_fan2d_bp_table(uint64_t nangles, const double *angles, const double *DSO,
@@ -867,6 +982,7 @@ CMI_API cmi_status cmi_plan2d_destroy(cmi_plan2d *plan) {
         (void)hipDeviceSynchronize();
         (void)hipFree(plan->d_geo);
         (void)hipFree(plan->d_meta);
+        (void)hipFree(plan->d_bpv);
         (void)hipFree(plan->d_ageo);
         (void)hipFree(plan->d_ev);
         (void)hipFree(plan->d_ns);
@@ -973,11 +1089,18 @@ CMI_API cmi_status cmi_backward_2d_weighted(const cmi_plan2d *plan, void *volume
         hipLaunchKernelGGL((k_stage_sino2d<SS>), pg, pb, 0, s, (const RS *)sinogram,
                            (RS *)scratch, D, A);
         dim3 b(64, 4), g((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)batch);
-        auto kern = weighting == CMI_BP2D_FBP ? k_bp2d<SV, SS, 2> : k_bp2d<SV, SS, 1>;
-        hipLaunchKernelGGL(kern, g, b, 0, s, (typename SV::raw *)volume, (const RS *)scratch,
-                           plan->d_meta, nx, ny, D, A, (float)plan->g.sx, (float)plan->g.sy,
-                           (float)plan->g.ox, (float)plan->g.oy, (float)(1.0 / plan->g.du),
-                           (float)D / 2.0f);
+        if (plan->d_bpv) {
+            auto kern = weighting == CMI_BP2D_FBP ? k_bp2d_vec<SV, SS, 2> : k_bp2d_vec<SV, SS, 1>;
+            hipLaunchKernelGGL(kern, g, b, 0, s, (typename SV::raw *)volume, (const RS *)scratch,
+                               plan->d_bpv, nx, ny, D, A, (float)plan->g.sx, (float)plan->g.sy,
+                               (float)plan->g.ox, (float)plan->g.oy);
+        } else {
+            auto kern = weighting == CMI_BP2D_FBP ? k_bp2d<SV, SS, 2> : k_bp2d<SV, SS, 1>;
+            hipLaunchKernelGGL(kern, g, b, 0, s, (typename SV::raw *)volume, (const RS *)scratch,
+                               plan->d_meta, nx, ny, D, A, (float)plan->g.sx, (float)plan->g.sy,
+                               (float)plan->g.ox, (float)plan->g.oy, (float)(1.0 / plan->g.du),
+                               (float)D / 2.0f);
+        }
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });

@@ -65,6 +65,61 @@ inline AngleGeom plan_angle(const Scalars &g, double angle, double dso, double d
     return r;
 }
 
+// ASTRA-like view (no counterpart in the reference, whose from_vec keeps only DSO / DSD /
+// angle, python/standalone/src/curadon/geometry.py:84-133): source s, detector centre c and cell
+// step u in world coordinates, v = (s.x, s.y, c.x, c.y, u.x, u.y). Cell 0's centre is
+// c - (D - 1)/2 u; the texel-frame quantities follow as in plan_angle.
+inline AngleGeom plan_angle_vec(const Scalars &g, const double v[6]) {
+    AngleGeom r{};
+    const double ext[2] = {(double)g.nx * g.sx, (double)g.ny * g.sy};
+    const double sp[2] = {g.sx, g.sy}, off[2] = {g.ox, g.oy};
+    const double h = ((double)g.D - 1.0) / 2.0;
+    for (int d = 0; d < 2; ++d) {
+        const double r0 = v[2 + d] - h * v[4 + d] - v[d]; // source to cell 0
+        const double r1 = r0 + v[4 + d];                   // source to cell 1
+        const double o = r0 / sp[d];
+        r.org[d] = o;
+        r.del[d] = r1 / sp[d] - o;
+        r.src[d] = (v[d] - off[d] + 0.5 * ext[d]) / sp[d];
+    }
+    return r;
+}
+
+// Backprojection constants of a view (the reference's formula, backward_2d.hpp:58-99, for an
+// arbitrary view): with n the unit normal of the detector line pointing from the source towards
+// it and e = u / |u|, a pixel at world x has depth n . x + n0 (n0 = -n . s), magnification
+// w = L / depth (L = n . (c - s)), detector cell coordinate w (e . x + e0) + c0 with
+// e0 = -e . s / |u| and c0 = e . (s - c) / |u| + D/2 (the e terms scaled by 1 / |u|), and
+// weight w^p / |u|. For a centred circular view these are the reference's quantities:
+// n . x + n0 = xi + DSO, L = DSD, e . x / |u| = yi, c0 = D/2.
+struct BpView {
+    double n[2], n0, L;    // depth = n . x + n0; w = L / depth
+    double e[2], e0, c0;   // cell = w (e . x + e0) + c0
+    double inv_du;         // 1 / |u|
+};
+
+inline BpView bp_view(const double v[6], int64_t D) {
+    BpView b{};
+    const double ul = std::sqrt(v[4] * v[4] + v[5] * v[5]);
+    const double e0 = v[4] / ul, e1 = v[5] / ul;
+    double n0 = -e1, n1 = e0;
+    const double cs0 = v[2] - v[0], cs1 = v[3] - v[1]; // source to detector centre
+    if (n0 * cs0 + n1 * cs1 < 0.0) {
+        n0 = -n0;
+        n1 = -n1;
+    }
+    b.n[0] = n0;
+    b.n[1] = n1;
+    b.n0 = -(n0 * v[0] + n1 * v[1]);
+    b.L = n0 * cs0 + n1 * cs1;
+    b.e[0] = e0 / ul;
+    b.e[1] = e1 / ul;
+    b.e0 = -(e0 * v[0] + e1 * v[1]) / ul;
+    b.c0 = -(e0 * cs0 + e1 * cs1) / ul + (double)D / 2.0;
+    b.inv_du = 1.0 / ul;
+    return b;
+}
+
 // One ray after clipping, in the form the kernels consume.
 struct Ray {
     double e[2];    // entry point, texel coordinates

@@ -52,24 +52,37 @@ class FanGeometry:
 
     def to_vec(self):
         """ASTRA-like vectors ``[src.x, src.y, det.x, det.y, u.x, u.y]`` per angle
-        (geometry.py:54-79): source at (0, DSO), detector centre at (0, -(DSD - DSO)) and unit
-        detector axis (1, 0), rotated by ``[[c, s], [-s, c]]``."""
+        (geometry.py:54-79): source at (det_offset, DSO), detector centre at (0, -(DSD - DSO))
+        and cell step (det_spacing, 0), rotated by ``[[c, s], [-s, c]]``. With det_offset = 0
+        and det_spacing = 1 these are the reference's vectors; otherwise the source shift that
+        the reference's forward applies (plan_2d.hpp:76-80) and the cell spacing are kept, so
+        that ``from_vec(to_vec())`` describes the same projector. A geometry made by
+        ``from_vec`` returns its views."""
+        if getattr(self, "vecs", None) is not None:
+            return self.vecs.tolist()
         out = []
         for alpha, dsd, dso in zip(self.angles, self.DSD, self.DSO):
             c, s = np.cos(alpha), np.sin(alpha)
             R = np.array([[c, s], [-s, c]])
-            src = R @ np.array([0.0, dso])
+            src = R @ np.array([self.det_offset, dso])
             det = R @ np.array([0.0, -(dsd - dso)])
-            u = R @ np.array([1.0, 0.0])
+            u = R @ np.array([self.det_spacing, 0.0])
             out.append([*src.tolist(), *det.tolist(), *u.tolist()])
         return out
 
     @staticmethod
     def from_vec(vol_shape: Tuple[int, int], det_shape: int, vecs: List[List[float]], **kwargs):
-        """Build a FanGeometry from ASTRA-like vectors (geometry.py:84-133): DSD = |src - det|,
-        DSO = |src|, angle = atan2(src2det) mod 2 pi, detector spacing = |u| (equal for all
-        views, else AttributeError)."""
+        """Build a FanGeometry from ASTRA-like vectors ``[src.x, src.y, det.x, det.y, u.x,
+        u.y]`` per view (world coordinates; ``det`` the detector centre, ``u`` the step from
+        cell 0's centre to cell 1's). The geometry keeps the views (``vecs``) and its plan
+        projects through them exactly (cmi_plan2d_create_vec): a moved source, a shifted or
+        tilted detector and a per-view cell spacing are all honoured. The reference's version
+        (geometry.py:84-133) keeps only the summary below and rejects unequal spacings; the
+        summary is still set, for display and for code that reads it: DSD = |src - det|,
+        DSO = |src|, angle = atan2(src - det) mod 2 pi, det_spacing = |u| of the first view."""
         vecs = np.asarray(vecs, dtype=np.float64)
+        if vecs.ndim != 2 or vecs.shape[1] != 6:
+            raise AttributeError(f"vecs must have shape (nangles, 6), got {vecs.shape}")
         det_spacing = float(np.linalg.norm(vecs[0][-2:]))
         dsd, dso, angles = [], [], []
         for v in vecs:
@@ -79,10 +92,12 @@ class FanGeometry:
             dsd.append(float(np.linalg.norm(s2d)))
             dso.append(float(np.linalg.norm(src)))
             angles.append(float(np.arctan2(s2d[0], s2d[1]) % (2 * np.pi)))
-            if not np.isclose(det_spacing, np.linalg.norm([ux, uy])):
-                raise AttributeError("Detector must have equal spacing for all projections")
-        return FanGeometry(DSD=dsd, DSO=dso, angles=angles, vol_shape=vol_shape,
-                           det_shape=det_shape, det_spacing=det_spacing, **kwargs)
+            if not np.linalg.norm([ux, uy]) > 0.0:
+                raise AttributeError("Every view needs a non-zero cell step u")
+        g = FanGeometry(DSD=dsd, DSO=dso, angles=angles, vol_shape=vol_shape,
+                        det_shape=det_shape, det_spacing=det_spacing, **kwargs)
+        g.vecs = np.array(vecs)
+        return g
 
     def __init__(self, DSD: Union[float, List[float]], DSO: Union[float, List[float]],
                  angles: List[float], vol_shape: Tuple[int, int], det_shape: int,
@@ -162,14 +177,21 @@ class FanGeometry:
         self.device = int(device)
         self.vol_prec = vol_prec
         self.sino_prec = sino_prec
+        self.vecs = None  # ASTRA-like views when built by from_vec
         self._plan = None
 
     @property
     def plan(self):
-        """The native plan (``curadon_amd._lib.plan_2d``), built on first access. COR is 0, as
+        """The native plan (``curadon_amd._lib.plan_2d``), built on first access: from the
+        views for a geometry made by ``from_vec``, else from angles / DSO / DSD with COR 0, as
         the reference passes it (geometry.py:226-228)."""
         if self._plan is None:
             from . import _lib
+            if self.vecs is not None:
+                self._plan = _lib.plan_2d.from_views(
+                    self.device, self.vol_prec, self.vol_shape, self.vol_spacing,
+                    self.vol_offset, self.sino_prec, self.det_shape, self.vecs)
+                return self._plan
             self._plan = _lib.plan_2d(self.device, self.vol_prec, self.vol_shape,
                                       self.vol_spacing, self.vol_offset, self.sino_prec,
                                       self.det_shape, self.det_spacing, self.det_offset,

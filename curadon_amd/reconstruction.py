@@ -116,6 +116,10 @@ def fbp_filter(sinogram: torch.Tensor, geom, window="ram-lak", frequency_scaling
     dev = sinogram.device
     dt = torch.float64 if sinogram.dtype == torch.float64 else torch.float32
     if isinstance(geom, FanGeometry):
+        if geom.vecs is not None:
+            raise NotImplementedError(
+                "fbp: geometries built from ASTRA-like views (from_vec) have no fan-beam FBP "
+                "weighting; use backward / adjoint, or a circular FanGeometry")
         H = ramp_response(geom.det_shape, window, frequency_scaling, dev)
         cw = cosine_weights_fan(geom, dev).to(dt)
         scale = (math.pi / geom.nangles) * torch.as_tensor(

@@ -95,6 +95,20 @@ CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vo
                                      const double *angles, uint64_t nangles, double det_pitch,
                                      double COR, cmi_plan2d **plan);
 
+/*
+ * A 2D plan from ASTRA-like views, one per projection: vecs[6 a .. 6 a + 5] = (source x, y,
+ * detector centre x, y, u x, y) in world coordinates, u the step from cell 0's centre to cell
+ * 1's. Every view may move the source, shift or tilt the detector and change the cell
+ * spacing. No counterpart in the reference: its FanGeometry.from_vec keeps only DSO, DSD and
+ * the angle of each view and rejects unequal spacings (python/standalone/src/curadon/
+ * geometry.py:84-133). Forward, backprojection (per-view magnification, weight divided by the
+ * view's |u|) and adjoint all follow the views; the serving tables (cmi_fan2d_bp_table) do not.
+ */
+CMI_API cmi_status cmi_plan2d_create_vec(int device, int vol_prec, const uint64_t vol_shape[2],
+                                         const double vol_spacing[2], const double vol_offset[2],
+                                         int det_prec, uint64_t det_count, const double *vecs,
+                                         uint64_t nangles, cmi_plan2d **plan);
+
 CMI_API cmi_status cmi_plan2d_destroy(cmi_plan2d *plan);
 
 /* Launch-shape tunables, the analogue of the r/w forward_block_x / forward_block_y

@@ -129,7 +129,32 @@ static inline void rot2(const double v[2], double c, double s, const double t[2]
     out[1] = -v[0] * s + v[1] * c - t[1];
 }
 
+/* Vector form (no counterpart in the reference, whose from_vec keeps only DSO / DSD / angle,
+ * python/standalone/src/curadon/geometry.py:84-133): the view gives the source s, the detector
+ * centre c and the cell step u in world coordinates, so cell 0's centre is c - (D - 1)/2 u, and
+ * the texel-frame quantities follow exactly as in plan_2d.hpp:28-48, 226-235. */
+static void plan2d_vec(const orc_fan2d *g, double *src, double *origin, double *delta) {
+    const double ext[2] = {(double)g->nx * g->sx, (double)g->ny * g->sy};
+    const double sp[2] = {g->sx, g->sy}, off[2] = {g->ox, g->oy};
+    const double h = ((double)g->det_count - 1.0) / 2.0;
+    for (int64_t a = 0; a < g->nangles; ++a) {
+        const double *v = g->vecs + 6 * a;
+        for (int d = 0; d < 2; ++d) {
+            const double r0 = v[2 + d] - h * v[4 + d] - v[d]; /* source to cell 0 */
+            const double r1 = r0 + v[4 + d];                   /* source to cell 1 */
+            const double o = r0 / sp[d];
+            origin[2 * a + d] = o;
+            delta[2 * a + d] = r1 / sp[d] - o;
+            src[2 * a + d] = (v[d] - off[d] + 0.5 * ext[d]) / sp[d];
+        }
+    }
+}
+
 void orc_plan2d(const orc_fan2d *g, double *src, double *origin, double *delta) {
+    if (g->vecs) {
+        plan2d_vec(g, src, origin, delta);
+        return;
+    }
     const double det_extent = (double)g->det_count * g->du;        /* plan_2d.hpp:116 */
     const double ext[2] = {(double)g->nx * g->sx, (double)g->ny * g->sy}; /* :112 */
     const double sp[2] = {g->sx, g->sy}, off[2] = {g->ox, g->oy};
@@ -240,8 +265,51 @@ void orc_adj2d(const orc_fan2d *g, const double *sino, double *img, int64_t a0, 
 
 /* ------------------------------------------------------ 2D backprojection (A.3) ------- */
 
+/* Vector form of the voxel-driven backprojection: the reference's formula (backward_2d.hpp:58-99)
+ * written for an arbitrary view. With n the unit normal of the detector line pointing from the
+ * source towards it and e = u / |u|: depth = n . (x - s), magnification w = L / depth with
+ * L = n . (c - s), detector cell coordinate w (e . (x - s)) / |u| + e . (s - c) / |u| + D/2
+ * (the reference's w yi + D/2 when s, c, u come from a centred circular view), and the view's
+ * weight w^wpow / |u| (the reference's w / du). */
+static void bp2d_vec(const orc_fan2d *g, const double *sino, double *img, int64_t y0,
+                     int64_t y1, int wpow) {
+    const int64_t A = g->nangles, D = g->det_count;
+    const double half_d = (double)D / 2.0;
+#pragma omp parallel for schedule(static)
+    for (int64_t iy = y0; iy < y1; ++iy) {
+        for (int64_t ix = 0; ix < g->nx; ++ix) {
+            /* the pixel's world position as the reference's backprojector writes it (:58-59) */
+            const double x = ((double)ix - (double)g->nx / 2.0) * g->sx + g->ox + 0.5;
+            const double y = ((double)iy - (double)g->ny / 2.0) * g->sy + g->oy + 0.5;
+            double acc = 0.0;
+            for (int64_t a = 0; a < A; ++a) {
+                const double *v = g->vecs + 6 * a;
+                const double ul = sqrt(v[4] * v[4] + v[5] * v[5]);
+                const double e0 = v[4] / ul, e1 = v[5] / ul;
+                double n0 = -e1, n1 = e0;
+                const double cs0 = v[2] - v[0], cs1 = v[3] - v[1]; /* source to centre */
+                if (n0 * cs0 + n1 * cs1 < 0.0) {
+                    n0 = -n0;
+                    n1 = -n1;
+                }
+                const double L = n0 * cs0 + n1 * cs1;
+                const double px = x - v[0], py = y - v[1];
+                const double w = L / (n0 * px + n1 * py);
+                const double u = w * (e0 * px + e1 * py) / ul - (e0 * cs0 + e1 * cs1) / ul + half_d;
+                const double wt = wpow == 2 ? w * w : w;
+                acc += orc_lerp1(sino + a * D, D, u) * wt / ul;
+            }
+            img[iy * g->nx + ix] += acc;
+        }
+    }
+}
+
 void orc_bp2d_w(const orc_fan2d *g, const double *sino, double *img, int64_t y0, int64_t y1,
                 int wpow) {
+    if (g->vecs) {
+        bp2d_vec(g, sino, img, y0, y1, wpow);
+        return;
+    }
     const int64_t A = g->nangles, D = g->det_count;
     double *sn = (double *)malloc((size_t)A * sizeof(double));
     double *cs = (double *)malloc((size_t)A * sizeof(double));

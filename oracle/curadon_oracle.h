@@ -42,6 +42,9 @@ typedef struct {
     const double *angles; /* [A] radians */
     const double *dso;    /* [A] source to isocentre */
     const double *dsd;    /* [A] source to detector */
+    const double *vecs;   /* NULL, or [A][6] ASTRA-like views (source x, y, detector centre x, y,
+                             u x, y; world coordinates, u = cell 0 centre to cell 1 centre),
+                             which then replace angles / DSO / DSD / du / det_offset */
 } orc_fan2d;
 
 /* Per-angle source position, vector from the source to cell 0 and the cell-to-cell step,

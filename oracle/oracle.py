@@ -28,7 +28,7 @@ class Fan2D(C.Structure):
         ("nx", _i64), ("ny", _i64), ("sx", C.c_double), ("sy", C.c_double),
         ("ox", C.c_double), ("oy", C.c_double), ("det_count", _i64), ("du", C.c_double),
         ("det_offset", C.c_double), ("nangles", _i64), ("angles", _dp), ("dso", _dp),
-        ("dsd", _dp),
+        ("dsd", _dp), ("vecs", _dp),
     ]
 
 
@@ -111,14 +111,17 @@ class Fan:
 
     ``geom`` is any record with the attributes of ``FanGeometry``: ``vol_shape`` (y, x),
     ``vol_spacing``/``vol_offset`` (y, x), ``det_shape``, ``det_spacing``, ``det_offset``,
-    ``angles``, ``DSO``, ``DSD`` (per angle).
+    ``angles``, ``DSO``, ``DSD`` (per angle); with a ``vecs`` attribute ([A][6] ASTRA-like
+    views) that is not None, the views define the geometry instead.
     """
 
     def __init__(self, geom):
         vs = np.asarray(geom.vol_shape)
         sp = np.asarray(geom.vol_spacing, dtype=np.float64)
         of = np.asarray(geom.vol_offset, dtype=np.float64)
-        self.angles = _c64(geom.angles)
+        vecs = getattr(geom, "vecs", None)
+        self.vecs = None if vecs is None else _c64(np.asarray(vecs, dtype=np.float64).reshape(-1, 6))
+        self.angles = _c64(geom.angles) if self.vecs is None else _c64(np.zeros(len(self.vecs)))
         A = self.angles.size
         self.dso = _c64(np.broadcast_to(np.asarray(geom.DSO, dtype=np.float64), (A,)))
         self.dsd = _c64(np.broadcast_to(np.asarray(geom.DSD, dtype=np.float64), (A,)))
@@ -127,6 +130,7 @@ class Fan:
             ox=float(of[1]), oy=float(of[0]), det_count=int(geom.det_shape),
             du=float(geom.det_spacing), det_offset=float(geom.det_offset), nangles=A,
             angles=_d(self.angles), dso=_d(self.dso), dsd=_d(self.dsd),
+            vecs=None if self.vecs is None else _d(self.vecs),
         )
         self.nx, self.ny, self.D, self.A = int(vs[1]), int(vs[0]), int(geom.det_shape), A
 

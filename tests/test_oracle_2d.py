@@ -3,6 +3,8 @@ closed forms of axis-aligned rays, a library interpolation routine, the referenc
 ASTRA-like vector form of the geometry (geometry.py:54-79), analytic ellipse line integrals,
 an explicitly assembled system matrix for the adjoint, and golden fixtures.
 """
+from types import SimpleNamespace
+
 import numpy as np
 import pytest
 from scipy import ndimage
@@ -52,7 +54,8 @@ def test_bilerp_trilerp_match_scipy_grid_constant(oracle):
 @pytest.mark.parametrize("spacing", [(1.0, 1.0), (0.5, 2.0)])
 def test_plan_matches_vector_form(oracle, spacing):
     """Source, detector centre and cell step agree with the reference's to_vec form
-    (source R(0, DSO), detector centre R(0, -DOD), axis R(1, 0), R = [[c, s], [-s, c]])."""
+    (source R(0, DSO), detector centre R(0, -DOD), axis R(1, 0), R = [[c, s], [-s, c]]; this
+    package's to_vec scales the axis by det_spacing, so that it is the cell step)."""
     g = fan(10, 13, 17, 30.0, 45.0, du=1.5, spacing=spacing, offset=(0.7, -1.3))
     g.angles = rng(1).uniform(0, 2 * np.pi, 17)
     fg = FanGeometry(DSD=g.DSD, DSO=g.DSO, angles=g.angles, vol_shape=g.vol_shape,
@@ -68,7 +71,8 @@ def test_plan_matches_vector_form(oracle, spacing):
     np.testing.assert_allclose(src_w, vec[:, 0:2], atol=1e-12)
     centre = src_w + (org + (g.det_shape - 1) / 2 * dl) * s
     np.testing.assert_allclose(centre, vec[:, 2:4], atol=1e-11)
-    np.testing.assert_allclose(dl * s, g.det_spacing * vec[:, 4:6], atol=1e-12)
+    np.testing.assert_allclose(dl * s, vec[:, 4:6], atol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(vec[:, 4:6], axis=1), g.det_spacing, rtol=1e-14)
     # invariants: |src| = DSO, |centre - src| = DSD, |step| = du
     np.testing.assert_allclose(np.linalg.norm(src_w, axis=1), g.DSO, rtol=1e-14)
     np.testing.assert_allclose(np.linalg.norm(centre - src_w, axis=1), g.DSD, rtol=1e-14)
@@ -276,3 +280,120 @@ def test_golden_fixture_2d(oracle):
     np.testing.assert_allclose(oracle.fwd2d(g, d["image"]), d["sino"], rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(oracle.bp2d(g, d["sino_in"]), d["bp"], rtol=1e-13, atol=1e-13)
     np.testing.assert_allclose(oracle.adj2d(g, d["sino_in"]), d["adj"], rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------- vector geometry ----
+
+def _views_from_circular(g):
+    """ASTRA-like views of a circular record, built independently of the oracle: source at
+    R(det_offset, DSO), detector centre at R(0, DSO - DSD), cell step R(du, 0), with R the
+    reference's clockwise rotation (rotation.hpp:60) and the det_offset source shift of
+    plan_2d.hpp:76-80."""
+    out = []
+    for th, dso, dsd in zip(g.angles, g.DSO, g.DSD):
+        c, s = np.cos(th), np.sin(th)
+        R = np.array([[c, s], [-s, c]])
+        out.append(np.concatenate([R @ [g.det_offset, dso], R @ [0.0, dso - dsd],
+                                   R @ [g.det_spacing, 0.0]]))
+    return np.array(out)
+
+
+def _with_views(g, views):
+    r = SimpleNamespace(**vars(g))
+    r.vecs = np.asarray(views, float)
+    return r
+
+
+def _general_views(seed, A, D, dso=60.0, dsd=100.0):
+    """Views with a tilted, shifted detector and a per-view cell spacing."""
+    r = rng(seed)
+    out = []
+    for th in r.uniform(0, 2 * np.pi, A):
+        c, s = np.cos(th), np.sin(th)
+        R = np.array([[c, s], [-s, c]])
+        tilt = r.uniform(-0.35, 0.35)
+        du = r.uniform(0.7, 1.4)
+        src = R @ [r.uniform(-3, 3), dso]
+        det = R @ [r.uniform(-4, 4), dso - dsd]
+        u = R @ (du * np.array([np.cos(tilt), np.sin(tilt)]))
+        out.append(np.concatenate([src, det, u]))
+    return np.array(out)
+
+
+def test_vector_views_reproduce_circular_plan_and_operators(oracle):
+    g = fan(20, 31, 12, 50.0, 85.0, du=1.3, det_offset=0.7, spacing=(1.1, 0.9),
+            offset=(0.3, -0.5))
+    g.DSO = np.linspace(48.0, 55.0, 12)
+    g.DSD = g.DSO + np.linspace(30.0, 40.0, 12)
+    v = _with_views(g, _views_from_circular(g))
+    for x, y in zip(oracle.Fan(g).plan(), oracle.Fan(v).plan()):
+        np.testing.assert_allclose(y, x, rtol=1e-12, atol=1e-12)
+    img = rng(20).random((20, 20))
+    np.testing.assert_allclose(oracle.fwd2d(v, img), oracle.fwd2d(g, img), rtol=1e-11,
+                               atol=1e-11)
+    y = rng(21).random((12, 31))
+    np.testing.assert_allclose(oracle.adj2d(v, y), oracle.adj2d(g, y), rtol=1e-11, atol=1e-11)
+    # the circular backprojector ignores det_offset (backward_2d.hpp:58-99), the views honour it
+    g.det_offset = 0.0
+    v = _with_views(g, _views_from_circular(g))
+    for wpow in (1, 2):
+        np.testing.assert_allclose(oracle.bp2d(v, y, wpow=wpow), oracle.bp2d(g, y, wpow=wpow),
+                                   rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("wpow", [1, 2])
+def test_vector_backprojection_matches_line_intersection(oracle, wpow):
+    """Each pixel (the reference's position, backward_2d.hpp:58-59) is joined to the source and
+    the line cut with the detector line d + q u; the sample at cell q + (D - 1)/2 is weighted by
+    the magnification t (pixel at s + (x - s), detector point at s + t (x - s)) to the power
+    wpow and divided by the view's cell spacing |u|."""
+    D, A = 23, 7
+    g = fan(14, D, A, 60.0, 100.0, spacing=(1.2, 0.8), offset=(0.5, -0.4), ny=10)
+    views = _general_views(22, A, D)
+    v = _with_views(g, views)
+    sino = rng(23).random((A, D))
+    ours = oracle.bp2d(v, sino, wpow=wpow)
+    xs = (np.arange(14) - 7) * 0.8 - 0.4 + 0.5
+    ys = (np.arange(10) - 5) * 1.2 + 0.5 + 0.5
+    X, Y = np.meshgrid(xs, ys)
+    expect = np.zeros((10, 14))
+    for a in range(A):
+        s, d, u = views[a, 0:2], views[a, 2:4], views[a, 4:6]
+        wx, wy = X - s[0], Y - s[1]
+        rhs = d - s
+        det = wx * (-u[1]) + wy * u[0]
+        t = (rhs[0] * (-u[1]) + rhs[1] * u[0]) / det
+        q = (wx * rhs[1] - wy * rhs[0]) / det
+        row = np.concatenate([[0.0], sino[a], [0.0]])
+        val = np.interp(q + (D - 1) / 2, np.arange(-1, D + 1), row, left=0.0, right=0.0)
+        expect += t ** wpow * val / np.linalg.norm(u)
+    np.testing.assert_allclose(ours, expect, rtol=1e-11, atol=1e-11)
+
+
+def test_vector_forward_matches_analytic_ellipses(oracle):
+    """Rays from the source through every cell centre d + (k - (D - 1)/2) u of general views
+    against exact line integrals of the (halved) ellipse phantom, as for circular views."""
+    n, D, A = 192, 301, 16
+    g = fan(n, D, A, 400.0, 650.0)
+    views = _general_views(24, A, D, dso=400.0, dsd=650.0)
+    v = _with_views(g, views)
+    p = phantom.ellipse_params(n, seed=6).copy()
+    p[:, :4] *= 0.5
+    sino = oracle.fwd2d(v, phantom.ellipses_2d(n, params=p))
+    k = np.arange(D) - (D - 1) / 2
+    src = views[:, None, 0:2]
+    cell = views[:, None, 2:4] + k[None, :, None] * views[:, None, 4:6]
+    ana = phantom.ellipse_line_integrals(p, np.broadcast_to(src, cell.shape), cell - src)
+    rel = np.linalg.norm(sino - ana) / np.linalg.norm(ana)
+    assert rel < 0.02, rel
+    assert sino.sum() / ana.sum() == pytest.approx(1.0, abs=3e-3)
+
+
+def test_vector_dot_product(oracle):
+    g = fan(24, 35, 10, 60.0, 100.0)
+    v = _with_views(g, _general_views(25, 10, 35))
+    x = rng(26).random((24, 24))
+    y = rng(27).random((10, 35))
+    lhs = np.vdot(oracle.fwd2d(v, x), y)
+    rhs = np.vdot(x, oracle.adj2d(v, y))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
