@@ -29,18 +29,33 @@ __device__ __forceinline__ float bp2d_pixel(rsrc_t R, const float4 *__restrict__
                                            float half_d) {
     const int Dp = D + 2 * kPadDet;
     const float hi = (float)D;
+    const float c0 = half_d - 0.5f; // D/2 and the texel-centre shift of the lerp, exact
     float acc = 0.0f;
     int row = kPadDet;
-#pragma unroll 4
-    for (int a = 0; a < A; ++a, row += Dp) {
-        const float4 m = meta[a]; // wave-uniform: scalar loads
-        const float xi = m.x * x - m.y * y;
-        const float eta = m.y * px + m.x * py;
-        const float w = m.z * __builtin_amdgcn_rcpf(xi + m.w);
-        const float u = __builtin_amdgcn_fmed3f(fmaf(w, eta, half_d) - 0.5f, -2.0f, hi);
-        const float al = __builtin_amdgcn_fractf(u);
-        const Pair p = buf_pair<SS>(R, row + (int)floorf(u), 0);
-        acc = fmaf(fmaf(al, p.hi - p.lo, p.lo), WPOW == 2 ? w * w : w, acc);
+    // one angle's sample: weight and lerp fraction, with the tap pair loaded
+    auto sample = [&](const float4 m, int r, float &wt, float &al, Pair &p) {
+        const float depth = fmaf(m.x, x, fmaf(-m.y, y, m.w)); // xi + DSO
+        const float eta = fmaf(m.y, px, m.x * py);
+        const float w = m.z * __builtin_amdgcn_rcpf(depth);
+        const float u = __builtin_amdgcn_fmed3f(fmaf(w, eta, c0), -2.0f, hi);
+        al = __builtin_amdgcn_fractf(u);
+        wt = WPOW == 2 ? w * w : w;
+        p = buf_pair<SS>(R, r + floor_i32(u), 0);
+    };
+    int a = 0;
+    for (; a + 3 < A; a += 4, row += 4 * Dp) { // four loads in flight
+        float wt[4], al[4];
+        Pair p[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sample(meta[a + k], row + k * Dp, wt[k], al[k], p[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc = fmaf(fmaf(al[k], p[k].hi - p[k].lo, p[k].lo), wt[k], acc);
+    }
+    for (; a < A; ++a, row += Dp) {
+        float wt, al;
+        Pair p;
+        sample(meta[a], row, wt, al, p);
+        acc = fmaf(fmaf(al, p.hi - p.lo, p.lo), wt, acc);
     }
     return acc;
 }

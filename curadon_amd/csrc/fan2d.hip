@@ -5,7 +5,7 @@
 //   plan tables  geo  [A] AngleGeom (FP64)       source / cell-0 vector / cell step
 //                meta [A] float4                 (-sin, cos, DSD, DSO) for the backprojector
 //                                                (plans from angles / DSO / DSD)
-//                bpv  [A][3] float4              (n.x, n.y, n0, L), (e.x, e.y, e0, c0),
+//                bpv  [A][3] float4              (n.x, n.y, n0, L), (e.x, e.y, e0, c0 - .5),
 //                                                (1/|u|, 0, 0, 0): the backprojector's view
 //                                                constants (plans from ASTRA-like views,
 //                                                geometry2d.hpp bp_view)
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp2d_vec(typename SV::raw *__rest
         const float idu = bpv[3 * a + 2].x;
         const float w = m0.w * __builtin_amdgcn_rcpf(fmaf(m0.x, x, fmaf(m0.y, y, m0.z)));
         const float eta = fmaf(m1.x, x, fmaf(m1.y, y, m1.z));
-        const float u = __builtin_amdgcn_fmed3f(fmaf(w, eta, m1.w) - 0.5f, -2.0f, hi);
+        const float u = __builtin_amdgcn_fmed3f(fmaf(w, eta, m1.w), -2.0f, hi); // c0 - 0.5
         const float al = __builtin_amdgcn_fractf(u);
         const Pair p = buf_pair<SS>(R, row + floor_i32(u), 0);
         acc = fmaf(fmaf(al, p.hi - p.lo, p.lo), (WPOW == 2 ? w * w : w) * idu, acc);
@@ -951,7 +951,9 @@ CMI_API cmi_status cmi_plan2d_create_vec(int device, int vol_prec, const uint64_
         geo[a] = fan2d::plan_angle_vec(p->g, vecs + 6 * a);
         const fan2d::BpView bv = fan2d::bp_view(vecs + 6 * a, p->g.D);
         bpv[3 * a] = make_float4((float)bv.n[0], (float)bv.n[1], (float)bv.n0, (float)bv.L);
-        bpv[3 * a + 1] = make_float4((float)bv.e[0], (float)bv.e[1], (float)bv.e0, (float)bv.c0);
+        // the cell offset carries the lerp's texel-centre shift (-0.5)
+        bpv[3 * a + 1] =
+            make_float4((float)bv.e[0], (float)bv.e[1], (float)bv.e0, (float)(bv.c0 - 0.5));
         bpv[3 * a + 2] = make_float4((float)bv.inv_du, 0.0f, 0.0f, 0.0f);
     }
     return finish_plan2d(std::move(p), geo, {}, bpv, plan);
