@@ -298,67 +298,97 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
     const int soff1 = rstride * (int)sizeof(typename SI::raw);
     float acc = 0.0f;
     int bi = 0;
-    for (int t0 = floor_div(tr0, kBand) * kBand; t0 <= tr1; t0 += kBand, ++bi) {
-        const int b_lo = max(tr0, t0), b_hi = min(tr1, t0 + kBand - 1);
-        const int ea = ray_col(e0, b_lo), eb = ray_col(e0, b_hi);
-        const int ec = ray_col(e1, b_lo), ed = ray_col(e1, b_hi);
-        // lower taps within [c_lo, c_hi]: one column of margin for rounding (the extreme rays
-        // are the same for every lane: the bounds live in scalar registers)
-        const int c_lo = __builtin_amdgcn_readfirstlane(min(min(ea, eb), min(ec, ed)) - 1);
-        const int c_hi = __builtin_amdgcn_readfirstlane(max(max(ea, eb), max(ec, ed)) + 1);
-        const int W = c_hi + 2 - c_lo; // columns [c_lo, c_hi + 1] hold the taps
-        // whether every lane has a sample on every row [b_lo, b_hi]
-        const bool full = all_in && ta_max <= b_lo && tz_min >= b_hi;
-        BandSums sums;
-        float jf = (float)(s * (b_lo - r0));
-        bool use_lds = false;
-        if (W <= kPitch - 4) {
-            // stage rows [t0, t0 + kBandRows) x columns [c_lo, c_lo + 4 Q) as FP32, with
-            // 4 Q <= kPitch - 4 (the zero cells stay untouched). Thread g Q + q (g < G groups
-            // of Q threads) takes quad q of rows g, g + G, ...; a row index past the band's last
-            // row is clamped to it (the same values to the same cells). Quotients come from a
-            // float reciprocal: exact, as every numerator is below 2^8 and every true quotient
-            // is at least 1/(2 Q) > 2^-8 away from the next integer.
+    // A band's staging plan: the columns it needs (bounded by the extreme rays, one column of
+    // margin each side for rounding; the bounds are the same for every lane and live in scalar
+    // registers) and this thread's share of the 16-byte row loads: rows [t0, t0 + kBandRows) x
+    // columns [c_lo, c_lo + 4 Q) as FP32, with 4 Q <= kPitch - 4 (the zero cells stay
+    // untouched). Thread g Q + q (g < G groups of Q threads) takes quad q of rows g, g + G, ...;
+    // a row index past the band's last row is clamped to it (the same values to the same cells).
+    // Quotients come from a float reciprocal: exact, as every numerator is below 2^8 and every
+    // true quotient is at least 1/(2 Q) > 2^-8 away from the next integer.
+    struct Stage {
+        int t0, b_lo, b_hi, c_lo, c_hi;
+        bool lds, mine;
+        int K, gb, gb_max, lb, lb_max, gstep, lstep;
+    };
+    constexpr int esz = (int)sizeof(typename SI::raw);
+    auto plan_band = [&](int t0) {
+        Stage P{};
+        P.t0 = t0;
+        P.b_lo = max(tr0, t0);
+        P.b_hi = min(tr1, t0 + kBand - 1);
+        const int ea = ray_col(e0, P.b_lo), eb = ray_col(e0, P.b_hi);
+        const int ec = ray_col(e1, P.b_lo), ed = ray_col(e1, P.b_hi);
+        P.c_lo = __builtin_amdgcn_readfirstlane(min(min(ea, eb), min(ec, ed)) - 1);
+        P.c_hi = __builtin_amdgcn_readfirstlane(max(max(ea, eb), max(ec, ed)) + 1);
+        const int W = P.c_hi + 2 - P.c_lo; // columns [c_lo, c_hi + 1] hold the taps
+        P.lds = W <= kPitch - 4;
+        if (P.lds) {
             const int Q = (W + 3) >> 2; // 1 .. 47, below nthreads (a multiple of 64)
             const float rq = __builtin_amdgcn_rcpf((float)Q);
-            const int G = (int)(((float)nthreads + 0.5f) * rq);
+            const int G = __builtin_amdgcn_readfirstlane((int)(((float)nthreads + 0.5f) * rq));
             const int g = (int)(((float)threadIdx.x + 0.5f) * rq);
-            if (g < G) {
-                const float rg = __builtin_amdgcn_rcpf((float)G);
-                const int K = (int)(((float)(kBandRows + G - 1) + 0.5f) * rg); // ceil(33 / G)
-                const int q4 = ((int)threadIdx.x - g * Q) << 2;
-                // byte offsets: global (this thread's quad in row g, and in the last row) and LDS
-                constexpr int esz = (int)sizeof(typename SI::raw);
-                const int gb = ((t0 + g) * rstride + c_lo + cofs + q4) * esz;
-                const int gb_max = ((t0 + kBandRows - 1) * rstride + c_lo + cofs + q4) * esz;
-                const int lb = (g * kPitch + q4) * 4;
-                const int lb_max = ((kBandRows - 1) * kPitch + q4) * 4;
-                const int gstep = G * rstride * esz, lstep = G * kPitch * 4;
-                // groups of eight rows in flight (K <= 17: three groups at most)
-                for (int k0 = 0; k0 < K; k0 += 8) {
-                    // uniform; opaque, so that the compiler keeps one base per group instead of
-                    // one induction variable per row
-                    int sg = k0 * gstep, sl = k0 * lstep;
-                    asm("" : "+s"(sg), "+s"(sl));
+            P.mine = g < G;
+            const float rg = __builtin_amdgcn_rcpf((float)G);
+            // ceil(kBandRows / G)
+            P.K = __builtin_amdgcn_readfirstlane((int)(((float)(kBandRows + G - 1) + 0.5f) * rg));
+            const int q4 = ((int)threadIdx.x - g * Q) << 2;
+            // byte offsets: global (this thread's quad in row g, and in the last row) and LDS
+            P.gb = ((t0 + g) * rstride + P.c_lo + cofs + q4) * esz;
+            P.gb_max = ((t0 + kBandRows - 1) * rstride + P.c_lo + cofs + q4) * esz;
+            P.lb = (g * kPitch + q4) * 4;
+            P.lb_max = ((kBandRows - 1) * kPitch + q4) * 4;
+            P.gstep = G * rstride * esz;
+            P.lstep = G * kPitch * 4;
+        }
+        return P;
+    };
+    // eight rows of the thread's share from row group k0 on, and their store into LDS
+    auto load_rows = [&](const Stage &P, int k0, Quad (&v)[8]) {
+        // uniform; opaque, so that the compiler keeps one base instead of one induction
+        // variable per row
+        int sg = __builtin_amdgcn_readfirstlane(k0 * P.gstep);
+        asm("" : "+s"(sg));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = buf_quad_at<SI>(R, min(P.gb + (sg + u * P.gstep), P.gb_max));
+    };
+    auto store_rows = [&](const Stage &P, int k0, const Quad (&v)[8]) {
+        int sl = __builtin_amdgcn_readfirstlane(k0 * P.lstep);
+        asm("" : "+s"(sl));
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            *reinterpret_cast<float4 *>(reinterpret_cast<char *>(band) +
+                                        min(P.lb + (sl + u * P.lstep), P.lb_max)) =
+                make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
+    };
+    // The first eight rows of a thread's share of band i + 1 are loaded while band i is sampled
+    // (registers `pre`), so their latency hides behind the sampling; K <= 17 rows in all.
+    Stage P = plan_band(floor_div(tr0, kBand) * kBand);
+    Quad pre[8];
+    if (P.lds && P.mine) load_rows(P, 0, pre);
+    for (; P.t0 <= tr1; ++bi) {
+        // whether every lane has a sample on every row [b_lo, b_hi]
+        const bool full = all_in && ta_max <= P.b_lo && tz_min >= P.b_hi;
+        BandSums sums;
+        float jf = (float)(s * (P.b_lo - r0));
+        bool use_lds = false;
+        if (P.lds) {
+            if (P.mine) {
+                store_rows(P, 0, pre);
+                for (int k0 = 8; k0 < P.K; k0 += 8) {
                     Quad v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        v[u] = buf_quad_at<SI>(R, min(gb + (sg + u * gstep), gb_max));
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        *reinterpret_cast<float4 *>(reinterpret_cast<char *>(band) +
-                                                    min(lb + (sl + u * lstep), lb_max)) =
-                            make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
+                    load_rows(P, k0, v);
+                    store_rows(P, k0, v);
                 }
             }
             // every lane checks that its samples in the band stay in the staged columns (the
             // column is monotonic in the row, so its first and last row decide)
-            const int f = max(ta, b_lo), l = min(tz, b_hi);
+            const int f = max(ta, P.b_lo), l = min(tz, P.b_hi);
             bool out = false;
             if (act && f <= l) {
                 const int c1 = floor_i32(fmaf((float)(s * (f - r0)), vc, mc));
                 const int c2 = floor_i32(fmaf((float)(s * (l - r0)), vc, mc));
-                out = min(c1, c2) < c_lo || max(c1, c2) > c_hi;
+                out = min(c1, c2) < P.c_lo || max(c1, c2) > P.c_hi;
             }
             if (__builtin_amdgcn_ballot_w64(out) != 0 && (threadIdx.x & 63) == 0)
                 flag[bi & 1] = 1;
@@ -367,70 +397,72 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
             // the other slot was last read in the previous band, and is next written in the
             // following one, after this band's closing barrier
             if (threadIdx.x == 0) flag[(bi + 1) & 1] = 0;
-            if (use_lds) {
-                // float offset of the band's first row, column 0, held opaque in a scalar
-                // register so that the compiler does not split a constant off every address
-                int rofs = (b_lo - t0) * kPitch - c_lo;
-                asm("" : "+s"(rofs));
-                const float *rowa = band + rofs;
-                int t = b_lo;
-                if (full) { // every lane has a sample on every row of the band
-                    for (; t + 3 <= b_hi; t += 4, jf += 4.0f * sf, rowa += 4 * kPitch) {
-                        float c[4], x[4][4];
+        }
+        const Stage N = plan_band(P.t0 + kBand);
+        if (N.t0 <= tr1 && N.lds && N.mine) load_rows(N, 0, pre);
+        if (use_lds) {
+            // float offset of the band's first row, column 0, held opaque in a scalar
+            // register so that the compiler does not split a constant off every address
+            int rofs = __builtin_amdgcn_readfirstlane((P.b_lo - P.t0) * kPitch - P.c_lo);
+            asm("" : "+s"(rofs));
+            const float *rowa = band + rofs;
+            int t = P.b_lo;
+            if (full) { // every lane has a sample on every row of the band
+                for (; t + 3 <= P.b_hi; t += 4, jf += 4.0f * sf, rowa += 4 * kPitch) {
+                    float c[4], x[4][4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            c[u] = fmaf(u == 0 ? jf : jf + (float)u * sf, vc, mc);
-                            const float *p = rowa + u * kPitch + floor_i32(c[u]);
-                            x[u][0] = p[0];
-                            x[u][1] = p[1];
-                            x[u][2] = p[kPitch];
-                            x[u][3] = p[kPitch + 1];
-                        }
+                    for (int u = 0; u < 4; ++u) {
+                        c[u] = fmaf(u == 0 ? jf : jf + (float)u * sf, vc, mc);
+                        const float *p = rowa + u * kPitch + floor_i32(c[u]);
+                        x[u][0] = p[0];
+                        x[u][1] = p[1];
+                        x[u][2] = p[kPitch];
+                        x[u][3] = p[kPitch + 1];
+                    }
 #pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            sums.add(x[u][0], x[u][1], x[u][2], x[u][3],
-                                     __builtin_amdgcn_fractf(c[u]));
-                    }
-                    for (; t <= b_hi; ++t, jf += sf, rowa += kPitch) {
-                        const float c = fmaf(jf, vc, mc);
-                        const float *p = rowa + floor_i32(c);
-                        sums.add(p[0], p[1], p[kPitch], p[kPitch + 1], __builtin_amdgcn_fractf(c));
-                    }
-                } else {
-                    const int zc = c_lo + kPitch - 2; // the zero cells
-                    for (; t + 1 <= b_hi; t += 2, jf += 2.0f * sf, rowa += 2 * kPitch) {
-                        float c[2], x[2][4];
+                    for (int u = 0; u < 4; ++u)
+                        sums.add(x[u][0], x[u][1], x[u][2], x[u][3],
+                                 __builtin_amdgcn_fractf(c[u]));
+                }
+                for (; t <= P.b_hi; ++t, jf += sf, rowa += kPitch) {
+                    const float c = fmaf(jf, vc, mc);
+                    const float *p = rowa + floor_i32(c);
+                    sums.add(p[0], p[1], p[kPitch], p[kPitch + 1], __builtin_amdgcn_fractf(c));
+                }
+            } else {
+                const int zc = P.c_lo + kPitch - 2; // the zero cells
+                for (; t + 1 <= P.b_hi; t += 2, jf += 2.0f * sf, rowa += 2 * kPitch) {
+                    float c[2], x[2][4];
 #pragma unroll
-                        for (int u = 0; u < 2; ++u) {
-                            c[u] = fmaf(u == 0 ? jf : jf + (float)u * sf, vc, mc);
-                            const int i = floor_i32(c[u]);
-                            const bool in = (unsigned)(t + u) - ta_m <= span_m;
-                            const float *p = rowa + u * kPitch + (in ? i : zc);
-                            x[u][0] = p[0];
-                            x[u][1] = p[1];
-                            x[u][2] = p[kPitch];
-                            x[u][3] = p[kPitch + 1];
-                        }
+                    for (int u = 0; u < 2; ++u) {
+                        c[u] = fmaf(u == 0 ? jf : jf + (float)u * sf, vc, mc);
+                        const int i = floor_i32(c[u]);
+                        const bool in = (unsigned)(t + u) - ta_m <= span_m;
+                        const float *p = rowa + u * kPitch + (in ? i : zc);
+                        x[u][0] = p[0];
+                        x[u][1] = p[1];
+                        x[u][2] = p[kPitch];
+                        x[u][3] = p[kPitch + 1];
+                    }
 #pragma unroll
-                        for (int u = 0; u < 2; ++u)
-                            sums.add(x[u][0], x[u][1], x[u][2], x[u][3],
-                                     __builtin_amdgcn_fractf(c[u]));
-                    }
-                    if (t <= b_hi) {
-                        const float c = fmaf(jf, vc, mc);
-                        const int i = floor_i32(c);
-                        const bool in = (unsigned)t - ta_m <= span_m;
-                        const float *p = rowa + (in ? i : zc);
-                        sums.add(p[0], p[1], p[kPitch], p[kPitch + 1], __builtin_amdgcn_fractf(c));
-                    }
+                    for (int u = 0; u < 2; ++u)
+                        sums.add(x[u][0], x[u][1], x[u][2], x[u][3],
+                                 __builtin_amdgcn_fractf(c[u]));
+                }
+                if (t <= P.b_hi) {
+                    const float c = fmaf(jf, vc, mc);
+                    const int i = floor_i32(c);
+                    const bool in = (unsigned)t - ta_m <= span_m;
+                    const float *p = rowa + (in ? i : zc);
+                    sums.add(p[0], p[1], p[kPitch], p[kPitch + 1], __builtin_amdgcn_fractf(c));
                 }
             }
-            __syncthreads(); // the band and its flag are read before the next band is staged
         }
+        if (P.lds) __syncthreads(); // the band and its flag are read before the next is staged
         if (!use_lds) {
             // lanes without a sample on the row sample their nearest own sample and add zeros
             const float jmin = (float)jlo, jmax = (float)(act ? jhi : jlo);
-            for (int t = b_lo; t <= b_hi; ++t, jf += sf) {
+            for (int t = P.b_lo; t <= P.b_hi; ++t, jf += sf) {
                 const float j = __builtin_amdgcn_fmed3f(jf, jmin, jmax);
                 const float c = fmaf(j, vc, mc);
                 const int o = __mul24(r0 + s * (int)j, rstride) + floor_i32(c) + cofs;
@@ -441,6 +473,7 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
             }
         }
         acc += sums.total(fb);
+        P = N;
     }
     return acc;
 }
