@@ -242,3 +242,37 @@ def test_errors_like_reference(g1, gpu):
         cm.forward(x.t(), geom)  # not contiguous
     with pytest.raises(ValueError, match="Unknown geometry"):
         cm.forward(x, torch.zeros(1), None)
+
+
+def _fan_case(rec, gpu, seed):
+    geom = as_fan_geometry(rec)
+    ny, nx = int(rec.vol_shape[0]), int(rec.vol_shape[1])
+    r = np.random.default_rng(seed)
+    x = r.random((ny, nx))
+    y = r.random((rec.nangles, rec.det_shape))
+    return geom, x, y
+
+
+@pytest.mark.parametrize("case", ["anisotropic", "wide_cells", "degenerate", "short_arc"])
+def test_unusual_geometries(oracle, gpu, case):
+    """Forward, backprojection and adjoint against the oracle on geometries the workloads do
+    not reach: a non-square odd image with anisotropic spacing, offsets, a detector offset and
+    per-angle DSO / DSD; cells 3 texels wide on a 256-wide image, so that the 128 rays of a
+    workgroup span about 230 columns and the forward samples those bands from global memory
+    (the LDS path holds at most 188); a 2 x 3 image seen by one cell at one angle; and a short
+    arc whose rays are all x-dominant."""
+    if case == "anisotropic":
+        rec = fan(45, 77, 50, 90.0, 150.0, du=1.3, det_offset=1.5, spacing=(0.7, 1.3),
+                  offset=(2.0, -1.0), ny=31)
+        rec.DSO = np.linspace(85.0, 95.0, 50)
+        rec.DSD = rec.DSO + np.linspace(55.0, 65.0, 50)
+    elif case == "wide_cells":
+        rec = fan(256, 140, 16, 400.0, 667.0, du=3.0)
+    elif case == "degenerate":
+        rec = fan(3, 1, 1, 20.0, 35.0, ny=2, angles=[0.3])
+    else:
+        rec = fan(64, 91, 24, 128.0, 192.0, angles=np.linspace(-0.3, 0.3, 24) + np.pi / 2)
+    geom, x, y = _fan_case(rec, gpu, 31)
+    check_close(host(cm.forward(dev(x, gpu), geom)), oracle.fwd2d(rec, x), f"{case} fwd2d")
+    check_close(host(cm.backward(dev(y, gpu), geom)), oracle.bp2d(rec, y), f"{case} bp2d")
+    check_close(host(cm.adjoint(dev(y, gpu), geom)), oracle.adj2d(rec, y), f"{case} adj2d")
