@@ -206,10 +206,20 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
                                                      int ny, int nz, int U, int V, int A,
                                                      typename SS::raw *__restrict__ sino) {
     const int lane = threadIdx.x; // blockDim (64, 4); consecutive lanes step along v
-    const int iu = blockIdx.x * 16 + threadIdx.y * 4 + (lane >> 4);
-    const int iv = blockIdx.y * 16 + (lane & 15);
-    const int a = blockIdx.z % A;
-    const size_t b = blockIdx.z / A;
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so hardware id
+    // i runs on XCD i % 8; logical tile (i % 8) * (n / 8) + i / 8 gives each XCD a contiguous
+    // run of tiles (neighbouring tiles read the same voxels, from that XCD's L2). The last
+    // n % 8 ids keep their own position.
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned n = gx * gy * gridDim.z;
+    const unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned n8 = n & ~7u;
+    const unsigned li = hw < n8 ? (hw & 7u) * (n8 >> 3) + (hw >> 3) : hw;
+    const unsigned tx = li % gx, ty = (li / gx) % gy, tz = li / (gx * gy);
+    const int iu = (int)tx * 16 + threadIdx.y * 4 + (lane >> 4);
+    const int iv = (int)ty * 16 + (lane & 15);
+    const int a = (int)tz % A;
+    const size_t b = tz / A;
     if (iu >= U || iv >= V) return;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
     const March3 p = march_params(geo[a], nx, ny, nz, iu, iv, Wp, Hp, Dp);
