@@ -479,7 +479,7 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
 }
 
 // One workgroup = nthreads <= kFwdRays adjacent detector cells of one angle, one thread per
-// ray; LDS holds one band of kBandRows x kPitch floats (24.8 KB).
+// ray; LDS holds one band of kBandRows x kPitch floats (24.6 KB).
 // sino[b][a][k] = scale * sum_{j<N} bilinear(img, e + j v)   (forward_2d.hpp:63-78)
 template <class SI, class SS>
 __global__ __launch_bounds__(kFwdRays) void k_fwd2d(const typename SI::raw *__restrict__ scratch,
