@@ -630,7 +630,8 @@ __device__ __forceinline__ float adj2d_ymax(const float *__restrict__ parts) {
 // 64-bit image with integer global atomics (several angle groups share a tile), and
 // k_adj2d_finish adds that image to the volume.
 constexpr int kT2 = 32;       // texel tile edge
-constexpr int kT2Rays = 128;  // candidate rays staged per chunk
+constexpr int kT2Rays = 256;  // candidate rays staged per chunk
+constexpr int kAdjAngles = 4; // angles whose candidate rays are staged together
 constexpr int kT2Pitch = kT2 + 1;
 
 template <class SS>
@@ -658,40 +659,66 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
     const float qx = (tid & 1) ? (float)tx0 + (float)kT2 + 0.5f : (float)tx0 - 0.5f;
     const float qy = (tid & 2) ? (float)ty0 + (float)kT2 + 0.5f : (float)ty0 - 0.5f;
     const typename SS::raw *sb = sino + b * (size_t)A * D;
-    for (int a = a0; a < a1; ++a) {
-        const float *gp = ageo + 6 * a; // wave-uniform
-        const float wx = qx - gp[0], wy = qy - gp[1];
-        float kk = -(wx * gp[3] - wy * gp[2]) / (wx * gp[5] - wy * gp[4]);
+    // kAdjAngles angles at a time: their candidate rays form one list, staged in LDS in chunks of
+    // kT2Rays (one pair of barriers per chunk instead of per angle, and more work per chunk)
+    for (int ag = a0; ag < a1; ag += kAdjAngles) {
+        const int na = min(kAdjAngles, a1 - ag);
+        // candidate range per angle: lanes 4 s .. 4 s + 3 of every wavefront project the four
+        // corners of the footprint box for angle ag + s
+        const int slot = (tid & 63) >> 2;
+        float kk = 0.0f;
+        if (slot < na) {
+            const float *gp = ageo + 6 * (ag + slot);
+            const float wx = qx - gp[0], wy = qy - gp[1];
+            kk = -(wx * gp[3] - wy * gp[2]) / (wx * gp[5] - wy * gp[4]);
+        }
         float kmin = kk, kmax = kk;
         for (int off = 1; off < 4; off <<= 1) {
             kmin = fminf(kmin, __shfl_xor(kmin, off, 64));
             kmax = fmaxf(kmax, __shfl_xor(kmax, off, 64));
         }
-        kmin = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(kmin)));
-        kmax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(kmax)));
-        if (!(kmax >= -1.0f) || !(kmin <= (float)D)) continue; // misses the detector (or NaN)
-        const int k0 = max((int)floorf(kmin - 0.01f), 0);
-        const int k1 = min((int)ceilf(kmax + 0.01f), D - 1);
-        for (int kc = k0; kc <= k1; kc += kT2Rays) {
-            const int nk = min(kT2Rays, k1 - kc + 1);
+        int k0s[kAdjAngles], offs[kAdjAngles], cnts[kAdjAngles];
+        int total = 0;
+#pragma unroll
+        for (int q = 0; q < kAdjAngles; ++q) {
+            const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kmin), 4 * q));
+            const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(kmax), 4 * q));
+            int cnt = 0, k0 = 0;
+            // an angle whose box misses the detector (or gives NaN) has no candidates
+            if (q < na && hi >= -1.0f && lo <= (float)D) {
+                k0 = max((int)floorf(lo - 0.01f), 0);
+                cnt = max(min((int)ceilf(hi + 0.01f), D - 1) - k0 + 1, 0);
+            }
+            k0s[q] = k0;
+            offs[q] = total;
+            cnts[q] = cnt;
+            total += cnt;
+        }
+        for (int rc = 0; rc < total; rc += kT2Rays) {
+            const int nk = min(kT2Rays, total - rc);
             __syncthreads(); // the previous chunk's readers are done
             if (tid < nk) {
-                const size_t r = (size_t)a * D + kc + tid;
-                const float4 q = ns[r];
-                const int n = __float_as_int(q.y);
+                const int ri = rc + tid;
+                int q = 0;
+#pragma unroll
+                for (int u = 0; u < kAdjAngles; ++u)
+                    if (ri >= offs[u] && ri < offs[u] + cnts[u]) q = u;
+                const size_t r = (size_t)(ag + q) * D + k0s[q] + (ri - offs[q]);
+                const float4 qn = ns[r];
+                const int n = __float_as_int(qn.y);
                 const float4 e = ev[r];
                 const bool ydom = fabsf(e.w) == 1.0f;
                 const float md = ydom ? e.y : e.x, sd = ydom ? e.w : e.z;
                 const int r0 = (int)floorf(md);
-                const float yv = n > 0 ? to_f32<SS>(sb[r]) * q.x * yscale : 0.0f; // |yv| <= 2^30
+                const float yv = n > 0 ? to_f32<SS>(sb[r]) * qn.x * yscale : 0.0f; // |yv| <= 2^30
                 r_mc[tid] = ydom ? e.x : e.y;
                 r_vc[tid] = ydom ? e.z : e.w;
                 r_yv[tid] = yv;
                 r_fb[tid] = md - (float)r0;
                 r_r0[tid] = r0;
                 r_sd[tid] = (sd > 0.0f ? 1 : 0) | (ydom ? 2 : 0);
-                r_jlo[tid] = yv != 0.0f ? (int)q.z : 1; // empty range for rays adding nothing
-                r_jhi[tid] = yv != 0.0f ? (int)q.z + n - 1 : 0;
+                r_jlo[tid] = yv != 0.0f ? (int)qn.z : 1; // empty range for rays adding nothing
+                r_jhi[tid] = yv != 0.0f ? (int)qn.z + n - 1 : 0;
             }
             __syncthreads();
             // items (ray, t): the sample whose dominant lower tap is pd = d0 - 1 + t
