@@ -93,6 +93,13 @@ __device__ __forceinline__ int floor_i32(float x) {
     return r;
 }
 
+// floor(x + 0.5) as int32 in one instruction (v_cvt_rpi_i32_f32) for finite |x| < 2^31.
+__device__ __forceinline__ int round_i32(float x) {
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // ------------------------------------------------------------------ buffer loads -------
 // Raw buffer resources give 32-bit offsets from a scalar base and hardware range checking:
 // a load that lies wholly beyond num_records bytes returns 0 instead of touching memory.

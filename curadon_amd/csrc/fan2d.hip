@@ -620,19 +620,22 @@ __device__ __forceinline__ float adj2d_ymax(const float *__restrict__ parts) {
 // Matched adjoint (exact transpose of k_fwd2d), tile-driven. A workgroup owns a kT2 x kT2 texel
 // tile and a group of angles. Per angle, the candidate rays are those through the tile's
 // footprint box (every texel's [t - .5, t + 1.5) box, in the ray search frame, projected onto
-// the detector through its corners); they are staged in LDS, and the (ray, row) pairs of the
-// kT2 + 1 dominant rows whose samples reach the tile are spread over the workgroup. A sample on
-// row pd (lower tap) with column c adds y q.x times its forward weights to the texels
-// (pd, floor c), (pd, floor c + 1), (pd + 1, ...) (fan2d.hip header), in 64-bit fixed point
-// (units of max|y q.x| / 2^30, one term fits in 32 bits) with integer LDS atomics: gfx950 runs
-// them about 45 times faster than FP32 LDS atomic adds, and integer sums are exact, so the
-// result does not depend on the order of the additions. Each tile then adds its sums to a
-// 64-bit image with integer global atomics (several angle groups share a tile), and
+// the detector through its corners); the candidates of kAdjAngles angles form one list, and
+// each thread takes one ray, keeps its parameters in registers and walks the kT2 + 1 dominant
+// rows whose samples reach the tile. A sample on row pd (lower tap) with column c adds y q.x
+// times its forward weights to the texels (pd, floor c), (pd, floor c + 1), (pd + 1, ...)
+// (fan2d.hip header), in 64-bit fixed point (units of max|y q.x| / 2^30, one term fits in 32
+// bits) with integer LDS atomics: gfx950 runs them about 45 times faster than FP32 LDS atomic
+// adds, and integer sums are exact, so the result does not depend on the order of the
+// additions. y- and x-dominant rays add to separate tiles whose rows run along their dominant
+// axis, padded by one row and column on each side: the four taps of a sample are then four
+// immediate offsets from one LDS address and need no bounds checks. Each tile then adds its
+// sums to a 64-bit image with integer global atomics (several angle groups share a tile), and
 // k_adj2d_finish adds that image to the volume.
 constexpr int kT2 = 32;       // texel tile edge
-constexpr int kT2Rays = 256;  // candidate rays staged per chunk
 constexpr int kAdjAngles = 4; // angles whose candidate rays are staged together
-constexpr int kT2Pitch = kT2 + 1;
+constexpr int kT2Pad = kT2 + 2;   // padded rows and columns
+constexpr int kT2Pitch = kT2 + 3; // odd pitch in 64-bit words
 
 template <class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__restrict__ accimg,
@@ -642,15 +645,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
                                                          const float4 *__restrict__ ns,
                                                          const float *__restrict__ parts, int nx,
                                                          int ny, int D, int A, int G) {
-    __shared__ unsigned long long tile[kT2 * kT2Pitch];
-    __shared__ float r_mc[kT2Rays], r_vc[kT2Rays], r_yv[kT2Rays], r_fb[kT2Rays];
-    __shared__ int r_r0[kT2Rays], r_sd[kT2Rays], r_jlo[kT2Rays], r_jhi[kT2Rays];
+    // tile[0]: y-dominant rays, rows along y; tile[1]: x-dominant rays, rows along x. Padded
+    // by one row and column before and after, so every tap of a kept sample lands in the array.
+    __shared__ unsigned long long tile[2][kT2Pad * kT2Pitch];
     const int tid = threadIdx.x;
     const int g = blockIdx.z % G;
     const size_t b = blockIdx.z / G;
     const int tx0 = blockIdx.x * kT2, ty0 = blockIdx.y * kT2;
     const int a0 = (int)((int64_t)g * A / G), a1 = (int)((int64_t)(g + 1) * A / G);
-    for (int i = tid; i < kT2 * kT2Pitch; i += kMaxBlock) tile[i] = 0ull;
+    for (int i = tid; i < 2 * kT2Pad * kT2Pitch; i += kMaxBlock) (&tile[0][0])[i] = 0ull;
+    __syncthreads(); // the tiles are clear before any wave adds to them
     const float ymax = adj2d_ymax(parts + b * kAdjParts);
     const float yscale = ymax > 0.0f ? 1073741824.0f / ymax : 0.0f;
     // the tile's first texel in the ray table's centred padded frame
@@ -659,8 +663,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
     const float qx = (tid & 1) ? (float)tx0 + (float)kT2 + 0.5f : (float)tx0 - 0.5f;
     const float qy = (tid & 2) ? (float)ty0 + (float)kT2 + 0.5f : (float)ty0 - 0.5f;
     const typename SS::raw *sb = sino + b * (size_t)A * D;
-    // kAdjAngles angles at a time: their candidate rays form one list, staged in LDS in chunks of
-    // kT2Rays (one pair of barriers per chunk instead of per angle, and more work per chunk)
+    // kAdjAngles angles at a time: their candidate rays form one list, one ray per thread
     for (int ag = a0; ag < a1; ag += kAdjAngles) {
         const int na = min(kAdjAngles, a1 - ag);
         // candidate range per angle: lanes 4 s .. 4 s + 3 of every wavefront project the four
@@ -694,59 +697,52 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
             cnts[q] = cnt;
             total += cnt;
         }
-        for (int rc = 0; rc < total; rc += kT2Rays) {
-            const int nk = min(kT2Rays, total - rc);
-            __syncthreads(); // the previous chunk's readers are done
-            if (tid < nk) {
-                const int ri = rc + tid;
-                int q = 0;
+        // one candidate ray per thread, its parameters in registers
+        for (int ri = tid; ri < total; ri += kMaxBlock) {
+            int q = 0;
 #pragma unroll
-                for (int u = 0; u < kAdjAngles; ++u)
-                    if (ri >= offs[u] && ri < offs[u] + cnts[u]) q = u;
-                const size_t r = (size_t)(ag + q) * D + k0s[q] + (ri - offs[q]);
-                const float4 qn = ns[r];
-                const int n = __float_as_int(qn.y);
-                const float4 e = ev[r];
-                const bool ydom = fabsf(e.w) == 1.0f;
-                const float md = ydom ? e.y : e.x, sd = ydom ? e.w : e.z;
-                const int r0 = (int)floorf(md);
-                const float yv = n > 0 ? to_f32<SS>(sb[r]) * qn.x * yscale : 0.0f; // |yv| <= 2^30
-                r_mc[tid] = ydom ? e.x : e.y;
-                r_vc[tid] = ydom ? e.z : e.w;
-                r_yv[tid] = yv;
-                r_fb[tid] = md - (float)r0;
-                r_r0[tid] = r0;
-                r_sd[tid] = (sd > 0.0f ? 1 : 0) | (ydom ? 2 : 0);
-                r_jlo[tid] = yv != 0.0f ? (int)qn.z : 1; // empty range for rays adding nothing
-                r_jhi[tid] = yv != 0.0f ? (int)qn.z + n - 1 : 0;
-            }
-            __syncthreads();
-            // items (ray, t): the sample whose dominant lower tap is pd = d0 - 1 + t
-            const int items = nk * (kT2 + 1);
-            for (int it = tid; it < items; it += kMaxBlock) {
-                const int ri = it / (kT2 + 1), t = it - ri * (kT2 + 1);
-                const int sdv = r_sd[ri];
-                const bool ydom = (sdv & 2) != 0;
-                const int s = (sdv & 1) ? 1 : -1;
-                const int d0 = ydom ? py0 : px0, c0 = ydom ? px0 : py0;
-                const int jf = s * (d0 - 1 + t - r_r0[ri]);
-                if (jf < r_jlo[ri] || jf > r_jhi[ri]) continue;
-                const float c = fmaf((float)jf, r_vc[ri], r_mc[ri]);
+            for (int u = 0; u < kAdjAngles; ++u)
+                if (ri >= offs[u] && ri < offs[u] + cnts[u]) q = u;
+            const size_t r = (size_t)(ag + q) * D + k0s[q] + (ri - offs[q]);
+            const float4 qn = ns[r];
+            const int n = __float_as_int(qn.y);
+            if (n <= 0) continue;
+            const float yv = to_f32<SS>(sb[r]) * qn.x * yscale; // |yv| <= 2^30
+            if (yv == 0.0f) continue;
+            const float4 e = ev[r];
+            const bool ydom = fabsf(e.w) == 1.0f;
+            const float md = ydom ? e.y : e.x, sd = ydom ? e.w : e.z;
+            const float mc = ydom ? e.x : e.y, vc = ydom ? e.z : e.w;
+            const int r0 = (int)floorf(md);
+            const float fb = md - (float)r0;
+            const float wl = yv * (1.0f - fb), wh = yv * fb; // rows pd, pd + 1
+            const int s = sd > 0.0f ? 1 : -1;
+            const int jlo = (int)qn.z, jhi = jlo + n - 1;
+            const int d0 = ydom ? py0 : px0, c0 = ydom ? px0 : py0;
+            // Step t is the sample whose dominant lower tap is pd = d0 - 1 + t (tile row t - 1):
+            // jf = s (pd - r0). Keep the steps with jf in [jlo, jhi] that reach the tile's rows.
+            const int ta = s > 0 ? jlo + r0 - d0 + 1 : r0 - jhi - d0 + 1;
+            const int t_lo = max(ta, 0), t_hi = min(ta + n - 1, kT2);
+            // Step t adds wl [1 - fa, fa] to tile row t - 1 and wh [1 - fa, fa] to row t at the
+            // columns lc, lc + 1: padded rows t, t + 1 and columns lc + 1, lc + 2, four
+            // immediate offsets from one address. Steps with lc outside [-1, kT2) miss the tile.
+            unsigned long long *row = &tile[ydom ? 0 : 1][t_lo * kT2Pitch + 1];
+            float jf = (float)(s * (d0 - 1 + t_lo - r0));
+            const float fs = (float)s;
+            for (int t = t_lo; t <= t_hi; ++t) {
+                const float c = fmaf(jf, vc, mc);
+                jf += fs;
                 const int lc = floor_i32(c) - c0; // the lower tap's column in the tile
-                if (lc < -1 || lc >= kT2) continue;
-                const float fa = __builtin_amdgcn_fractf(c), fb = r_fb[ri], yv = r_yv[ri];
-                const float wa[2] = {yv * (1.0f - fb), yv * fb}; // rows t - 1, t
-                const float wc[2] = {1.0f - fa, fa};             // columns lc, lc + 1
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int ld = t - 1 + (k >> 1), lcc = lc + (k & 1);
-                    if ((unsigned)ld < (unsigned)kT2 && (unsigned)lcc < (unsigned)kT2) {
-                        const int q = (int)__builtin_rintf(wa[k >> 1] * wc[k & 1]);
-                        // tile rows run along y: (ld, lcc) is (y, x) for y-dominant rays
-                        const int ix = ydom ? lcc : ld, iy = ydom ? ld : lcc;
-                        atomicAdd(&tile[iy * kT2Pitch + ix], (unsigned long long)(long long)q);
-                    }
+                if ((unsigned)(lc + 1) <= (unsigned)kT2) {
+                    const float fa = __builtin_amdgcn_fractf(c);
+                    const float v1 = wl * fa, v0 = wl - v1, u1 = wh * fa, u0 = wh - u1;
+                    unsigned long long *q = row + lc;
+                    atomicAdd(q, (unsigned long long)(long long)round_i32(v0));
+                    atomicAdd(q + 1, (unsigned long long)(long long)round_i32(v1));
+                    atomicAdd(q + kT2Pitch, (unsigned long long)(long long)round_i32(u0));
+                    atomicAdd(q + kT2Pitch + 1, (unsigned long long)(long long)round_i32(u1));
                 }
+                row += kT2Pitch;
             }
         }
     }
@@ -755,7 +751,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
     for (int i = tid; i < kT2 * kT2; i += kMaxBlock) {
         const int lx = i % kT2, ly = i / kT2;
         const int X = tx0 + lx, Y = ty0 + ly;
-        const unsigned long long v = tile[ly * kT2Pitch + lx];
+        const unsigned long long v =
+            tile[0][(ly + 1) * kT2Pitch + lx + 1] + tile[1][(lx + 1) * kT2Pitch + ly + 1];
         if (X < nx && Y < ny && v != 0ull) atomicAdd(&out[(size_t)Y * nx + X], v);
     }
 }
