@@ -320,10 +320,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_absmax_parts(const typename S::ra
 // The tile is added to the volume once, in its storage type.
 constexpr int kAT = 16;
 constexpr int kPh = 4; // candidate rays of one wavefront are kPh pixels apart in u and v
-// LDS tile pitches: odd row and slice pitches (17 and 273 floats, 273 = 17 mod 32) spread the
-// voxels that differ only in y or z over different banks, as the lanes of one atomic
-// instruction hit voxels at various depths along their rays
-constexpr int kATRow = kAT + 1, kATSlice = kAT * kATRow + 1;
+// LDS tile: padded by one voxel before and after on each axis, so that every tap of a sample
+// whose lower corner lies in [-1, kAT) on each axis lands in the array (taps outside the tile
+// are added to the border and dropped): the eight taps are then immediate offsets from one LDS
+// address. Odd row and slice pitches (in 64-bit words) spread voxels that differ only in y or
+// z over different banks, as the lanes of one atomic instruction hit voxels at various depths.
+constexpr int kATP = kAT + 2, kATRow = kATP + 1, kATSlice = kATP * kATRow + 1;
 
 template <class SV, class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restrict__ vol,
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
                                                      const float *__restrict__ M,
                                                      const float *__restrict__ parts, int nx,
                                                      int ny, int nz, int U, int V, int A) {
-    __shared__ unsigned long long tile[kAT * kATSlice];
+    __shared__ unsigned long long tile[kATP * kATSlice];
     const int tid = threadIdx.x;
     const int ntz = (nz + kAT - 1) / kAT;
     const int x0 = blockIdx.x * kAT, y0 = blockIdx.y * kAT, z0 = (blockIdx.z % ntz) * kAT;
@@ -343,10 +345,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
     for (int i = tid & 63; i < kMaxParts; i += 64) ymax = fmaxf(ymax, parts[b * kMaxParts + i]);
     for (int off = 32; off > 0; off >>= 1) ymax = fmaxf(ymax, __shfl_xor(ymax, off, 64));
     const float yscale = ymax > 0.0f ? 1073741824.0f / ymax : 0.0f;
-    for (int i = tid; i < kAT * kATSlice; i += kMaxBlock) tile[i] = 0ull;
+    for (int i = tid; i < kATP * kATSlice; i += kMaxBlock) tile[i] = 0ull;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
     // centred padded coordinate p -> voxel index floor(p) + c
     const int cx = Wp / 2 - kPad, cy = Hp / 2 - kPad, cz = Dp / 2 - kPad;
+    // centred padded coordinate p -> padded tile index floor(p) + o
+    const int ox = cx - x0 + 1, oy = cy - y0 + 1, oz = cz - z0 + 1;
     // samples reaching the tile: floor(p) + c in [t0 - 1, t0 + kAT - 1] on each axis
     const float lox = (float)(x0 - 1 - cx), loy = (float)(y0 - 1 - cy), loz = (float)(z0 - 1 - cz);
     const float hix = lox + (float)(kAT + 1), hiy = loy + (float)(kAT + 1),
@@ -431,26 +435,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
                 // the forward's own positions (k_fwd3d)
                 const float px = fmaf(jf, s4.x, m4.x), py = fmaf(jf, s4.y, m4.y),
                             pz = fmaf(jf, s4.z, m4.z);
-                const int lx = floor_i32(px) + cx - x0, ly = floor_i32(py) + cy - y0,
-                          lz = floor_i32(pz) + cz - z0;
-                if ((unsigned)(lx + 1) > (unsigned)kAT || (unsigned)(ly + 1) > (unsigned)kAT ||
-                    (unsigned)(lz + 1) > (unsigned)kAT)
+                // padded tile indices of the lower corner, kept if in [0, kAT] on each axis
+                const int lx = floor_i32(px) + ox, ly = floor_i32(py) + oy,
+                          lz = floor_i32(pz) + oz;
+                if ((unsigned)lx > (unsigned)kAT || (unsigned)ly > (unsigned)kAT ||
+                    (unsigned)lz > (unsigned)kAT)
                     continue;
                 const float fx = __builtin_amdgcn_fractf(px), fy = __builtin_amdgcn_fractf(py),
                             fz = __builtin_amdgcn_fractf(pz);
-                const float wx[2] = {y * (1.0f - fx), y * fx};
-                const float wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+                // the eight trilinear weights times y, as differences of products
+                const float x1 = y * fx, x0w = y - x1;
+                const float y01 = x0w * fy, y00 = x0w - y01, y11 = x1 * fy, y10 = x1 - y11;
+                const float w[8] = {y00 - y00 * fz, y10 - y10 * fz, y01 - y01 * fz,
+                                    y11 - y11 * fz, y00 * fz, y10 * fz, y01 * fz, y11 * fz};
+                unsigned long long *t = &tile[__mul24(lz, kATSlice) + __mul24(ly, kATRow) + lx];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int X = lx + (k & 1), Y = ly + ((k >> 1) & 1), Z = lz + (k >> 2);
-                    if ((unsigned)X < (unsigned)kAT && (unsigned)Y < (unsigned)kAT &&
-                        (unsigned)Z < (unsigned)kAT) {
-                        const int q = (int)__builtin_rintf(wx[k & 1] * wy[(k >> 1) & 1] *
-                                                           wz[k >> 2]);
-                        atomicAdd(&tile[Z * kATSlice + Y * kATRow + X],
-                                  (unsigned long long)(long long)q);
-                    }
-                }
+                for (int k = 0; k < 8; ++k)
+                    atomicAdd(t + (k >> 2) * kATSlice + ((k >> 1) & 1) * kATRow + (k & 1),
+                              (unsigned long long)(long long)round_i32(w[k]));
             }
         }
     }
@@ -460,7 +462,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
         const int X = x0 + lx, Y = y0 + ly, Z = z0 + lz;
         if (X < nx && Y < ny && Z < nz) {
             const size_t o = ((b * nz + Z) * ny + Y) * (size_t)nx + X;
-            const long long q = (long long)tile[lz * kATSlice + ly * kATRow + lx];
+            const long long q =
+                (long long)tile[(lz + 1) * kATSlice + (ly + 1) * kATRow + lx + 1];
             const float add = (float)((double)q * ((double)ymax * 0x1p-30));
             vol[o] = from_f32<SV>(to_f32<SV>(vol[o]) + add);
         }
