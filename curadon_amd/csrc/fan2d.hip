@@ -302,16 +302,21 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
     // margin each side for rounding; the bounds are the same for every lane and live in scalar
     // registers) and this thread's share of the 16-byte row loads: rows [t0, t0 + kBandRows) x
     // columns [c_lo, c_lo + 4 Q) as FP32, with 4 Q <= kPitch - 4 (the zero cells stay
-    // untouched). Thread g Q + q (g < G groups of Q threads) takes quad q of rows g, g + G, ...;
-    // a row index past the band's last row is clamped to it (the same values to the same cells).
-    // Quotients come from a float reciprocal: exact, as every numerator is below 2^8 and every
-    // true quotient is at least 1/(2 Q) > 2^-8 away from the next integer.
+    // untouched). G' groups of Q threads, G' the largest power of two with G' Q <= nthreads:
+    // thread g Q + q (g < G') takes quad q of the K = kBandRows / G' consecutive rows g K ..
+    // g K + K - 1. A row then lies at a wave-uniform byte offset from the thread's first row in
+    // the copy (one add; the offset stays in the load's range-checked vector offset, so rows
+    // outside the copy read zeros) and at a constant one in LDS (the store's immediate offset),
+    // and no row needs a bound. Quotients come from a float reciprocal: exact, as every
+    // numerator is below 2^8 and every true quotient is at least 1/(2 Q) > 2^-8 away from the
+    // next integer.
     struct Stage {
         int t0, b_lo, b_hi, c_lo, c_hi;
         bool lds, mine;
-        int K, gb, gb_max, lb, lb_max, gstep, lstep;
+        int K, gb, lb;
     };
     constexpr int esz = (int)sizeof(typename SI::raw);
+    const int rowb = __builtin_amdgcn_readfirstlane(rstride * esz); // bytes per row of the copy
     auto plan_band = [&](int t0) {
         Stage P{};
         P.t0 = t0;
@@ -327,42 +332,35 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
             const int Q = (W + 3) >> 2; // 1 .. 47, below nthreads (a multiple of 64)
             const float rq = __builtin_amdgcn_rcpf((float)Q);
             const int G = __builtin_amdgcn_readfirstlane((int)(((float)nthreads + 0.5f) * rq));
+            // G' = 2^lg <= min(G, kBandRows), K = kBandRows >> lg (G >= 2 as Q <= 47)
+            const int lg = min(31 - __builtin_clz((unsigned)G), 5);
+            P.K = kBandRows >> lg;
             const int g = (int)(((float)threadIdx.x + 0.5f) * rq);
-            P.mine = g < G;
-            const float rg = __builtin_amdgcn_rcpf((float)G);
-            // ceil(kBandRows / G)
-            P.K = __builtin_amdgcn_readfirstlane((int)(((float)(kBandRows + G - 1) + 0.5f) * rg));
+            P.mine = g < (1 << lg);
             const int q4 = ((int)threadIdx.x - g * Q) << 2;
-            // byte offsets: global (this thread's quad in row g, and in the last row) and LDS
-            P.gb = ((t0 + g) * rstride + P.c_lo + cofs + q4) * esz;
-            P.gb_max = ((t0 + kBandRows - 1) * rstride + P.c_lo + cofs + q4) * esz;
-            P.lb = (g * kPitch + q4) * 4;
-            P.lb_max = ((kBandRows - 1) * kPitch + q4) * 4;
-            P.gstep = G * rstride * esz;
-            P.lstep = G * kPitch * 4;
+            // byte offsets of this thread's quad in its first row: copy and LDS
+            P.gb = ((t0 + g * P.K) * rstride + P.c_lo + cofs + q4) * esz;
+            P.lb = (g * P.K * kPitch + q4) * 4;
         }
         return P;
     };
-    // eight rows of the thread's share from row group k0 on, and their store into LDS
+    static_assert(kBandRows == 32, "row shares are K = 32 / G' rows");
+    // rows k0 .. k0 + 7 of the thread's share (those below K), and their store into LDS
     auto load_rows = [&](const Stage &P, int k0, Quad (&v)[8]) {
-        // uniform; opaque, so that the compiler keeps one base instead of one induction
-        // variable per row
-        int sg = __builtin_amdgcn_readfirstlane(k0 * P.gstep);
-        asm("" : "+s"(sg));
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = buf_quad_at<SI>(R, min(P.gb + (sg + u * P.gstep), P.gb_max));
-    };
-    auto store_rows = [&](const Stage &P, int k0, const Quad (&v)[8]) {
-        int sl = __builtin_amdgcn_readfirstlane(k0 * P.lstep);
-        asm("" : "+s"(sl));
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            *reinterpret_cast<float4 *>(reinterpret_cast<char *>(band) +
-                                        min(P.lb + (sl + u * P.lstep), P.lb_max)) =
-                make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
+            if (k0 + u < P.K) v[u] = buf_quad_at<SI>(R, P.gb + (k0 + u) * rowb);
+    };
+    auto store_rows = [&](const Stage &P, int k0, const Quad (&v)[8]) {
+        char *l = reinterpret_cast<char *>(band) + P.lb;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < P.K)
+                *reinterpret_cast<float4 *>(l + (k0 + u) * kPitch * 4) =
+                    make_float4(v[u].v[0], v[u].v[1], v[u].v[2], v[u].v[3]);
     };
     // The first eight rows of a thread's share of band i + 1 are loaded while band i is sampled
-    // (registers `pre`), so their latency hides behind the sampling; K <= 17 rows in all.
+    // (registers `pre`), so their latency hides behind the sampling; K <= 16 rows in all.
     Stage P = plan_band(floor_div(tr0, kBand) * kBand);
     Quad pre[8];
     if (P.lds && P.mine) load_rows(P, 0, pre);
@@ -375,10 +373,10 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
         if (P.lds) {
             if (P.mine) {
                 store_rows(P, 0, pre);
-                for (int k0 = 8; k0 < P.K; k0 += 8) {
+                if (P.K > 8) {
                     Quad v[8];
-                    load_rows(P, k0, v);
-                    store_rows(P, k0, v);
+                    load_rows(P, 8, v);
+                    store_rows(P, 8, v);
                 }
             }
             // every lane checks that its samples in the band stay in the staged columns (the
