@@ -15,6 +15,16 @@ constexpr int kPadDet = 2; // zero border of staged sinograms and panels, in cel
 constexpr int kZB = 8;     // z voxels per thread in the 3D backprojector (general geometry)
 constexpr int kYB = 8;     // y voxels per thread in the 3D backprojector (y-invariant U, W)
 
+// Per-angle tables are read through the constant address space: they do not change during a
+// launch, so the compiler may use scalar loads for their wave-uniform addresses even where the
+// pointer comes from memory (the serving kernel's job records) and volume stores follow.
+using ctab = const __attribute__((address_space(4))) float *;
+__device__ __forceinline__ ctab const_table(const void *p) { return (ctab)p; }
+// Entry i of a table of float4 (four scalar loads, merged into one).
+__device__ __forceinline__ float4 table4(ctab t, int i) {
+    return make_float4(t[4 * i], t[4 * i + 1], t[4 * i + 2], t[4 * i + 3]);
+}
+
 // Linear interpolation of a pair at weight f, FP32 arithmetic.
 __device__ __forceinline__ float lerp_pair(const Pair &p, float f) {
     return fmaf(f, p.hi - p.lo, p.lo);
@@ -24,7 +34,7 @@ __device__ __forceinline__ float lerp_pair(const Pair &p, float f) {
 //   sum_a w_a^WPOW lerp(sino[a], w_a eta_a + D/2),  w_a = DSD / (xi_a + DSO)
 // meta[a] = (-sin, cos, DSD, DSO); px, py = x / du, y / du. `R` covers [A][D + 4].
 template <class SS, int WPOW>
-__device__ __forceinline__ float bp2d_pixel(rsrc_t R, const float4 *__restrict__ meta, int A,
+__device__ __forceinline__ float bp2d_pixel(rsrc_t R, ctab meta, int A,
                                            int D, float x, float y, float px, float py,
                                            float half_d) {
     const int Dp = D + 2 * kPadDet;
@@ -47,14 +57,14 @@ __device__ __forceinline__ float bp2d_pixel(rsrc_t R, const float4 *__restrict__
         float wt[4], al[4];
         Pair p[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) sample(meta[a + k], row + k * Dp, wt[k], al[k], p[k]);
+        for (int k = 0; k < 4; ++k) sample(table4(meta, a + k), row + k * Dp, wt[k], al[k], p[k]);
 #pragma unroll
         for (int k = 0; k < 4; ++k) acc = fmaf(fmaf(al[k], p[k].hi - p[k].lo, p[k].lo), wt[k], acc);
     }
     for (; a < A; ++a, row += Dp) {
         float wt, al;
         Pair p;
-        sample(meta[a], row, wt, al, p);
+        sample(table4(meta, a), row, wt, al, p);
         acc = fmaf(fmaf(al, p.hi - p.lo, p.lo), wt, acc);
     }
     return acc;
@@ -66,7 +76,7 @@ __device__ __forceinline__ float bp2d_pixel(rsrc_t R, const float4 *__restrict__
 // 1 / W^2. (hx, hy, hz) is the centred index of the column's first voxel. `R` covers
 // [A][V + 4][U + 4].
 template <class SS, bool MATCHED, bool FDK>
-__device__ __forceinline__ void bp3d_column(rsrc_t R, const float *__restrict__ M, int A, int U,
+__device__ __forceinline__ void bp3d_column(rsrc_t R, ctab M, int A, int U,
                                            int V, float hx, float hy, float hz,
                                            float (&acc)[kZB]) {
     const int Up = U + 2 * kPadDet, Vp = V + 2 * kPadDet;
@@ -75,7 +85,7 @@ __device__ __forceinline__ void bp3d_column(rsrc_t R, const float *__restrict__ 
     const float ulim = (float)U, vlim = (float)V;
     int base = kPadDet * Up + kPadDet;
     for (int a = 0; a < A; ++a, base += panel) {
-        const float *m = M + 12 * a; // wave-uniform: scalar loads
+        const ctab m = M + 12 * a; // wave-uniform: scalar loads
         float Uh = fmaf(m[0], hx, fmaf(m[1], hy, fmaf(m[2], hz, m[3])));
         float Vh = fmaf(m[4], hx, fmaf(m[5], hy, fmaf(m[6], hz, m[7])));
         float Wh = fmaf(m[8], hx, fmaf(m[9], hy, fmaf(m[10], hz, m[11])));
@@ -135,7 +145,7 @@ struct RunAngle {
 };
 
 template <bool MATCHED, bool FDK>
-__device__ __forceinline__ RunAngle run_angle(const float *m, float hx, float hy, float hz,
+__device__ __forceinline__ RunAngle run_angle(ctab m, float hx, float hy, float hz,
                                               float ulim) {
     const float Uh = fmaf(m[0], hx, fmaf(m[2], hz, m[3]));
     const float Wh = fmaf(m[8], hx, fmaf(m[10], hz, m[11]));
@@ -159,7 +169,7 @@ static __shared__ __align__(16) float foot_lds[kFootFloats];
 // (hx, hy, hz): centred index of this thread's run start; (tx, ty, tz): of the tile origin.
 // All 256 threads of the workgroup call it together (lanes without voxels too).
 template <class SS, bool MATCHED, bool FDK>
-__device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, const float *__restrict__ M, int A, int U,
+__device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, ctab M, int A, int U,
                                              int V, float hx, float hy, float hz, float tx,
                                              float ty, float tz, float (&acc)[kYB]) {
     const int Up = U + 2 * kPadDet, Vp = V + 2 * kPadDet;
@@ -178,7 +188,7 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, const float *__restrict_
         float umin = 0.0f, umax = 0.0f, vmin = 0.0f, vmax = 0.0f;
         int ok = 1;
         if (k8 < na) {
-            const float *m = M + 12 * (a0 + k8);
+            const ctab m = M + 12 * (a0 + k8);
             const float Uh = fmaf(m[0], cx, fmaf(m[2], cz, m[3]));
             const float Wh = fmaf(m[8], cx, fmaf(m[10], cz, m[11]));
             const float Vh = fmaf(m[4], cx, fmaf(m[5], cy, fmaf(m[6], cz, m[7])));
@@ -217,7 +227,7 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, const float *__restrict_
         }
         if (!fits) { // from global memory
             for (int k = 0; k < na; ++k) {
-                const float *m = M + 12 * (a0 + k); // wave-uniform: scalar loads
+                const ctab m = M + 12 * (a0 + k); // wave-uniform: scalar loads
                 RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
                 const int col = (a0 + k) * panel + kPadDet * Up + kPadDet + r.col;
 #pragma unroll
@@ -248,7 +258,7 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, const float *__restrict_
 #pragma unroll
         for (int k = 0; k < kChunk; ++k) {
             if (k >= na) break;
-            const float *m = M + 12 * (a0 + k); // wave-uniform: scalar loads
+            const ctab m = M + 12 * (a0 + k); // wave-uniform: scalar loads
             RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
             // LDS element of padded (row pv, column pu): slot + (pv - sv0) * kFootCols + pu - su0
             const int ik = k * kFootSlot + r.col + kPadDet - su0[k] - (sv0[k] - kPadDet) * kFootCols;

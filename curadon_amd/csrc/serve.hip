@@ -3,19 +3,22 @@
 // zero-bordered layout it reads.
 //
 // Work distribution
-//   The host lists the jobs of a launch with a tile range each; job j owns launch tiles
-//   [first_j, first_j + (tile_end_j - tile_begin_j)). The grid is sized to what the device
-//   holds resident; each workgroup takes the next launch tile from an atomic counter, finds
-//   its job by binary search over `first`, backprojects that tile and repeats until the
-//   counter passes the total. Every workgroup exits after at most total + grid iterations of
-//   the counter, so the launch drains.
+//   The host lists the jobs of a launch with a tile range each; the fan-beam and the
+//   cone-beam jobs go to two persistent kernels on the same stream (fan-beam tiles need no
+//   LDS and few registers, so their kernel keeps more workgroups resident). Within a kernel,
+//   job j owns launch tiles [first_j, first_j + (tile_end_j - tile_begin_j)). The grid is
+//   sized to what the device holds resident of that kernel; each workgroup takes the next
+//   launch tile from the kernel's atomic counter, finds its job by binary search over
+//   `first`, backprojects that tile and repeats until the counter passes the total. Every
+//   workgroup exits after at most total + grid iterations of the counter, so the launch
+//   drains.
 //
 // Per-tile arithmetic is the plain backprojectors' own (bp2d_pixel / bp3d_column with the
 // same scalar inputs), so a voxel's value does not depend on the launch it ran in.
 //
 // Data layout in HBM (per server)
 //   job tables  [kRing][max_jobs] DevJob      one slot per launch in flight
-//   counters    [kRing] uint32                tile counter of each slot
+//   counters    [kRing][2] uint32             tile counters of each slot (fan, cone)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -82,8 +85,8 @@ __global__ __launch_bounds__(kThreads) void k_stage_projections(
     dst[(a * Vp + vp) * (size_t)Up + up] = from_f32<S>(val);
 }
 
-__global__ __launch_bounds__(64) void k_zero_counter(unsigned *__restrict__ c) {
-    if (threadIdx.x == 0) c[0] = 0u;
+__global__ __launch_bounds__(64) void k_zero_counter2(unsigned *__restrict__ c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------- tiles -------------------
@@ -96,7 +99,7 @@ __device__ __forceinline__ void serve_tile_2d(const DevJob &J, uint32_t t) {
     const int iy = ty * kTile2Y + (int)(threadIdx.x / kTile2X);
     if (ix >= J.nx || iy >= J.ny) return;
     const rsrc_t R = make_rsrc(J.sino, J.sino_bytes);
-    const float4 *meta = reinterpret_cast<const float4 *>(J.table);
+    const ctab meta = const_table(J.table);
     const float x = ((float)ix - (float)J.nx / 2.0f) * J.sx + J.ox + 0.5f;
     const float y = ((float)iy - (float)J.ny / 2.0f) * J.sy + J.oy + 0.5f;
     const float px = x * J.inv_du, py = y * J.inv_du;
@@ -124,11 +127,11 @@ __device__ __forceinline__ void serve_tile_3d(const DevJob &J, uint32_t t) {
     for (int z = 0; z < kZB; ++z) acc[z] = 0.0f;
     const float hx = (float)ix - J.cx, hy = (float)iy - J.cy, hz = (float)iz0 - J.cz;
     if (J.mode == CMI_BP3D_FDK)
-        bp3d_column<SS, true, true>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+        bp3d_column<SS, true, true>(R, const_table(J.table), J.nangles, J.nu, J.nv, hx, hy, hz, acc);
     else if (J.mode == CMI_BP3D_MATCHED)
-        bp3d_column<SS, true, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+        bp3d_column<SS, true, false>(R, const_table(J.table), J.nangles, J.nu, J.nv, hx, hy, hz, acc);
     else
-        bp3d_column<SS, false, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, acc);
+        bp3d_column<SS, false, false>(R, const_table(J.table), J.nangles, J.nu, J.nv, hx, hy, hz, acc);
     auto *vol = static_cast<typename SV::raw *>(J.volume);
 #pragma unroll
     for (int z = 0; z < kZB; ++z) {
@@ -158,13 +161,13 @@ __device__ __forceinline__ void serve_tile_3d_y(const DevJob &J, uint32_t t) {
     const float hx = (float)ix - J.cx, hy = (float)iy0 - J.cy, hz = (float)iz - J.cz;
     const float tx = (float)ix0 - J.cx, tz = (float)iz0 - J.cz;
     if (J.mode == CMI_BP3D_FDK)
-        bp3d_yrun_lds<SS, true, true>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
+        bp3d_yrun_lds<SS, true, true>(R, const_table(J.table), J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
                                       acc);
     else if (J.mode == CMI_BP3D_MATCHED)
-        bp3d_yrun_lds<SS, true, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
+        bp3d_yrun_lds<SS, true, false>(R, const_table(J.table), J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
                                        acc);
     else
-        bp3d_yrun_lds<SS, false, false>(R, J.table, J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
+        bp3d_yrun_lds<SS, false, false>(R, const_table(J.table), J.nangles, J.nu, J.nv, hx, hy, hz, tx, hy, tz,
                                         acc);
     if (ix >= J.nx || iz >= J.nz) return;
     auto *vol = static_cast<typename SV::raw *>(J.volume);
@@ -178,27 +181,53 @@ __device__ __forceinline__ void serve_tile_3d_y(const DevJob &J, uint32_t t) {
     }
 }
 
+// Next launch tile from the counter, the same for every thread of the workgroup.
+__device__ __forceinline__ uint32_t next_tile(unsigned *counter, unsigned *s_tile) {
+    __syncthreads(); // every thread has read the previous tile
+    if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*s_tile);
+}
+
+// The job owning launch tile t: the last with first <= t.
+__device__ __forceinline__ const DevJob &job_of(const DevJob *__restrict__ jobs, int njobs,
+                                               uint32_t t) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return jobs[lo];
+}
+
+// Fan-beam tiles. A kernel of its own: it needs no LDS and few registers, so more workgroups
+// stay resident than for the cone-beam tiles (the 2D body is bound by issue and latency).
 template <class SV, class SS>
-__global__ __launch_bounds__(kThreads) void k_serve_bp(const DevJob *__restrict__ jobs,
-                                                       int njobs, uint32_t total,
-                                                       unsigned *__restrict__ counter) {
+__global__ __launch_bounds__(kThreads) void k_serve_bp2d(const DevJob *__restrict__ jobs,
+                                                         int njobs, uint32_t total,
+                                                         unsigned *__restrict__ counter) {
     __shared__ unsigned s_tile;
     for (;;) {
-        __syncthreads(); // every thread has read the previous s_tile
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);
+        const uint32_t t = next_tile(counter, &s_tile);
         if (t >= total) return;
-        int lo = 0, hi = njobs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].first <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const DevJob &J = jobs[lo];
+        const DevJob &J = job_of(jobs, njobs, t);
+        serve_tile_2d<SV, SS>(J, t - J.first + J.tile_begin);
+    }
+}
+
+// Cone-beam tiles.
+template <class SV, class SS>
+__global__ __launch_bounds__(kThreads) void k_serve_bp3d(const DevJob *__restrict__ jobs,
+                                                         int njobs, uint32_t total,
+                                                         unsigned *__restrict__ counter) {
+    __shared__ unsigned s_tile;
+    for (;;) {
+        const uint32_t t = next_tile(counter, &s_tile);
+        if (t >= total) return;
+        const DevJob &J = job_of(jobs, njobs, t);
         const uint32_t lt = t - J.first + J.tile_begin;
-        if (J.kind == CMI_JOB_FAN2D) serve_tile_2d<SV, SS>(J, lt);
-        else if (J.columns == 1) serve_tile_3d_y<SV, SS>(J, lt);
+        if (J.columns == 1) serve_tile_3d_y<SV, SS>(J, lt);
         else serve_tile_3d<SV, SS>(J, lt);
     }
 }
@@ -264,7 +293,8 @@ cmi_status check_job(const cmi_bp_job &j, uint32_t i, int sino_dtype) {
 struct cmi_server {
     int device = 0;
     uint32_t max_jobs = 0;
-    uint32_t grid = 0; // resident workgroups of the serving kernel, per dtype pair
+    uint32_t grid = 0;   // resident workgroups of the cone-beam serving kernel
+    uint32_t grid2d = 0; // resident workgroups of the fan-beam serving kernel
     DevJob *d_jobs = nullptr;
     unsigned *d_counter = nullptr;
     DevJob *h_jobs = nullptr; // pinned
@@ -330,10 +360,13 @@ CMI_API cmi_status cmi_server_create(int device, uint32_t max_jobs, cmi_server *
     p->device = device;
     p->max_jobs = max_jobs;
     CMI_HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, device));
-    int per_cu = 0;
-    CMI_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_serve_bp<F32, F32>,
+    int per_cu = 0, per_cu2 = 0;
+    CMI_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_serve_bp3d<F32, F32>,
+                                                             kThreads, 0));
+    CMI_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, k_serve_bp2d<F32, F32>,
                                                              kThreads, 0));
     p->grid = (uint32_t)std::max(1, per_cu) * (uint32_t)p->cus;
+    p->grid2d = (uint32_t)std::max(1, per_cu2) * (uint32_t)p->cus;
     struct Cleanup {
         cmi_server *p;
         bool armed = true;
@@ -348,7 +381,7 @@ CMI_API cmi_status cmi_server_create(int device, uint32_t max_jobs, cmi_server *
     } cleanup{p.get()};
     const size_t table = sizeof(DevJob) * max_jobs * kRing;
     CMI_HIP_TRY(hipMalloc(&p->d_jobs, table));
-    CMI_HIP_TRY(hipMalloc(&p->d_counter, sizeof(unsigned) * kRing));
+    CMI_HIP_TRY(hipMalloc(&p->d_counter, sizeof(unsigned) * 2 * kRing));
     CMI_HIP_TRY(hipHostMalloc(&p->h_jobs, table, hipHostMallocDefault));
     for (auto &e : p->done) CMI_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     cleanup.armed = false;
@@ -406,8 +439,12 @@ CMI_API cmi_status cmi_serve_backproject(cmi_server *p, const cmi_bp_job *jobs, 
     const int slot = p->next;
     // the slot's previous launch has finished reading its table and counter
     if (p->used[slot]) CMI_HIP_TRY(hipEventSynchronize(p->done[slot]));
+    // the slot's table: fan-beam jobs first, then cone-beam ones, each kind with its own
+    // launch-tile numbering (the two kinds run in separate launches)
     DevJob *h = p->h_jobs + (size_t)slot * p->max_jobs;
-    uint32_t first = 0;
+    uint32_t n2 = 0, total2 = 0, total3 = 0;
+    for (uint32_t i = 0; i < njobs; ++i) n2 += jobs[i].kind == CMI_JOB_FAN2D ? 1u : 0u;
+    uint32_t k2 = 0, k3 = n2;
     for (uint32_t i = 0; i < njobs; ++i) {
         const cmi_bp_job &j = jobs[i];
         DevJob d{};
@@ -430,6 +467,7 @@ CMI_API cmi_status cmi_serve_backproject(cmi_server *p, const cmi_bp_job *jobs, 
             d.tiles_x = (d.nx + kTile3X - 1) / kTile3X;
             d.tiles_y = (d.ny + kTile3Y - 1) / kTile3Y;
         }
+        uint32_t &first = fan ? total2 : total3;
         d.first = first;
         d.tile_begin = j.tile_begin;
         const uint64_t elems = (uint64_t)j.nangles * (j.nu + 2 * kPadDet) *
@@ -441,21 +479,30 @@ CMI_API cmi_status cmi_serve_backproject(cmi_server *p, const cmi_bp_job *jobs, 
         d.cx = (float)(((int)j.nx - 1) / 2.0);
         d.cy = (float)(((int)j.ny - 1) / 2.0);
         d.cz = (float)(((int)j.nz - 1) / 2.0);
-        h[i] = d;
+        h[fan ? k2++ : k3++] = d;
         first += j.tile_end - j.tile_begin;
     }
     DevJob *dj = p->d_jobs + (size_t)slot * p->max_jobs;
-    unsigned *counter = p->d_counter + slot;
+    unsigned *counter = p->d_counter + 2 * slot;
     CMI_HIP_TRY(hipMemcpyAsync(dj, h, sizeof(DevJob) * njobs, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL(k_zero_counter2, dim3(1), dim3(64), 0, s, counter);
     CMI_HIP_TRY(hipGetLastError());
-    const unsigned grid = (unsigned)std::min<uint64_t>(total, p->grid);
     const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
         using SV = decltype(sv);
         using SS = decltype(ss);
-        hipLaunchKernelGGL((k_serve_bp<SV, SS>), dim3(grid), dim3(kThreads), 0, s,
-                           (const DevJob *)dj, (int)njobs, (uint32_t)total, counter);
-        CMI_HIP_TRY(hipGetLastError());
+        if (total3) {
+            const unsigned grid = (unsigned)std::min<uint64_t>(total3, p->grid);
+            hipLaunchKernelGGL((k_serve_bp3d<SV, SS>), dim3(grid), dim3(kThreads), 0, s,
+                               (const DevJob *)(dj + n2), (int)(njobs - n2), total3,
+                               counter + 1);
+            CMI_HIP_TRY(hipGetLastError());
+        }
+        if (total2) {
+            const unsigned grid = (unsigned)std::min<uint64_t>(total2, p->grid2d);
+            hipLaunchKernelGGL((k_serve_bp2d<SV, SS>), dim3(grid), dim3(kThreads), 0, s,
+                               (const DevJob *)dj, (int)n2, total2, counter);
+            CMI_HIP_TRY(hipGetLastError());
+        }
         return CMI_OK;
     });
     CMI_TRY(st);

@@ -23,6 +23,7 @@ fftshift(|linspace(-1, 1, D)|), times pi / (2 A)), for callers that want its out
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import torch
@@ -65,10 +66,8 @@ def fft_length(n_det: int) -> int:
         L += 1
 
 
-def ramp_response(n_det: int, window="ram-lak", frequency_scaling: float = 1.0,
-                  device=None) -> torch.Tensor:
-    """Real frequency response (rfft layout) of the windowed band-limited ramp for rows of
-    n_det cells, FFT length L = fft_length(n_det). Returns (L // 2 + 1,) float32."""
+@functools.lru_cache(maxsize=64)
+def _ramp_response_host(n_det: int, window, frequency_scaling: float) -> torch.Tensor:
     L = fft_length(n_det)
     k = torch.arange(L, dtype=torch.float64)
     m = torch.where(k <= L // 2, k, k - L)                   # signed index, FFT order
@@ -79,7 +78,24 @@ def ramp_response(n_det: int, window="ram-lak", frequency_scaling: float = 1.0,
     H = torch.fft.rfft(h).real                               # symmetric kernel: real response
     W = torch.arange(L // 2 + 1, dtype=torch.float64) / (L // 2)
     H = H * _window(W, window, frequency_scaling)
-    return H.to(torch.float32).to(device)
+    return H.to(torch.float32)
+
+
+@functools.lru_cache(maxsize=64)
+def _ramp_response_dev(n_det: int, window, frequency_scaling: float, device) -> torch.Tensor:
+    return _ramp_response_host(n_det, window, frequency_scaling).to(device)
+
+
+def ramp_response(n_det: int, window="ram-lak", frequency_scaling: float = 1.0,
+                  device=None) -> torch.Tensor:
+    """Real frequency response (rfft layout) of the windowed band-limited ramp for rows of
+    n_det cells, FFT length L = fft_length(n_det). Returns (L // 2 + 1,) float32. Responses
+    are computed once per (n_det, window, scaling) and kept on each device they were asked
+    for; the caller gets its own copy."""
+    key = (int(n_det), window, float(frequency_scaling))
+    if device is None or torch.device(device).type == "cpu":
+        return _ramp_response_host(*key).clone()
+    return _ramp_response_dev(*key, torch.device(device)).clone()
 
 
 def filter_rows(p: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
@@ -93,21 +109,23 @@ def filter_rows(p: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
 
 
 def cosine_weights_fan(geom: FanGeometry, device=None) -> torch.Tensor:
-    """(A, D) cosine pre-weights DSD / sqrt(DSD^2 + u^2), u the cell centre on the detector."""
+    """(A, D) cosine pre-weights DSD / sqrt(DSD^2 + u^2), u the cell centre on the detector.
+    Evaluated in FP64 on `device` (correctly rounded operations: the same values on any
+    device) and returned as FP32."""
     D = geom.det_shape
-    u = (torch.arange(D, dtype=torch.float64) - (D - 1) / 2) * geom.det_spacing
-    dsd = torch.as_tensor(geom.DSD, dtype=torch.float64)[:, None]
-    return (dsd / torch.sqrt(dsd ** 2 + u[None, :] ** 2)).to(torch.float32).to(device)
+    u = (torch.arange(D, dtype=torch.float64, device=device) - (D - 1) / 2) * geom.det_spacing
+    dsd = torch.as_tensor(geom.DSD, dtype=torch.float64).to(device)[:, None]
+    return (dsd / torch.sqrt(dsd * dsd + (u * u)[None, :])).to(torch.float32)
 
 
 def cosine_weights_cone(geom: ConeGeometry, device=None) -> torch.Tensor:
-    """(V, U) cosine pre-weights DSD / sqrt(DSD^2 + u^2 + v^2)."""
+    """(V, U) cosine pre-weights DSD / sqrt(DSD^2 + u^2 + v^2), FP64 on `device`, as FP32."""
     V, U = (int(v) for v in geom.det_shape)
     dv, du = (float(v) for v in geom.det_spacing)
-    u = (torch.arange(U, dtype=torch.float64) - (U - 1) / 2) * du
-    v = (torch.arange(V, dtype=torch.float64) - (V - 1) / 2) * dv
-    r2 = v[:, None] ** 2 + u[None, :] ** 2
-    return (geom.DSD / torch.sqrt(geom.DSD ** 2 + r2)).to(torch.float32).to(device)
+    u = (torch.arange(U, dtype=torch.float64, device=device) - (U - 1) / 2) * du
+    v = (torch.arange(V, dtype=torch.float64, device=device) - (V - 1) / 2) * dv
+    r2 = (v * v)[:, None] + (u * u)[None, :]
+    return (geom.DSD / torch.sqrt(geom.DSD * geom.DSD + r2)).to(torch.float32)
 
 
 def fbp_filter(sinogram: torch.Tensor, geom, window="ram-lak", frequency_scaling=1.0):
