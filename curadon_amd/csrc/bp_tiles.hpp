@@ -15,12 +15,15 @@ constexpr int kPadDet = 2; // zero border of staged sinograms and panels, in cel
 constexpr int kZB = 8;     // z voxels per thread in the 3D backprojector (general geometry)
 constexpr int kYB = 8;     // y voxels per thread in the 3D backprojector (y-invariant U, W)
 
-// Per-angle tables are read through the constant address space: they do not change during a
-// launch, so the compiler may use scalar loads for their wave-uniform addresses even where the
-// pointer comes from memory (the serving kernel's job records) and volume stores follow.
+// Per-angle tables: the bodies take the table pointer type as a template parameter. The plain
+// kernels pass their `const __restrict__` arguments (scalar loads already). The serving kernels
+// pass constant-address-space pointers: their tables come from job records in memory, and
+// with generic pointers the compiler loads every entry per lane, since the volume stores of
+// earlier tiles might alias; tables do not change during a launch.
 using ctab = const __attribute__((address_space(4))) float *;
 __device__ __forceinline__ ctab const_table(const void *p) { return (ctab)p; }
-// Entry i of a table of float4 (four scalar loads, merged into one).
+// Entry i of a table of float4.
+__device__ __forceinline__ float4 table4(const float4 *__restrict__ t, int i) { return t[i]; }
 __device__ __forceinline__ float4 table4(ctab t, int i) {
     return make_float4(t[4 * i], t[4 * i + 1], t[4 * i + 2], t[4 * i + 3]);
 }
@@ -33,8 +36,8 @@ __device__ __forceinline__ float lerp_pair(const Pair &p, float f) {
 // Sum over angles for one pixel at world position (x, y) (backward_2d.hpp:58-96):
 //   sum_a w_a^WPOW lerp(sino[a], w_a eta_a + D/2),  w_a = DSD / (xi_a + DSO)
 // meta[a] = (-sin, cos, DSD, DSO); px, py = x / du, y / du. `R` covers [A][D + 4].
-template <class SS, int WPOW>
-__device__ __forceinline__ float bp2d_pixel(rsrc_t R, ctab meta, int A,
+template <class SS, int WPOW, class TAB>
+__device__ __forceinline__ float bp2d_pixel(rsrc_t R, TAB meta, int A,
                                            int D, float x, float y, float px, float py,
                                            float half_d) {
     const int Dp = D + 2 * kPadDet;
@@ -75,8 +78,8 @@ __device__ __forceinline__ float bp2d_pixel(rsrc_t R, ctab meta, int A,
 // lower-tap panel coordinate. MATCHED drops voxels behind the source (W <= 0); FDK weights by
 // 1 / W^2. (hx, hy, hz) is the centred index of the column's first voxel. `R` covers
 // [A][V + 4][U + 4].
-template <class SS, bool MATCHED, bool FDK>
-__device__ __forceinline__ void bp3d_column(rsrc_t R, ctab M, int A, int U,
+template <class SS, bool MATCHED, bool FDK, class TAB>
+__device__ __forceinline__ void bp3d_column(rsrc_t R, TAB M, int A, int U,
                                            int V, float hx, float hy, float hz,
                                            float (&acc)[kZB]) {
     const int Up = U + 2 * kPadDet, Vp = V + 2 * kPadDet;
@@ -85,7 +88,7 @@ __device__ __forceinline__ void bp3d_column(rsrc_t R, ctab M, int A, int U,
     const float ulim = (float)U, vlim = (float)V;
     int base = kPadDet * Up + kPadDet;
     for (int a = 0; a < A; ++a, base += panel) {
-        const ctab m = M + 12 * a; // wave-uniform: scalar loads
+        const TAB m = M + 12 * a; // wave-uniform: scalar loads
         float Uh = fmaf(m[0], hx, fmaf(m[1], hy, fmaf(m[2], hz, m[3])));
         float Vh = fmaf(m[4], hx, fmaf(m[5], hy, fmaf(m[6], hz, m[7])));
         float Wh = fmaf(m[8], hx, fmaf(m[9], hy, fmaf(m[10], hz, m[11])));
@@ -144,8 +147,8 @@ struct RunAngle {
     float fu, v, dv, w;
 };
 
-template <bool MATCHED, bool FDK>
-__device__ __forceinline__ RunAngle run_angle(ctab m, float hx, float hy, float hz,
+template <bool MATCHED, bool FDK, class TAB>
+__device__ __forceinline__ RunAngle run_angle(TAB m, float hx, float hy, float hz,
                                               float ulim) {
     const float Uh = fmaf(m[0], hx, fmaf(m[2], hz, m[3]));
     const float Wh = fmaf(m[8], hx, fmaf(m[10], hz, m[11]));
@@ -168,8 +171,8 @@ static __shared__ __align__(16) float foot_lds[kFootFloats];
 
 // (hx, hy, hz): centred index of this thread's run start; (tx, ty, tz): of the tile origin.
 // All 256 threads of the workgroup call it together (lanes without voxels too).
-template <class SS, bool MATCHED, bool FDK>
-__device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, ctab M, int A, int U,
+template <class SS, bool MATCHED, bool FDK, class TAB>
+__device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, TAB M, int A, int U,
                                              int V, float hx, float hy, float hz, float tx,
                                              float ty, float tz, float (&acc)[kYB]) {
     const int Up = U + 2 * kPadDet, Vp = V + 2 * kPadDet;
@@ -188,7 +191,7 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, ctab M, int A, int U,
         float umin = 0.0f, umax = 0.0f, vmin = 0.0f, vmax = 0.0f;
         int ok = 1;
         if (k8 < na) {
-            const ctab m = M + 12 * (a0 + k8);
+            const TAB m = M + 12 * (a0 + k8);
             const float Uh = fmaf(m[0], cx, fmaf(m[2], cz, m[3]));
             const float Wh = fmaf(m[8], cx, fmaf(m[10], cz, m[11]));
             const float Vh = fmaf(m[4], cx, fmaf(m[5], cy, fmaf(m[6], cz, m[7])));
@@ -227,7 +230,7 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, ctab M, int A, int U,
         }
         if (!fits) { // from global memory
             for (int k = 0; k < na; ++k) {
-                const ctab m = M + 12 * (a0 + k); // wave-uniform: scalar loads
+                const TAB m = M + 12 * (a0 + k); // wave-uniform: scalar loads
                 RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
                 const int col = (a0 + k) * panel + kPadDet * Up + kPadDet + r.col;
 #pragma unroll
@@ -258,7 +261,7 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, ctab M, int A, int U,
 #pragma unroll
         for (int k = 0; k < kChunk; ++k) {
             if (k >= na) break;
-            const ctab m = M + 12 * (a0 + k); // wave-uniform: scalar loads
+            const TAB m = M + 12 * (a0 + k); // wave-uniform: scalar loads
             RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
             // LDS element of padded (row pv, column pu): slot + (pv - sv0) * kFootCols + pu - su0
             const int ik = k * kFootSlot + r.col + kPadDet - su0[k] - (sv0[k] - kPadDet) * kFootCols;

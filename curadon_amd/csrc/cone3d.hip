@@ -491,7 +491,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp3d(typename SV::raw *__restrict
     float acc[kZB];
 #pragma unroll
     for (int z = 0; z < kZB; ++z) acc[z] = 0.0f;
-    bp3d_column<SS, MATCHED, FDK>(R, const_table(M), A, U, V, (float)ix - cx, (float)iy - cy,
+    bp3d_column<SS, MATCHED, FDK>(R, M, A, U, V, (float)ix - cx, (float)iy - cy,
                                   (float)iz0 - cz, acc);
 #pragma unroll
     for (int z = 0; z < kZB; ++z) {
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp3d_y(typename SV::raw *__restri
     float acc[kYB];
 #pragma unroll
     for (int y = 0; y < kYB; ++y) acc[y] = 0.0f;
-    bp3d_yrun_lds<SS, MATCHED, FDK>(R, const_table(M), A, U, V, (float)ix - cx, (float)iy0 - cy,
+    bp3d_yrun_lds<SS, MATCHED, FDK>(R, M, A, U, V, (float)ix - cx, (float)iy0 - cy,
                                     (float)iz - cz, (float)ix0 - cx, (float)iy0 - cy,
                                     (float)iz0 - cz, acc);
     if (ix >= nx || iz >= nz) return;

@@ -538,8 +538,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp2d(typename SV::raw *__restrict
     const rsrc_t R = make_rsrc(spad + b * sino_elems, bytes_of<SS>(sino_elems));
     const float x = ((float)ix - (float)nx / 2.0f) * sx + ox + 0.5f;
     const float y = ((float)iy - (float)ny / 2.0f) * sy + oy + 0.5f;
-    const float acc =
-        bp2d_pixel<SS, WPOW>(R, const_table(meta), A, D, x, y, x * inv_du, y * inv_du, half_d);
+    const float acc = bp2d_pixel<SS, WPOW>(R, meta, A, D, x, y, x * inv_du, y * inv_du, half_d);
     const size_t i = b * (size_t)nx * ny + (size_t)iy * nx + ix;
     vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc * inv_du);
 }
