@@ -106,3 +106,31 @@ def test_reference_filter_matches_examples_fbp():
     expect = torch.real(torch.fft.ifft(torch.fft.fft(s, dim=1) * torch.fft.fftshift(filt), dim=1))
     torch.testing.assert_close(F.reference_filter(s), expect * (math.pi / 12), rtol=1e-5,
                                atol=1e-6)
+
+
+def test_ramp_response_is_cached_and_copied():
+    a = F.ramp_response(73, "hann", 0.8)
+    b = F.ramp_response(73, "hann", 0.8)
+    assert torch.equal(a, b) and a.data_ptr() != b.data_ptr()
+    a.zero_()  # a caller's copy: the cached response is untouched
+    assert torch.equal(F.ramp_response(73, "hann", 0.8), b)
+    assert not torch.equal(F.ramp_response(73, "ram-lak", 0.8), b)
+    with pytest.raises(RuntimeError):
+        F.ramp_response(73, "nope")
+
+
+def test_cosine_weights_closed_form():
+    ang = np.linspace(0, 2 * np.pi, 5, endpoint=False)
+    g = FanGeometry(DSD=[24.0, 25.0, 26.0, 27.0, 28.0], DSO=16.0, angles=ang, vol_shape=(8, 8),
+                    det_shape=37, det_spacing=0.75)
+    u = (np.arange(37) - 18.0) * 0.75
+    dsd = np.array([24.0, 25.0, 26.0, 27.0, 28.0])[:, None]
+    np.testing.assert_array_equal(F.cosine_weights_fan(g, "cpu").numpy(),
+                                  (dsd / np.sqrt(dsd ** 2 + u[None, :] ** 2)).astype(np.float32))
+    c = ConeGeometry(DSD=30.0, DSO=20.0, angles=ang[:3], vol_shape=(8, 8, 8), det_shape=(7, 11),
+                     det_spacing=(1.5, 0.75))
+    uu = (np.arange(11) - 5.0) * 0.75
+    vv = (np.arange(7) - 3.0) * 1.5
+    r2 = vv[:, None] ** 2 + uu[None, :] ** 2
+    np.testing.assert_array_equal(F.cosine_weights_cone(c, "cpu").numpy(),
+                                  (30.0 / np.sqrt(30.0 ** 2 + r2)).astype(np.float32))
