@@ -135,6 +135,8 @@ SIGNATURES = {
                                       C.POINTER(C.c_int)]),
     "cmi_stage_projections": (_i32, [_i32, _vp, _u64, _u64, _u64, _u64, C.c_uint32, _vp,
                                      C.c_float, _vp, _i32, _vp]),
+    "cmi_filter_pack": (_i32, [_i32, _vp, _u64, _u64, _vp, _u64, _u64, _vp, _vp]),
+    "cmi_filter_unpack": (_i32, [_i32, _vp, _u64, _u64, _u64, _vp, _vp]),
     "cmi_bp_job_tiles": (_i32, [C.POINTER(BPJob), _u64p]),
     "cmi_server_create": (_i32, [_i32, C.c_uint32, C.POINTER(_vp)]),
     "cmi_server_destroy": (_i32, [_vp]),
@@ -573,6 +575,42 @@ def fan2d_bp_table(angles, DSO, DSD) -> np.ndarray:
                                   dsd.ctypes.data_as(_dp),
                                   out.ctypes.data_as(C.POINTER(C.c_float))))
     return out
+
+
+def filter_pack(src, cw, dst):
+    """dst (ceil(rows / 2), L, 2) FP32 := row pairs of src (rows, D) times their weight rows cw
+    (cw_rows, D), zero padded to L (cmi_filter_pack). Enqueued on the current stream."""
+    for t, name in ((src, "source"), (cw, "weights"), (dst, "destination")):
+        _require_device_tensor(t, name)
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise TypeError(f"{name} must be a contiguous FP32 tensor")
+    D = src.shape[-1]
+    rows = src.numel() // D
+    if cw.shape[-1] != D or dst.dim() != 3 or dst.shape[0] != (rows + 1) // 2 or dst.shape[2] != 2:
+        raise ValueError("filter_pack: shapes do not match")
+    if src.device != cw.device or src.device != dst.device:
+        raise ValueError("filter_pack: tensors on different devices")
+    check(_lib.cmi_filter_pack(dst.device.index or 0, src.data_ptr(), rows, D, cw.data_ptr(),
+                               cw.numel() // D, dst.shape[1], dst.data_ptr(),
+                               torch.cuda.current_stream(dst.device).cuda_stream))
+
+
+def filter_unpack(src, dst):
+    """dst (rows, D) FP32 := the real and imaginary parts of the first D entries of the row
+    pairs in src (ceil(rows / 2), L, 2) FP32 (cmi_filter_unpack). Enqueued on the current
+    stream."""
+    for t, name in ((src, "source"), (dst, "destination")):
+        _require_device_tensor(t, name)
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise TypeError(f"{name} must be a contiguous FP32 tensor")
+    D = dst.shape[-1]
+    rows = dst.numel() // D
+    if src.dim() != 3 or src.shape[0] != (rows + 1) // 2 or src.shape[2] != 2 or src.shape[1] < D:
+        raise ValueError("filter_unpack: shapes do not match")
+    if src.device != dst.device:
+        raise ValueError("filter_unpack: tensors on different devices")
+    check(_lib.cmi_filter_unpack(dst.device.index or 0, src.data_ptr(), rows, D, src.shape[1],
+                                 dst.data_ptr(), torch.cuda.current_stream(dst.device).cuda_stream))
 
 
 def stage_projections(src, dst, row_scale=None, scale=1.0):

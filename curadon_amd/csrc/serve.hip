@@ -85,6 +85,43 @@ __global__ __launch_bounds__(kThreads) void k_stage_projections(
     dst[(a * Vp + vp) * (size_t)Up + up] = from_f32<S>(val);
 }
 
+// Ramp filtering of projection rows through complex FFTs of row pairs (the filter's response
+// is real and even, so the real and imaginary parts of a filtered pair are the filtered rows).
+// Pack: complex row r of length L holds rows 2r and 2r + 1 of `src` times their cosine-weight
+// rows (row i uses weight row i mod cw_rows), zero past D; an odd last row pairs with zeros.
+__global__ __launch_bounds__(kThreads) void k_filter_pack(const float *__restrict__ src,
+                                                          uint64_t rows, int D,
+                                                          const float *__restrict__ cw,
+                                                          uint64_t cw_rows, int L,
+                                                          float2 *__restrict__ dst) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= L) return;
+    const uint64_t pairs = (rows + 1) / 2;
+    for (uint64_t r = blockIdx.y; r < pairs; r += gridDim.y) {
+        const uint64_t i0 = 2 * r, i1 = i0 + 1;
+        float a = 0.0f, b = 0.0f;
+        if (k < D) {
+            a = src[i0 * D + k] * cw[(i0 % cw_rows) * D + k];
+            if (i1 < rows) b = src[i1 * D + k] * cw[(i1 % cw_rows) * D + k];
+        }
+        dst[r * L + k] = make_float2(a, b);
+    }
+}
+
+// Unpack: dst row 2r (2r + 1) = the real (imaginary) part of the first D entries of row r.
+__global__ __launch_bounds__(kThreads) void k_filter_unpack(const float2 *__restrict__ src,
+                                                            uint64_t rows, int D, int L,
+                                                            float *__restrict__ dst) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= D) return;
+    const uint64_t pairs = (rows + 1) / 2;
+    for (uint64_t r = blockIdx.y; r < pairs; r += gridDim.y) {
+        const float2 v = src[r * L + k];
+        dst[2 * r * D + k] = v.x;
+        if (2 * r + 1 < rows) dst[(2 * r + 1) * D + k] = v.y;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_zero_counter2(unsigned *__restrict__ c) {
     if (threadIdx.x < 2) c[threadIdx.x] = 0u;
 }
@@ -335,6 +372,44 @@ CMI_API cmi_status cmi_stage_projections(int device, const float *src, uint64_t 
     if (dst_dtype == CMI_F32) launch(F32{});
     else if (dst_dtype == CMI_F16) launch(F16{});
     else launch(BF16{});
+    CMI_HIP_TRY(hipGetLastError());
+    return CMI_OK;
+}
+
+CMI_API cmi_status cmi_filter_pack(int device, const float *src, uint64_t rows, uint64_t D,
+                                   const float *cw, uint64_t cw_rows, uint64_t L, float *dst,
+                                   void *stream) {
+    clear_error();
+    if (!src || !cw || !dst) return fail(CMI_ERR_INVALID_ARGUMENT, "filter_pack: null pointer");
+    if (rows == 0 || D == 0 || cw_rows == 0 || L < D || L > (1u << 24))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "filter_pack: bad shape");
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(device));
+    CMI_TRY(check_device_ptr(src, device, "source"));
+    CMI_TRY(check_device_ptr(cw, device, "weights"));
+    CMI_TRY(check_device_ptr(dst, device, "destination"));
+    const uint64_t pairs = (rows + 1) / 2;
+    dim3 g((unsigned)((L + kThreads - 1) / kThreads), (unsigned)std::min<uint64_t>(pairs, 65535));
+    hipLaunchKernelGGL(k_filter_pack, g, dim3(kThreads), 0, (hipStream_t)stream, src, rows,
+                       (int)D, cw, cw_rows, (int)L, reinterpret_cast<float2 *>(dst));
+    CMI_HIP_TRY(hipGetLastError());
+    return CMI_OK;
+}
+
+CMI_API cmi_status cmi_filter_unpack(int device, const float *src, uint64_t rows, uint64_t D,
+                                     uint64_t L, float *dst, void *stream) {
+    clear_error();
+    if (!src || !dst) return fail(CMI_ERR_INVALID_ARGUMENT, "filter_unpack: null pointer");
+    if (rows == 0 || D == 0 || L < D || L > (1u << 24))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "filter_unpack: bad shape");
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(device));
+    CMI_TRY(check_device_ptr(src, device, "source"));
+    CMI_TRY(check_device_ptr(dst, device, "destination"));
+    const uint64_t pairs = (rows + 1) / 2;
+    dim3 g((unsigned)((D + kThreads - 1) / kThreads), (unsigned)std::min<uint64_t>(pairs, 65535));
+    hipLaunchKernelGGL(k_filter_unpack, g, dim3(kThreads), 0, (hipStream_t)stream,
+                       reinterpret_cast<const float2 *>(src), rows, (int)D, (int)L, dst);
     CMI_HIP_TRY(hipGetLastError());
     return CMI_OK;
 }

@@ -108,6 +108,51 @@ def filter_rows(p: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
     return torch.fft.irfft(P * H.to(dt), n=L, dim=-1)[..., :D]
 
 
+# Longest FFT run as complex row pairs: rocFFT transforms single-precision rows up to 4096
+# points in one pass through LDS; longer ones take several passes, and a real transform of
+# length L uses a complex one of L / 2.
+PAIR_FFT_MAX = 4096
+
+
+@functools.lru_cache(maxsize=64)
+def _full_response(n_det: int, window, frequency_scaling: float, device) -> torch.Tensor:
+    H = _ramp_response_dev(n_det, window, frequency_scaling, device)
+    return torch.cat([H, H[1:-1].flip(0)]).contiguous()    # length L, real and even
+
+
+def filter_rows_weighted(p: torch.Tensor, cw: torch.Tensor, n_det: int, window="ram-lak",
+                         frequency_scaling: float = 1.0) -> torch.Tensor:
+    """Linear convolution of every row of p * cw (cw's rows repeat over p's rows: (A, D) for
+    a fan beam, (V, U) for a cone beam) with the windowed ramp for rows of n_det cells, for FP32
+    device tensors, through FFTs of length L = fft_length(n_det). Up to L = PAIR_FFT_MAX, rows
+    are filtered in pairs through complex FFTs (the response is real and even, so the real and
+    imaginary parts stay apart), with native kernels for the weighting, padding, pairing and
+    unpairing; longer rows through real FFTs, whose half-length complex transform is then the
+    shorter one. Returns a contiguous FP32 tensor shaped like p."""
+    from . import _lib
+    D = p.shape[-1]
+    p = p.contiguous()
+    cw = cw.to(device=p.device, dtype=torch.float32).contiguous()
+    L = fft_length(int(n_det))
+    if L > PAIR_FFT_MAX:
+        H = _ramp_response_dev(int(n_det), window, float(frequency_scaling), p.device)
+        w = (p.reshape(-1, cw.shape[0], D) * cw).reshape(p.shape)
+        y = torch.fft.irfft(torch.fft.rfft(w, n=L, dim=-1) * H, n=L, dim=-1)
+        return y[..., :D].contiguous()
+    Hf = _full_response(int(n_det), window, float(frequency_scaling), p.device)
+    rows = p.numel() // D
+    z = torch.empty(((rows + 1) // 2, L, 2), dtype=torch.float32, device=p.device)
+    _lib.filter_pack(p, cw, z)
+    Z = torch.fft.fft(torch.view_as_complex(z), dim=-1)
+    del z
+    Z.mul_(Hf)
+    y = torch.view_as_real(torch.fft.ifft(Z, dim=-1))
+    del Z
+    out = torch.empty(p.shape, dtype=torch.float32, device=p.device)
+    _lib.filter_unpack(y, out)
+    return out
+
+
 def cosine_weights_fan(geom: FanGeometry, device=None) -> torch.Tensor:
     """(A, D) cosine pre-weights DSD / sqrt(DSD^2 + u^2), u the cell centre on the detector.
     Evaluated in FP64 on `device` (correctly rounded operations: the same values on any
@@ -128,6 +173,12 @@ def cosine_weights_cone(geom: ConeGeometry, device=None) -> torch.Tensor:
     return (geom.DSD / torch.sqrt(geom.DSD * geom.DSD + r2)).to(torch.float32)
 
 
+def _native_filter(sinogram: torch.Tensor, dt) -> bool:
+    # FP32 device projections go through the native pair filter; FP64 and host tensors
+    # through torch.fft's real transforms
+    return sinogram.is_cuda and dt == torch.float32 and sinogram.dtype == torch.float32
+
+
 def fbp_filter(sinogram: torch.Tensor, geom, window="ram-lak", frequency_scaling=1.0):
     """Cosine-weighted, ramp-filtered and scaled projections (FP32), ready for the weighted
     backprojection (`backward(..., mode="fbp")` for fan beam, mode="fdk" for cone beam)."""
@@ -138,15 +189,21 @@ def fbp_filter(sinogram: torch.Tensor, geom, window="ram-lak", frequency_scaling
             raise NotImplementedError(
                 "fbp: geometries built from ASTRA-like views (from_vec) have no fan-beam FBP "
                 "weighting; use backward / adjoint, or a circular FanGeometry")
-        H = ramp_response(geom.det_shape, window, frequency_scaling, dev)
         cw = cosine_weights_fan(geom, dev).to(dt)
         scale = (math.pi / geom.nangles) * torch.as_tensor(
             geom.DSO / geom.DSD, dtype=dt, device=dev)[:, None]
+        if _native_filter(sinogram, dt):
+            return filter_rows_weighted(sinogram, cw, geom.det_shape, window,
+                                        frequency_scaling) * scale
+        H = ramp_response(geom.det_shape, window, frequency_scaling, dev)
         return filter_rows(sinogram.to(dt) * cw, H) * scale
     if isinstance(geom, ConeGeometry):
-        H = ramp_response(int(geom.det_shape[1]), window, frequency_scaling, dev)
         cw = cosine_weights_cone(geom, dev).to(dt)
         scale = (math.pi / geom.nangles) * geom.DSD / (float(geom.det_spacing[1]) * geom.DSO)
+        if _native_filter(sinogram, dt):
+            return filter_rows_weighted(sinogram, cw, int(geom.det_shape[1]), window,
+                                        frequency_scaling) * scale
+        H = ramp_response(int(geom.det_shape[1]), window, frequency_scaling, dev)
         return filter_rows(sinogram.to(dt) * cw, H) * scale
     raise ValueError("Unknown geometry type")
 

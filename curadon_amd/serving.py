@@ -4,10 +4,10 @@ Jobs (raw FP32 projections + a FanGeometry or ConeGeometry) arrive over time. Th
 
 * admits a job once its HBM footprint fits the remaining budget and an active slot is free
   (first come, first admitted);
-* prepares it on a prep stream: cosine weights, ramp filter through torch.fft (rocFFT) in
-  FP32, scale, and staging of the filtered projections into the zero-bordered layout of the
-  backprojector in the storage precision (FP16 by default); the volume is allocated in the
-  storage precision and stays resident;
+* prepares it on a prep stream: cosine weights, ramp filter (row pairs through complex
+  torch.fft transforms, rocFFT) in FP32, scale, and staging of the filtered projections into
+  the zero-bordered layout of the backprojector in the storage precision (FP16 by default);
+  the volume is allocated in the storage precision and stays resident;
 * backprojects in rounds: each round is one persistent launch of the native multi-job
   backprojector (cmi_serve_backproject) over tiles of every prepared job, at most
   `round_cost` pixel-angle units, shortest remaining work first with a guaranteed share for
@@ -35,7 +35,8 @@ import torch
 
 from . import _lib
 from .geometry import ConeGeometry, FanGeometry
-from .reconstruction import cosine_weights_cone, cosine_weights_fan, fft_length, ramp_response
+from .reconstruction import (cosine_weights_cone, cosine_weights_fan, fft_length,
+                             filter_rows_weighted)
 
 # Relative cost of one voxel-angle of the cone-beam backprojector to one pixel-angle of the
 # fan-beam one (trilinear-free but 2 x 2 taps and a perspective divide per sample).
@@ -88,8 +89,9 @@ class _Active:
 def footprint(job: FBPJob, storage: torch.dtype):
     """(resident, transient) HBM bytes of a job. Resident, held while the job is active:
     staged projections, table and volume. Transient, held while it is prepared: the FP32
-    weighted projections, two spectra and the filtered rows (and the raw projections when
-    they come from the host)."""
+    filtering buffers (packed row pairs, their spectrum, the filtered pairs and the filtered
+    rows; at most two of the first three at once, bounded here by the three), and the raw
+    projections when they come from the host."""
     s = torch.tensor([], dtype=storage).element_size()
     g = job.geom
     if isinstance(g, FanGeometry):
@@ -141,19 +143,17 @@ class FBPServer:
             return hit[1]
         dev = self.device
         if isinstance(g, FanGeometry):
-            H = ramp_response(g.det_shape, window, fs, dev)
             cw = cosine_weights_fan(g, dev)
             scale = ((math.pi / g.nangles) * torch.as_tensor(
                 g.DSO / g.DSD, dtype=torch.float32, device=dev)).contiguous()
             table = torch.from_numpy(_lib.fan2d_bp_table(g.angles, g.DSO, g.DSD)).to(dev)
-            parts = (H, cw, scale, 1.0, table)
+            parts = (cw, scale, 1.0, table)
         else:
-            H = ramp_response(int(g.det_shape[1]), window, fs, dev)
             cw = cosine_weights_cone(g, dev)
             scale = (math.pi / g.nangles) * g.DSD / (float(g.det_spacing[1]) * g.DSO)
             m, yinv = g.plan.bp_matrices("fdk", with_layout=True)
             table = torch.from_numpy(m).to(dev)
-            parts = (H, cw, None, scale, (table, yinv))
+            parts = (cw, None, scale, (table, yinv))
         if len(self._cache) > 256:
             self._cache.clear()
         self._cache[key] = (g, parts)
@@ -166,8 +166,8 @@ class FBPServer:
             raise ValueError("Unknown geometry type")
         dev = self.device
         with torch.cuda.stream(self._prep):
-            H, cw, row_scale, scale, table = self._filter_parts(g, job.window,
-                                                                job.frequency_scaling)
+            cw, row_scale, scale, table = self._filter_parts(g, job.window,
+                                                             job.frequency_scaling)
             ycols = False
             if not fan:
                 table, ycols = table
@@ -177,8 +177,7 @@ class FBPServer:
                 if tuple(p.shape) != (A, D):
                     raise ValueError(f"projections must be {(A, D)}, got {tuple(p.shape)}")
                 ny, nx = (int(v) for v in g.vol_shape)
-                L = 2 * (H.shape[0] - 1)
-                y = torch.fft.irfft(torch.fft.rfft(p * cw, n=L, dim=-1) * H, n=L, dim=-1)
+                y = filter_rows_weighted(p, cw, D, job.window, job.frequency_scaling)
                 staged = torch.empty((A, D + 4), dtype=self.storage, device=dev)
                 keep = [_lib.stage_projections(y, staged, row_scale=row_scale)]
                 volume = torch.zeros((ny, nx), dtype=self.storage, device=dev)
@@ -190,8 +189,7 @@ class FBPServer:
                 if tuple(p.shape) != (A, V, U):
                     raise ValueError(f"projections must be {(A, V, U)}, got {tuple(p.shape)}")
                 nz, ny, nx = (int(v) for v in g.vol_shape)
-                L = 2 * (H.shape[0] - 1)
-                y = torch.fft.irfft(torch.fft.rfft(p * cw, n=L, dim=-1) * H, n=L, dim=-1)
+                y = filter_rows_weighted(p, cw, U, job.window, job.frequency_scaling)
                 staged = torch.empty((A, V + 4, U + 4), dtype=self.storage, device=dev)
                 keep = [_lib.stage_projections(y, staged, scale=scale)]
                 volume = torch.zeros((nz, ny, nx), dtype=self.storage, device=dev)

@@ -278,6 +278,25 @@ CMI_API cmi_status cmi_stage_projections(int device, const float *src, uint64_t 
                                          const float *row_scale, float scale, void *dst,
                                          int dst_dtype, void *stream);
 
+/*
+ * Ramp filtering of projection rows through complex FFTs of row pairs. No counterpart in the
+ * reference, which filters with a circular FFT per row in Python (examples/fbp.py:51-56); these
+ * are the native halves of curadon_amd.reconstruction.filter_rows_weighted. The filter's
+ * response is real and even, so a complex row whose real and imaginary parts are two projection
+ * rows filters both at once.
+ * cmi_filter_pack: dst (complex, [ceil(rows / 2)][L] interleaved FP32) row r := (src[2r] * cw[2r
+ * mod cw_rows], src[2r + 1] * cw[2r + 1 mod cw_rows]) in its first D entries, 0 after; an odd
+ * last row pairs with zeros. src is FP32 [rows][D], cw FP32 [cw_rows][D]; L >= D.
+ * cmi_filter_unpack: dst (FP32 [rows][D]) rows 2r, 2r + 1 := the real and imaginary parts of
+ * the first D entries of src row r (complex [ceil(rows / 2)][L]).
+ * Device memory on `device`, enqueued on `stream`.
+ */
+CMI_API cmi_status cmi_filter_pack(int device, const float *src, uint64_t rows, uint64_t D,
+                                   const float *cw, uint64_t cw_rows, uint64_t L, float *dst,
+                                   void *stream);
+CMI_API cmi_status cmi_filter_unpack(int device, const float *src, uint64_t rows, uint64_t D,
+                                     uint64_t L, float *dst, void *stream);
+
 /* Job kinds of the serving backprojector. */
 typedef enum cmi_job_kind { CMI_JOB_FAN2D = 0, CMI_JOB_CONE3D = 1 } cmi_job_kind;
 

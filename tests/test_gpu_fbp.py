@@ -115,3 +115,23 @@ def test_fdk_pipeline_fp32(oracle, gpu):
     r = rel_l2(got, ref)
     print(f"fdk FP32 pipeline 48^3: rel-L2 {r:.3e}")
     assert r <= 1e-4
+
+
+@pytest.mark.parametrize("shape,cw_rows,window", [((7, 37), 7, "ram-lak"), ((3, 5, 64), 5, "hann"),
+                                                  ((2, 9, 33), 9, "shepp-logan"),
+                                                  ((3, 2100), 3, "ram-lak")])
+def test_pair_filter_matches_real_transforms(gpu, shape, cw_rows, window):
+    """The native pair filter (two rows per complex FFT) against torch.fft's real transforms in
+    FP64 on the same weighted rows: odd and even row counts, batches, windows, and rows long
+    enough (L = 4320) to take the real-transform branch."""
+    g = torch.Generator().manual_seed(3)
+    p = torch.rand(shape, generator=g, dtype=torch.float64) - 0.3
+    D = shape[-1]
+    cw = torch.rand((cw_rows, D), generator=g, dtype=torch.float64) + 0.5
+    rep = p.numel() // D // cw_rows
+    weighted = (p.reshape(rep, cw_rows, D) * cw).reshape(shape)
+    ref = F.filter_rows(weighted, F.ramp_response(D, window, 0.9).double())
+    got = F.filter_rows_weighted(p.float().to(gpu), cw.float().to(gpu), D, window, 0.9)
+    assert got.shape == p.shape and got.dtype == torch.float32
+    err = (got.double().cpu() - ref).norm() / ref.norm()
+    assert err < 1e-6, err
