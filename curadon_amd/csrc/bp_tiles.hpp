@@ -13,7 +13,7 @@ namespace cmi {
 
 constexpr int kPadDet = 2; // zero border of staged sinograms and panels, in cells
 constexpr int kZB = 8;     // z voxels per thread in the 3D backprojector (general geometry)
-constexpr int kYB = 8;     // y voxels per thread in the 3D backprojector (y-invariant U, W)
+constexpr int kYB = 16;    // y voxels per thread in the 3D backprojector (y-invariant U, W)
 
 // Per-angle tables: the bodies take the table pointer type as a template parameter. The plain
 // kernels pass their `const __restrict__` arguments (scalar loads already). The serving kernels
@@ -121,23 +121,27 @@ __device__ __forceinline__ void bp3d_column(rsrc_t R, TAB M, int A, int U,
 // constant step.
 //
 // A workgroup of 256 threads owns a tile of kTileX x kYB x kTileZ voxels (x, y, z); thread t
-// takes the run of kYB y voxels at (x0 + t % kTileX, z0 + t / kTileX). Angles go in chunks of
-// kChunk: every wavefront computes the chunk's panel footprints from the tile's 8 corners (8
+// takes the run of kYB y voxels at (x0 + t % kTileX, z0 + t / kTileX). Angles go in groups of
+// kGroup: every wavefront computes the group's panel footprints from the tile's 8 corners (8
 // lanes per angle; U/W and V/W are linear-fractional, so over a box where W > 0 their extremes
-// are at corners), the workgroup stages the footprints (rows of at most kFootW columns, at most
-// kFootRows rows, from the zero-bordered panel) into LDS and the runs sample from there. A chunk
-// with a footprint that does not fit (or a corner with W <= 0) is sampled from global memory
-// instead, with the same arithmetic.
+// are at corners). Each group goes in chunks of kChunk angles: the workgroup stages the
+// chunk's footprints (rows of at most kFootW columns, at most kFootRows rows, from the
+// zero-bordered panel) into LDS and the runs sample from there. A chunk with a footprint that
+// does not fit (or a corner with W <= 0) is sampled from global memory instead, with the same
+// arithmetic.
 //
-// Per voxel: u = clamp(U/W), v = clamp(V0/W + y dv) with dv = M[a][5]/W, lower taps floor(u),
+// Per voxel: u = clamp(U/W), v = clamp(fma(y, dv, V0/W)) with dv = M[a][5]/W, lower taps floor(u),
 // floor(v) in the staged panel, bilinear weights fract(u), fract(v); MATCHED drops W <= 0, FDK
 // weights by 1/W^2.
 constexpr int kTileX = 16, kTileZ = 16;
 constexpr int kFootW = 64;                        // staged columns per row
 constexpr int kFootCols = 68;                     // LDS row pitch: 16-byte rows, banks shifted
-constexpr int kFootRows = 44;
-constexpr int kFootSlot = kFootCols * kFootRows;  // floats per staged angle
-constexpr int kChunk = 4;                         // angles per chunk: 4 x 12 KB of LDS
+constexpr int kFootRows = 60;
+constexpr int kFootSlot = kFootCols * kFootRows;  // floats per staged angle (16 KB)
+// Angles per chunk. One: the 16 KB of LDS and the body's 80 VGPRs keep 6 workgroups (24
+// wavefronts) per CU; four angles per chunk (64 KB) kept two and ran the 384^3 FDK
+// backprojection at half the speed.
+constexpr int kChunk = 1;
 constexpr int kFootFloats = kChunk * kFootSlot;
 
 // Per-angle set-up of a run: u tap column (element offset of the lower tap's column relative to
@@ -165,9 +169,31 @@ __device__ __forceinline__ RunAngle run_angle(TAB m, float hx, float hy, float h
     return r;
 }
 
-// Staged footprints: one workgroup-shared array (47.9 KB) for every instantiation a kernel
+// Staged footprints: one workgroup-shared array (16 KB) for every instantiation a kernel
 // uses, so a kernel that dispatches between modes allocates it once.
 static __shared__ __align__(16) float foot_lds[kFootFloats];
+
+// Angles whose footprints one pass of a wavefront evaluates: 8 lanes (the tile's corners) each.
+constexpr int kGroup = 8;
+
+// The kYB samples of one run at one angle from its staged footprint. `bk`: byte offset in
+// foot_lds of the lower tap's column in the slot's row 0; `vbase`: the padded-panel row of that
+// slot row minus kPadDet, so the lower tap's slot row is floor(v) - vbase.
+template <bool WEIGHTED>
+__device__ __forceinline__ void yrun_samples(const RunAngle &r, unsigned bk, int vbase,
+                                             float vlim, float (&acc)[kYB]) {
+#pragma unroll
+    for (int y = 0; y < kYB; ++y) {
+        const float v = __builtin_amdgcn_fmed3f(fmaf((float)y, r.dv, r.v), -2.0f, vlim);
+        const unsigned addr = __umul24((unsigned)(floor_i32(v) - vbase), 4u * kFootCols) + bk;
+        const float *p =
+            reinterpret_cast<const float *>(reinterpret_cast<const char *>(foot_lds) + addr);
+        const float r0 = fmaf(r.fu, p[1] - p[0], p[0]);
+        const float r1 = fmaf(r.fu, p[kFootCols + 1] - p[kFootCols], p[kFootCols]);
+        const float s = fmaf(__builtin_amdgcn_fractf(v), r1 - r0, r0);
+        acc[y] = WEIGHTED ? fmaf(s, r.w, acc[y]) : acc[y] + s;
+    }
+}
 
 // (hx, hy, hz): centred index of this thread's run start; (tx, ty, tz): of the tile origin.
 // All 256 threads of the workgroup call it together (lanes without voxels too).
@@ -185,13 +211,15 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, TAB M, int A, int U,
     const float cx = (c8 & 1) ? tx + (float)(kTileX - 1) : tx;
     const float cy = (c8 & 2) ? ty + (float)(kYB - 1) : ty;
     const float cz = (c8 & 4) ? tz + (float)(kTileZ - 1) : tz;
-    for (int a0 = 0; a0 < A; a0 += kChunk) {
-        const int na = min(kChunk, A - a0);
-        // footprints: lanes 8k .. 8k+7 take angle a0 + k
+    // staging: thread tid copies quad tid % 16 of footprint row tid / 16 (and rows + 16 ...)
+    const int qrow = tid >> 4, qcol = (tid & 15) << 2;
+    for (int g0 = 0; g0 < A; g0 += kGroup) {
+        const int ng = min(kGroup, A - g0);
+        // footprints: lanes 8k .. 8k+7 take angle g0 + k
         float umin = 0.0f, umax = 0.0f, vmin = 0.0f, vmax = 0.0f;
         int ok = 1;
-        if (k8 < na) {
-            const TAB m = M + 12 * (a0 + k8);
+        if (k8 < ng) {
+            const TAB m = M + 12 * (g0 + k8);
             const float Uh = fmaf(m[0], cx, fmaf(m[2], cz, m[3]));
             const float Wh = fmaf(m[8], cx, fmaf(m[10], cz, m[11]));
             const float Vh = fmaf(m[4], cx, fmaf(m[5], cy, fmaf(m[6], cz, m[7])));
@@ -208,72 +236,88 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, TAB M, int A, int U,
             vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
             ok = min(ok, __shfl_xor(ok, off, 64));
         }
-        // slot k: padded-panel origin (su0, sv0) and row count; lower taps lie within
+        // every lane now holds its angle's bounds: the slot's padded-panel origin (su0, sv0) and
+        // row count, evaluated once for the group's angles. Lower taps lie within
         // [floor(min) - 1, floor(max) + 1] (one-texel margin for the runs' accumulated v and
-        // rounding) and the upper tap adds one more row / column
-        int su0[kChunk], sv0[kChunk], srows[kChunk];
-        bool fits = true;
+        // rounding) and the upper tap adds one more row / column.
+        const int ulo = floor_i32(umin) - 1 + kPadDet, uhi = floor_i32(umax) + 2 + kPadDet;
+        const int vlo = floor_i32(vmin) - 1 + kPadDet, vhi = floor_i32(vmax) + 2 + kPadDet;
+        const int su0 = max(ulo, 0), sv0 = max(vlo, 0);
+        const int srows = min(vhi, Vp - 1) - sv0 + 1;
+        const uint64_t fitm =
+            __ballot(ok && uhi - ulo + 1 <= kFootW && vhi - vlo + 1 <= kFootRows);
+        // not unrolled: eight copies of the sampling code per mode cost registers and
+        // instruction cache (the readlanes take the uniform lane index from an SGPR)
+#pragma unroll 1
+        for (int c = 0; c < kGroup; c += kChunk) {
+            if (c >= ng) break;
+            const int na = min(kChunk, ng - c);
+            const int a0 = g0 + c;
+            int ssu[kChunk], ssv[kChunk], srw[kChunk];
+            bool fits = true;
 #pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            const float kumin = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(umin), 8 * k));
-            const float kumax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(umax), 8 * k));
-            const float kvmin = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vmin), 8 * k));
-            const float kvmax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vmax), 8 * k));
-            const int kok = __builtin_amdgcn_readlane(ok, 8 * k);
-            const int ulo = (int)floorf(kumin) - 1 + kPadDet, uhi = (int)floorf(kumax) + 2 + kPadDet;
-            const int vlo = (int)floorf(kvmin) - 1 + kPadDet, vhi = (int)floorf(kvmax) + 2 + kPadDet;
-            su0[k] = max(ulo, 0);
-            sv0[k] = max(vlo, 0);
-            srows[k] = min(vhi, Vp - 1) - sv0[k] + 1;
-            if (k < na)
-                fits = fits && kok && uhi - ulo + 1 <= kFootW && vhi - vlo + 1 <= kFootRows;
-        }
-        if (!fits) { // from global memory
-            for (int k = 0; k < na; ++k) {
-                const TAB m = M + 12 * (a0 + k); // wave-uniform: scalar loads
-                RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
-                const int col = (a0 + k) * panel + kPadDet * Up + kPadDet + r.col;
+            for (int k = 0; k < kChunk; ++k) {
+                ssu[k] = __builtin_amdgcn_readlane(su0, 8 * (c + k));
+                ssv[k] = __builtin_amdgcn_readlane(sv0, 8 * (c + k));
+                srw[k] = __builtin_amdgcn_readlane(srows, 8 * (c + k));
+                if (k < na) fits = fits && ((fitm >> (8 * (c + k))) & 1u);
+            }
+            if (!fits) { // from global memory
+                for (int k = 0; k < na; ++k) {
+                    const TAB m = M + 12 * (a0 + k); // wave-uniform: scalar loads
+                    const RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
+                    const int col = (a0 + k) * panel + kPadDet * Up + kPadDet + r.col;
 #pragma unroll
-                for (int y = 0; y < kYB; ++y, r.v += r.dv) {
-                    const float v = __builtin_amdgcn_fmed3f(r.v, -2.0f, vlim);
-                    const int o = col + __mul24(floor_i32(v), Up);
-                    const float r0 = lerp_pair(buf_pair<SS>(R, o, 0), r.fu);
-                    const float r1 = lerp_pair(buf_pair<SS>(R, o, sRow), r.fu);
-                    const float s = fmaf(__builtin_amdgcn_fractf(v), r1 - r0, r0);
-                    acc[y] = (MATCHED || FDK) ? fmaf(s, r.w, acc[y]) : acc[y] + s;
+                    for (int y = 0; y < kYB; ++y) {
+                        const float v =
+                            __builtin_amdgcn_fmed3f(fmaf((float)y, r.dv, r.v), -2.0f, vlim);
+                        const int o = col + __mul24(floor_i32(v), Up);
+                        const float r0 = lerp_pair(buf_pair<SS>(R, o, 0), r.fu);
+                        const float r1 = lerp_pair(buf_pair<SS>(R, o, sRow), r.fu);
+                        const float s = fmaf(__builtin_amdgcn_fractf(v), r1 - r0, r0);
+                        acc[y] = (MATCHED || FDK) ? fmaf(s, r.w, acc[y]) : acc[y] + s;
+                    }
+                }
+                continue;
+            }
+            __syncthreads(); // the previous chunk's samples are read
+            // staging: thread tid copies quad tid % 16 of rows tid / 16 + 16 j of every slot;
+            // the first rows of the chunk's slots with their loads in flight together, then the
+            // rest (more loads in flight raised the body to 120 VGPRs, four wavefronts a SIMD)
+            {
+                Quad first[kChunk];
+#pragma unroll
+                for (int k = 0; k < kChunk; ++k)
+                    if (k < na && qrow < srw[k])
+                        first[k] = buf_quad<SS>(R, (a0 + k) * panel + (ssv[k] + qrow) * Up +
+                                                       ssu[k] + qcol);
+#pragma unroll
+                for (int k = 0; k < kChunk; ++k)
+                    if (k < na && qrow < srw[k])
+                        *reinterpret_cast<float4 *>(
+                            &foot_lds[k * kFootSlot + qrow * kFootCols + qcol]) =
+                            make_float4(first[k].v[0], first[k].v[1], first[k].v[2],
+                                        first[k].v[3]);
+#pragma unroll
+                for (int k = 0; k < kChunk; ++k) {
+                    if (k >= na) break;
+                    const int g = (a0 + k) * panel + ssv[k] * Up + ssu[k] + qcol;
+                    for (int row = qrow + 16; row < srw[k]; row += 16) {
+                        const Quad v = buf_quad<SS>(R, g + row * Up);
+                        *reinterpret_cast<float4 *>(
+                            &foot_lds[k * kFootSlot + row * kFootCols + qcol]) =
+                            make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+                    }
                 }
             }
-            continue;
-        }
-        __syncthreads(); // the previous chunk's samples are read
+            __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            if (k >= na) break;
-            const int g0 = (a0 + k) * panel + sv0[k] * Up + su0[k];
-            for (int i = tid; i < srows[k] * (kFootW / 4); i += 256) {
-                const int row = i >> 4, q = (i & 15) << 2;
-                const Quad v = buf_quad<SS>(R, g0 + row * Up + q);
-                *reinterpret_cast<float4 *>(&foot_lds[k * kFootSlot + row * kFootCols + q]) =
-                    make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kChunk; ++k) {
-            if (k >= na) break;
-            const TAB m = M + 12 * (a0 + k); // wave-uniform: scalar loads
-            RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
-            // LDS element of padded (row pv, column pu): slot + (pv - sv0) * kFootCols + pu - su0
-            const int ik = k * kFootSlot + r.col + kPadDet - su0[k] - (sv0[k] - kPadDet) * kFootCols;
-#pragma unroll
-            for (int y = 0; y < kYB; ++y, r.v += r.dv) {
-                const float v = __builtin_amdgcn_fmed3f(r.v, -2.0f, vlim);
-                const int i = ik + __mul24(floor_i32(v), kFootCols);
-                const float r0 = fmaf(r.fu, foot_lds[i + 1] - foot_lds[i], foot_lds[i]);
-                const float b0 = foot_lds[i + kFootCols], b1 = foot_lds[i + kFootCols + 1];
-                const float r1 = fmaf(r.fu, b1 - b0, b0);
-                const float s = fmaf(__builtin_amdgcn_fractf(v), r1 - r0, r0);
-                acc[y] = (MATCHED || FDK) ? fmaf(s, r.w, acc[y]) : acc[y] + s;
+            for (int k = 0; k < kChunk; ++k) {
+                if (k >= na) break;
+                const TAB m = M + 12 * (a0 + k); // wave-uniform: scalar loads
+                const RunAngle r = run_angle<MATCHED, FDK>(m, hx, hy, hz, ulim);
+                const unsigned bk = 4u * (unsigned)(k * kFootSlot + r.col + kPadDet - ssu[k]);
+                yrun_samples<MATCHED || FDK>(r, bk, ssv[k] - kPadDet, vlim, acc);
             }
         }
     }

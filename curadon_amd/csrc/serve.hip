@@ -253,9 +253,11 @@ __global__ __launch_bounds__(kThreads) void k_serve_bp2d(const DevJob *__restric
     }
 }
 
-// Cone-beam tiles.
-template <class SV, class SS>
-__global__ __launch_bounds__(kThreads) void k_serve_bp3d(const DevJob *__restrict__ jobs,
+// Cone-beam tiles. YRUNS: every job of the launch takes the y-run body (columns == 1), and the
+// general column body is left out. Five wavefronts per SIMD: the persistent loop around the
+// body's three modes took 97 VGPRs (four wavefronts per SIMD); 96 fit without spills, 80 spilled.
+template <class SV, class SS, bool YRUNS>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_serve_bp3d(const DevJob *__restrict__ jobs,
                                                          int njobs, uint32_t total,
                                                          unsigned *__restrict__ counter) {
     __shared__ unsigned s_tile;
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(kThreads) void k_serve_bp3d(const DevJob *__restric
         if (t >= total) return;
         const DevJob &J = job_of(jobs, njobs, t);
         const uint32_t lt = t - J.first + J.tile_begin;
-        if (J.columns == 1) serve_tile_3d_y<SV, SS>(J, lt);
+        if (YRUNS || J.columns == 1) serve_tile_3d_y<SV, SS>(J, lt);
         else serve_tile_3d<SV, SS>(J, lt);
     }
 }
@@ -330,7 +332,8 @@ cmi_status check_job(const cmi_bp_job &j, uint32_t i, int sino_dtype) {
 struct cmi_server {
     int device = 0;
     uint32_t max_jobs = 0;
-    uint32_t grid = 0;   // resident workgroups of the cone-beam serving kernel
+    uint32_t grid = 0;   // resident workgroups of the cone-beam serving kernel (any job)
+    uint32_t grid3y = 0; // resident workgroups of the cone-beam serving kernel (y runs only)
     uint32_t grid2d = 0; // resident workgroups of the fan-beam serving kernel
     DevJob *d_jobs = nullptr;
     unsigned *d_counter = nullptr;
@@ -435,12 +438,15 @@ CMI_API cmi_status cmi_server_create(int device, uint32_t max_jobs, cmi_server *
     p->device = device;
     p->max_jobs = max_jobs;
     CMI_HIP_TRY(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, device));
-    int per_cu = 0, per_cu2 = 0;
-    CMI_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_serve_bp3d<F32, F32>,
-                                                             kThreads, 0));
+    int per_cu = 0, per_cu3y = 0, per_cu2 = 0;
+    CMI_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, k_serve_bp3d<F32, F32, false>, kThreads, 0));
+    CMI_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu3y, k_serve_bp3d<F32, F32, true>, kThreads, 0));
     CMI_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, k_serve_bp2d<F32, F32>,
                                                              kThreads, 0));
     p->grid = (uint32_t)std::max(1, per_cu) * (uint32_t)p->cus;
+    p->grid3y = (uint32_t)std::max(1, per_cu3y) * (uint32_t)p->cus;
     p->grid2d = (uint32_t)std::max(1, per_cu2) * (uint32_t)p->cus;
     struct Cleanup {
         cmi_server *p;
@@ -518,7 +524,11 @@ CMI_API cmi_status cmi_serve_backproject(cmi_server *p, const cmi_bp_job *jobs, 
     // launch-tile numbering (the two kinds run in separate launches)
     DevJob *h = p->h_jobs + (size_t)slot * p->max_jobs;
     uint32_t n2 = 0, total2 = 0, total3 = 0;
-    for (uint32_t i = 0; i < njobs; ++i) n2 += jobs[i].kind == CMI_JOB_FAN2D ? 1u : 0u;
+    bool yruns = true; // every cone-beam job takes the y-run body
+    for (uint32_t i = 0; i < njobs; ++i) {
+        n2 += jobs[i].kind == CMI_JOB_FAN2D ? 1u : 0u;
+        if (jobs[i].kind != CMI_JOB_FAN2D && jobs[i].columns != 1) yruns = false;
+    }
     uint32_t k2 = 0, k3 = n2;
     for (uint32_t i = 0; i < njobs; ++i) {
         const cmi_bp_job &j = jobs[i];
@@ -566,8 +576,10 @@ CMI_API cmi_status cmi_serve_backproject(cmi_server *p, const cmi_bp_job *jobs, 
         using SV = decltype(sv);
         using SS = decltype(ss);
         if (total3) {
-            const unsigned grid = (unsigned)std::min<uint64_t>(total3, p->grid);
-            hipLaunchKernelGGL((k_serve_bp3d<SV, SS>), dim3(grid), dim3(kThreads), 0, s,
+            const unsigned grid =
+                (unsigned)std::min<uint64_t>(total3, yruns ? p->grid3y : p->grid);
+            auto kern = yruns ? k_serve_bp3d<SV, SS, true> : k_serve_bp3d<SV, SS, false>;
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, s,
                                (const DevJob *)(dj + n2), (int)(njobs - n2), total3,
                                counter + 1);
             CMI_HIP_TRY(hipGetLastError());

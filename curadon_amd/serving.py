@@ -39,10 +39,12 @@ from .reconstruction import (cosine_weights_cone, cosine_weights_fan, fft_length
                              filter_rows_weighted)
 
 # Relative cost of one voxel-angle of the cone-beam backprojector to one pixel-angle of the
-# fan-beam one (trilinear-free but 2 x 2 taps and a perspective divide per sample).
-CONE_COST = 2.0
+# fan-beam one (2 x 2 taps from LDS-staged footprints against 2 taps from the sinogram; measured
+# 2.1e12 voxel-angles/s at 384^3 against 2.35e12 pixel-angles/s at 2048^2).
+CONE_COST = 1.15
 PIXELS_PER_TILE_2D = 64 * 4
-VOXELS_PER_TILE_3D = 16 * 16 * 8
+# voxels per tile of the cone-beam kernel: y runs (16 x 16 x 16) / z columns (16 x 16 x 8)
+VOXELS_PER_TILE_3D = {True: 16 * 16 * 16, False: 16 * 16 * 8}
 
 
 @dataclass
@@ -181,7 +183,7 @@ class FBPServer:
                 staged = torch.empty((A, D + 4), dtype=self.storage, device=dev)
                 keep = [_lib.stage_projections(y, staged, row_scale=row_scale)]
                 volume = torch.zeros((ny, nx), dtype=self.storage, device=dev)
-                ntiles = ((nx + 63) // 64) * ((ny + 3) // 4)
+                ntiles = _lib.bp_job_tiles(_lib.BPJob(kind=_lib.JOB_FAN2D, nx=nx, ny=ny))
                 tile_cost = float(PIXELS_PER_TILE_2D * A)
             else:
                 A = g.nangles
@@ -193,11 +195,9 @@ class FBPServer:
                 staged = torch.empty((A, V + 4, U + 4), dtype=self.storage, device=dev)
                 keep = [_lib.stage_projections(y, staged, scale=scale)]
                 volume = torch.zeros((nz, ny, nx), dtype=self.storage, device=dev)
-                if ycols:  # runs of 8 y voxels, tiles of 16 x 8 x 16
-                    ntiles = ((nx + 15) // 16) * ((ny + 7) // 8) * ((nz + 15) // 16)
-                else:      # columns of 8 z voxels, tiles of 16 x 16 x 8
-                    ntiles = ((nx + 15) // 16) * ((ny + 15) // 16) * ((nz + 7) // 8)
-                tile_cost = float(VOXELS_PER_TILE_3D * A * CONE_COST)
+                ntiles = _lib.bp_job_tiles(_lib.BPJob(kind=_lib.JOB_CONE3D, columns=int(ycols),
+                                                      nx=nx, ny=ny, nz=nz))
+                tile_cost = float(VOXELS_PER_TILE_3D[bool(ycols)] * A * CONE_COST)
             ev = torch.cuda.Event()
             ev.record(self._prep)
         job.volume = volume

@@ -304,7 +304,7 @@ typedef enum cmi_job_kind { CMI_JOB_FAN2D = 0, CMI_JOB_CONE3D = 1 } cmi_job_kind
  * One backprojection job of a multi-job launch: volume += B sinogram over the tile range
  * [tile_begin, tile_end) of the job's volume. Fan-beam tiles are 64 x 4 pixels (row-major
  * over ceil(nx / 64) x ceil(ny / 4)). Cone-beam tiles are 16 x 16 x 8 voxels (x fastest, then
- * y, then z blocks) with columns = 0, or 16 x 8 x 16 (x, y, z; x fastest, then y, then z
+ * y, then z blocks) with columns = 0, or 16 x 16 x 16 (x, y, z; x fastest, then y, then z
  * blocks) with columns = 1, which the caller may set only for y_invariant matrices. The
  * arithmetic of every voxel equals cmi_backward_2d_weighted / cmi_backward_3d for the same
  * tables (cmi_backward_3d runs along y exactly when the matrices are y_invariant), so a job's

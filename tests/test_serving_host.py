@@ -140,5 +140,8 @@ def test_y_invariance_flag():
 
 
 def test_job_tiles_y_runs():
+    # tiles of 16 (x) x 16 (y) x 16 (z) voxels
     j = _lib.BPJob(kind=_lib.JOB_CONE3D, columns=1, nx=33, ny=16, nz=17)
-    assert _lib.bp_job_tiles(j) == 3 * 2 * 2
+    assert _lib.bp_job_tiles(j) == 3 * 1 * 2
+    j = _lib.BPJob(kind=_lib.JOB_CONE3D, columns=1, nx=16, ny=17, nz=48)
+    assert _lib.bp_job_tiles(j) == 1 * 2 * 3
