@@ -9,9 +9,10 @@ along detector rows with the band-limited ramp h of unit cell spacing (Kak and S
 "Principles of Computerized Tomographic Imaging", eq. 3.61: h[0] = 1/4, h[k odd] =
 -1/(pi k)^2, h[k even] = 0) times a frequency window, B_fbp the 2D backprojector with the
 squared magnification weight and B_fdk the geometry-matched 3D backprojector with the
-(DSO / depth)^2 weight. The convolution runs through an FFT of length >= 2 D (zero padded, so
-it is linear, not circular; the length has only the prime factors 2, 3 and 5). The result is
-in the image's density units.
+(DSO / depth)^2 weight. The convolution is that of an FFT of length >= 2 D (zero padded, so
+it is linear, not circular; the length has only the prime factors 2, 3 and 5), evaluated on
+the device for rows of up to 1024 cells as a product with the filter's Toeplitz matrix. The
+result is in the image's density units.
 
 Windows follow examples/fbp.py:27-48 over the normalised frequency |W| in [0, 1] (1 = Nyquist):
 ram-lak 1, shepp-logan sinc(W / (2 s)), cosine cos(W pi / (2 s)) (the reference multiplies by
@@ -68,6 +69,11 @@ def fft_length(n_det: int) -> int:
 
 @functools.lru_cache(maxsize=64)
 def _ramp_response_host(n_det: int, window, frequency_scaling: float) -> torch.Tensor:
+    return _ramp_response_f64(n_det, window, frequency_scaling).to(torch.float32)
+
+
+@functools.lru_cache(maxsize=16)
+def _ramp_response_f64(n_det: int, window, frequency_scaling: float) -> torch.Tensor:
     L = fft_length(n_det)
     k = torch.arange(L, dtype=torch.float64)
     m = torch.where(k <= L // 2, k, k - L)                   # signed index, FFT order
@@ -77,8 +83,7 @@ def _ramp_response_host(n_det: int, window, frequency_scaling: float) -> torch.T
     h[odd] = -1.0 / (math.pi * m[odd]) ** 2
     H = torch.fft.rfft(h).real                               # symmetric kernel: real response
     W = torch.arange(L // 2 + 1, dtype=torch.float64) / (L // 2)
-    H = H * _window(W, window, frequency_scaling)
-    return H.to(torch.float32)
+    return H * _window(W, window, frequency_scaling)
 
 
 @functools.lru_cache(maxsize=64)
@@ -114,6 +119,27 @@ def filter_rows(p: torch.Tensor, H: torch.Tensor) -> torch.Tensor:
 PAIR_FFT_MAX = 4096
 
 
+# Rows of at most this many cells are filtered as one FP32 matrix product with the D x D
+# Toeplitz matrix of the filter (a library GEMM on the matrix cores): 2 D multiply-adds per
+# output instead of the FFTs' five passes over HBM. Measured on MI355X for (rows, D):
+# (393216, 384) 0.58 ms against 1.78 ms by pair FFTs, (442368, 576) 2.8 against 6.0 ms,
+# (2048, 1449) even, (4096, 2897) 0.77 against 0.45 ms.
+TOEPLITZ_MAX = 1024
+
+
+@functools.lru_cache(maxsize=16)
+def _toeplitz(n_det: int, window, frequency_scaling: float, device) -> torch.Tensor:
+    """(n_det, n_det) FP32 matrix T[j, k] = h[(k - j) mod L], h the impulse response of the
+    windowed ramp (FP64 inverse transform of its length-L response): a row x times T is the
+    first n_det entries of the circular convolution of x, zero-padded to L = fft_length(n_det),
+    with h, which is the linear convolution filter_rows computes."""
+    L = fft_length(n_det)
+    h = torch.fft.irfft(_ramp_response_f64(n_det, window, frequency_scaling), n=L)
+    j = torch.arange(n_det)[:, None]
+    k = torch.arange(n_det)[None, :]
+    return h[(k - j) % L].to(torch.float32).to(device).contiguous()
+
+
 @functools.lru_cache(maxsize=64)
 def _full_response(n_det: int, window, frequency_scaling: float, device) -> torch.Tensor:
     H = _ramp_response_dev(n_det, window, frequency_scaling, device)
@@ -124,15 +150,20 @@ def filter_rows_weighted(p: torch.Tensor, cw: torch.Tensor, n_det: int, window="
                          frequency_scaling: float = 1.0) -> torch.Tensor:
     """Linear convolution of every row of p * cw (cw's rows repeat over p's rows: (A, D) for
     a fan beam, (V, U) for a cone beam) with the windowed ramp for rows of n_det cells, for FP32
-    device tensors, through FFTs of length L = fft_length(n_det). Up to L = PAIR_FFT_MAX, rows
-    are filtered in pairs through complex FFTs (the response is real and even, so the real and
-    imaginary parts stay apart), with native kernels for the weighting, padding, pairing and
-    unpairing; longer rows through real FFTs, whose half-length complex transform is then the
-    shorter one. Returns a contiguous FP32 tensor shaped like p."""
+    device tensors. Rows of at most TOEPLITZ_MAX cells: one FP32 product with the filter's
+    Toeplitz matrix. Longer rows: FFTs of length L = fft_length(n_det); up to L = PAIR_FFT_MAX
+    in pairs through complex FFTs (the response is real and even, so the real and imaginary
+    parts stay apart), with native kernels for the weighting, padding, pairing and unpairing;
+    longer ones through real FFTs, whose half-length complex transform is then the shorter one.
+    Returns a contiguous FP32 tensor shaped like p."""
     from . import _lib
     D = p.shape[-1]
     p = p.contiguous()
     cw = cw.to(device=p.device, dtype=torch.float32).contiguous()
+    if D == int(n_det) and D <= TOEPLITZ_MAX:
+        T = _toeplitz(D, window, float(frequency_scaling), p.device)
+        w = (p.reshape(-1, cw.shape[0], D) * cw).reshape(-1, D)
+        return torch.mm(w, T).reshape(p.shape)
     L = fft_length(int(n_det))
     if L > PAIR_FFT_MAX:
         H = _ramp_response_dev(int(n_det), window, float(frequency_scaling), p.device)

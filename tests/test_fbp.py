@@ -99,6 +99,26 @@ def test_filter_rows_is_linear_convolution():
     np.testing.assert_allclose(got, expect, atol=1e-6)
 
 
+def test_toeplitz_filter_is_linear_convolution():
+    """The Toeplitz matrix of the GPU filter path against numpy's linear convolution with the
+    closed-form unwindowed kernel (h[0] = 1/4, h[odd k] = -1/(pi k)^2), and against filter_rows
+    in FP64 for windowed filters."""
+    r = np.random.default_rng(1)
+    p = r.random((3, 37))
+    got = (torch.tensor(p, dtype=torch.float32) @ F._toeplitz(37, None, 1.0, torch.device("cpu")))
+    k = np.arange(-36, 37)
+    h = np.zeros(k.size)
+    h[k == 0] = 0.25
+    h[k % 2 != 0] = -1.0 / (np.pi * k[k % 2 != 0]) ** 2
+    expect = np.stack([np.convolve(row, h, mode="full")[36:36 + 37] for row in p])
+    np.testing.assert_allclose(got.double().numpy(), expect, atol=1e-6)
+    for D, win, s in ((64, "hann", 0.9), (181, "shepp-logan", 1.0), (100, "cosine", 0.5)):
+        x = torch.tensor(r.random((5, D)))
+        y = x.float() @ F._toeplitz(D, win, s, torch.device("cpu"))
+        ref = F.filter_rows(x, F.ramp_response(D, win, s).double())
+        assert ((y.double() - ref).norm() / ref.norm()).item() < 1e-6
+
+
 def test_reference_filter_matches_examples_fbp():
     r = np.random.default_rng(1)
     s = torch.tensor(r.random((6, 41)), dtype=torch.float32)

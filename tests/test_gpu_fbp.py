@@ -119,11 +119,14 @@ def test_fdk_pipeline_fp32(oracle, gpu):
 
 @pytest.mark.parametrize("shape,cw_rows,window", [((7, 37), 7, "ram-lak"), ((3, 5, 64), 5, "hann"),
                                                   ((2, 9, 33), 9, "shepp-logan"),
+                                                  ((5, 1100), 5, "hamming"),
+                                                  ((3, 1025), 1, "cosine"),
                                                   ((3, 2100), 3, "ram-lak")])
 def test_pair_filter_matches_real_transforms(gpu, shape, cw_rows, window):
-    """The native pair filter (two rows per complex FFT) against torch.fft's real transforms in
-    FP64 on the same weighted rows: odd and even row counts, batches, windows, and rows long
-    enough (L = 4320) to take the real-transform branch."""
+    """The device filter against torch.fft's real transforms in FP64 on the same weighted rows:
+    the Toeplitz product (rows up to 1024 cells), the native pair filter (two rows per complex
+    FFT; L = 2250, 2160) with odd and even row counts, batches and windows, and rows long enough
+    (L = 4320) to take the real-transform branch."""
     g = torch.Generator().manual_seed(3)
     p = torch.rand(shape, generator=g, dtype=torch.float64) - 0.3
     D = shape[-1]
