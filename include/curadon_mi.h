@@ -341,8 +341,10 @@ typedef struct cmi_server cmi_server;
 CMI_API cmi_status cmi_server_create(int device, uint32_t max_jobs, cmi_server **server);
 CMI_API cmi_status cmi_server_destroy(cmi_server *server);
 
-/* Workgroups of the serving kernel the device holds resident at once (the persistent grid
- * of a launch with at least that many tiles). */
+/* Workgroups of the cone-beam serving kernel for jobs of either tile layout that the device
+ * holds resident at once (the persistent grid of a launch with at least that many cone-beam
+ * tiles). Launches whose cone-beam jobs all take y runs, and the fan-beam tiles, use kernels of
+ * their own with their own resident counts. */
 CMI_API cmi_status cmi_server_get_capacity(const cmi_server *server, uint32_t *workgroups);
 
 /*
