@@ -263,12 +263,28 @@ def cpu_baseline(np, geom_rec):
     t2 = time.perf_counter()
     t_fwd = (t1 - t0) * w["A"] / fa
     t_bwd = (t2 - t1) * w["n"] / fr
+    cores = oracle.num_threads()
+    # one core: a smaller sample of the same operators
+    fa1, fr1 = 16, 32
+    oracle.set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        oracle.fwd2d_f32(geom_rec, img, angles=(0, fa1))
+        t1 = time.perf_counter()
+        oracle.bp2d_f32(geom_rec, sino, rows=(0, fr1))
+        t2 = time.perf_counter()
+    finally:
+        oracle.set_num_threads(cores)
+    t_fwd1 = (t1 - t0) * w["A"] / fa1
+    t_bwd1 = (t2 - t1) * w["n"] / fr1
     return {
         "value": 1.0 / (t_fwd + t_bwd), "unit": "fwd+bwd pairs/s",
-        "cores": oracle.num_threads(), "kind": "port",
+        "cores": cores, "kind": "port",
         "sample": f"FP32 port of the reference arithmetic, forward on {fa}/{w['A']} angles "
-                  f"and backprojection on {fr}/{w['n']} rows, scaled to full operators",
+                  f"and backprojection on {fr}/{w['n']} rows, scaled to full operators; one "
+                  f"core: {fa1} angles and {fr1} rows",
         "fwd_s": t_fwd, "bwd_s": t_bwd, "cpu_model": _cpu_model(),
+        "single_core": {"value": 1.0 / (t_fwd1 + t_bwd1), "fwd_s": t_fwd1, "bwd_s": t_bwd1},
     }
 
 

@@ -34,6 +34,14 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ---------------------------------------------------------------- interpolation (A.0) -- */
 
 /* Texel fetch with border addressing: 0 outside [0, n). */

@@ -81,8 +81,10 @@ void orc_bp2d_w(const orc_fan2d *g, const double *sino, double *img, int64_t y0,
 void orc_fwd2d_f32(const orc_fan2d *g, const float *img, float *sino, int64_t a0, int64_t a1);
 void orc_bp2d_f32(const orc_fan2d *g, const float *sino, float *img, int64_t y0, int64_t y1);
 
-/* OpenMP threads the parallel routines use (1 when built without OpenMP). */
+/* OpenMP threads the parallel routines use (1 when built without OpenMP), and their setter
+ * (n > 0; no effect without OpenMP). */
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 /* Exact-weight interpolation helpers (texture semantics), exported for known-answer tests. */
 double orc_lerp1(const double *row, int64_t n, double u);

@@ -67,6 +67,7 @@ def lib():
             "orc_fwd2d_f32": (None, [F2, _fp, _fp, _i64, _i64]),
             "orc_bp2d_f32": (None, [F2, _fp, _fp, _i64, _i64]),
             "orc_num_threads": (C.c_int, []),
+            "orc_set_num_threads": (None, [C.c_int]),
             "orc_lerp1": (C.c_double, [_dp, _i64, C.c_double]),
             "orc_bilerp": (C.c_double, [_dp, _i64, _i64, C.c_double, C.c_double]),
             "orc_trilerp": (C.c_double, [_dp, _i64, _i64, _i64] + [C.c_double] * 3),
@@ -90,6 +91,10 @@ def lib():
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
 
 
 def _d(a):
