@@ -163,7 +163,14 @@ def filter_rows_weighted(p: torch.Tensor, cw: torch.Tensor, n_det: int, window="
     if D == int(n_det) and D <= TOEPLITZ_MAX:
         T = _toeplitz(D, window, float(frequency_scaling), p.device)
         w = (p.reshape(-1, cw.shape[0], D) * cw).reshape(-1, D)
-        return torch.mm(w, T).reshape(p.shape)
+        # full FP32 products whatever the caller's TF32 setting
+        tf32 = torch.backends.cuda.matmul.allow_tf32
+        torch.backends.cuda.matmul.allow_tf32 = False
+        try:
+            y = torch.mm(w, T)
+        finally:
+            torch.backends.cuda.matmul.allow_tf32 = tf32
+        return y.reshape(p.shape)
     L = fft_length(int(n_det))
     if L > PAIR_FFT_MAX:
         H = _ramp_response_dev(int(n_det), window, float(frequency_scaling), p.device)

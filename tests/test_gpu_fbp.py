@@ -138,3 +138,20 @@ def test_pair_filter_matches_real_transforms(gpu, shape, cw_rows, window):
     assert got.shape == p.shape and got.dtype == torch.float32
     err = (got.double().cpu() - ref).norm() / ref.norm()
     assert err < 1e-6, err
+
+
+def test_toeplitz_filter_ignores_tf32_setting(gpu):
+    """The Toeplitz product stays in full FP32 when the caller allows TF32 matmuls: the same
+    bits as with TF32 off, and the caller's setting is restored."""
+    g = torch.Generator().manual_seed(5)
+    p = (torch.rand((6, 300), generator=g) - 0.5).to(gpu)
+    cw = (torch.rand((6, 300), generator=g) + 0.5).to(gpu)
+    ref = F.filter_rows_weighted(p, cw, 300)
+    old = torch.backends.cuda.matmul.allow_tf32
+    torch.backends.cuda.matmul.allow_tf32 = True
+    try:
+        got = F.filter_rows_weighted(p, cw, 300)
+        assert torch.backends.cuda.matmul.allow_tf32 is True
+    finally:
+        torch.backends.cuda.matmul.allow_tf32 = old
+    assert torch.equal(got, ref)
