@@ -238,8 +238,8 @@ __device__ __forceinline__ void bp3d_yrun_lds(rsrc_t R, TAB M, int A, int U,
         }
         // every lane now holds its angle's bounds: the slot's padded-panel origin (su0, sv0) and
         // row count, evaluated once for the group's angles. Lower taps lie within
-        // [floor(min) - 1, floor(max) + 1] (one-texel margin for the runs' accumulated v and
-        // rounding) and the upper tap adds one more row / column.
+        // [floor(min) - 1, floor(max) + 1] (one-texel margin for the rounding of the samples'
+        // u and v against the corners') and the upper tap adds one more row / column.
         const int ulo = floor_i32(umin) - 1 + kPadDet, uhi = floor_i32(umax) + 2 + kPadDet;
         const int vlo = floor_i32(vmin) - 1 + kPadDet, vhi = floor_i32(vmax) + 2 + kPadDet;
         const int su0 = max(ulo, 0), sv0 = max(vlo, 0);
