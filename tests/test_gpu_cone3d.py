@@ -239,3 +239,21 @@ def test_circular_orbit_odd_shapes(oracle, gpu):
     yb = np.stack([y, 0.5 * y])
     out = host(cm.backward(dev(yb, gpu), geom, mode="fdk"))
     check_close(out[1], 0.5 * oracle.bp3d(rec, y, fdk=True), "circular odd-shape bp3d batch 1")
+
+
+@pytest.mark.parametrize("dso,maxabs", [(34.0, 2e-5), (30.0, 5e-5)])
+def test_circular_orbit_high_magnification(oracle, gpu, dso, maxabs):
+    """Circular orbit with the source just outside the volume's bounding circle (radius 28.3
+    for a 40-voxel-wide volume; DSD 60): tiles near the source have panel footprints far wider
+    and taller than the staged LDS slot (64 x 60 cells) and are sampled from global memory,
+    tiles far from it are staged; both against the oracle, in every mode. At DSO 30 the voxels
+    next to the source sit 1.7 units from it: magnification about 35 and FDK weight
+    (DSO / depth)^2 about 300 amplify the FP32 rounding of their panel coordinate, so the
+    max-abs bound there is 5e-5 (rel-L2 stays below 1e-5)."""
+    rec = cone(40, 96, 88, 12, dso, 60.0, ny=36)
+    geom = as_cone_geometry(rec)
+    y = np.random.default_rng(21).random((12, 88, 96))
+    for mode in ("reference", "matched", "fdk"):
+        ref = oracle.bp3d(rec, y, matched=mode == "matched", fdk=mode == "fdk")
+        check_close(host(cm.backward(dev(y, gpu), geom, mode=mode)), ref,
+                    f"high-magnification (DSO {dso}) bp3d {mode}", maxabs=maxabs)
