@@ -151,21 +151,54 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage3d_t(const typename S::raw *
     }
 }
 
-// ppad[b][a][vp][up] = sino[b][a][vp-2][up-2], 0 in the border.
+// ppad[b][a][vp][up] = sino[b][a][vp-2][up-2], 0 in the border. Flat over the n destination
+// elements, four consecutive ones per thread with one vector store (its offset is a multiple
+// of four elements); the source row is followed element by element.
 template <class S>
 __global__ __launch_bounds__(kMaxBlock) void k_stage_panel(const typename S::raw *__restrict__ sino,
                                                            typename S::raw *__restrict__ ppad,
-                                                           int U, int V, int A) {
+                                                           int U, int V, uint64_t n) {
     using raw = typename S::raw;
+    struct alignas(4 * sizeof(raw)) Four {
+        raw v[4];
+    };
+    const uint64_t i0 = ((uint64_t)blockIdx.x * kMaxBlock + threadIdx.x) * 4;
+    if (i0 >= n) return;
     const int Up = U + 2 * kPad, Vp = V + 2 * kPad;
-    const int up = blockIdx.x * blockDim.x + threadIdx.x;
-    if (up >= Up) return;
-    const int vp = blockIdx.y;
-    const size_t ab = blockIdx.z; // b * A + a
-    const int u = up - kPad, v = vp - kPad;
-    raw val = (raw)0.0f;
-    if (u >= 0 && u < U && v >= 0 && v < V) val = sino[(ab * V + v) * (size_t)U + u];
-    ppad[(ab * Vp + vp) * (size_t)Up + up] = val;
+    int up, vp;
+    uint64_t ab;
+    if (n <= 0xffffffffull) { // 32-bit divisions
+        const uint32_t j = (uint32_t)i0, r0 = j / (uint32_t)Up; // (b * A + a) * Vp + vp
+        up = (int)(j - r0 * (uint32_t)Up);
+        vp = (int)(r0 % (uint32_t)Vp);
+        ab = r0 / (uint32_t)Vp;
+    } else {
+        const uint64_t r0 = i0 / (uint64_t)Up;
+        up = (int)(i0 - r0 * Up);
+        vp = (int)(r0 % (uint64_t)Vp);
+        ab = r0 / (uint64_t)Vp;
+    }
+    Four f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int u = up - kPad, v = vp - kPad;
+        raw val = (raw)0.0f;
+        if (i0 + k < n && u >= 0 && u < U && v >= 0 && v < V)
+            val = sino[(ab * V + v) * (size_t)U + u];
+        f.v[k] = val;
+        if (++up == Up) { // next destination row
+            up = 0;
+            if (++vp == Vp) {
+                vp = 0;
+                ++ab;
+            }
+        }
+    }
+    if (i0 + 4 <= n) {
+        *reinterpret_cast<Four *>(ppad + i0) = f;
+    } else {
+        for (uint64_t k = 0; i0 + k < n; ++k) ppad[i0 + k] = f.v[k];
+    }
 }
 
 // ---------------------------------------------------------------- forward -----------------
@@ -950,9 +983,9 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
         using SV = decltype(sv);
         using SS = decltype(ss);
         using RS = typename SS::raw;
-        dim3 pb(256), pg((unsigned)((Up + 255) / 256), (unsigned)Vp, (unsigned)(A * batch));
-        hipLaunchKernelGGL((k_stage_panel<SS>), pg, pb, 0, s, (const RS *)sinogram,
-                           (RS *)scratch, U, V, A);
+        const uint64_t n = (uint64_t)Up * Vp * A * batch;
+        hipLaunchKernelGGL((k_stage_panel<SS>), dim3((unsigned)((n + 4 * 256 - 1) / (4 * 256))),
+                           dim3(256), 0, s, (const RS *)sinogram, (RS *)scratch, U, V, n);
         if (plan->ycols[mode]) {
             const int nzt = (nz + kTileZ - 1) / kTileZ;
             dim3 bb(kTileX * kTileZ), bg((unsigned)((nx + kTileX - 1) / kTileX),
