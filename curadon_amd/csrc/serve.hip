@@ -67,22 +67,57 @@ uint64_t tiles_of(const cmi_bp_job &j) {
 
 // ---------------------------------------------------------------- staging -----------------
 
+// dst[a][vp][up] = scale * row_scale[a] * src[a][vp - vpad][up - 2], 0 in the border. Flat
+// over the n destination elements, four consecutive ones per thread with one vector store (its
+// offset is a multiple of four elements); the source row is followed element by element.
 template <class S>
 __global__ __launch_bounds__(kThreads) void k_stage_projections(
     const float *__restrict__ src, uint64_t stride, int V, int U, int vpad,
-    const float *__restrict__ row_scale, float scale, typename S::raw *__restrict__ dst) {
+    const float *__restrict__ row_scale, float scale, uint64_t n,
+    typename S::raw *__restrict__ dst) {
+    using raw = typename S::raw;
+    struct alignas(4 * sizeof(raw)) Four {
+        raw v[4];
+    };
+    const uint64_t i0 = ((uint64_t)blockIdx.x * kThreads + threadIdx.x) * 4;
+    if (i0 >= n) return;
     const int Up = U + 2 * kPadDet, Vp = V + 2 * vpad;
-    const int up = blockIdx.x * blockDim.x + threadIdx.x;
-    if (up >= Up) return;
-    const int vp = blockIdx.y;
-    const size_t a = blockIdx.z;
-    const int u = up - kPadDet, v = vp - vpad;
-    float val = 0.0f;
-    if (u >= 0 && u < U && v >= 0 && v < V) {
-        val = src[(a * V + v) * stride + u] * scale;
-        if (row_scale) val *= row_scale[a];
+    int up, vp;
+    uint64_t a;
+    if (n <= 0xffffffffull) { // 32-bit divisions
+        const uint32_t j = (uint32_t)i0, r0 = j / (uint32_t)Up; // a * Vp + vp
+        up = (int)(j - r0 * (uint32_t)Up);
+        vp = (int)(r0 % (uint32_t)Vp);
+        a = r0 / (uint32_t)Vp;
+    } else {
+        const uint64_t r0 = i0 / (uint64_t)Up;
+        up = (int)(i0 - r0 * Up);
+        vp = (int)(r0 % (uint64_t)Vp);
+        a = r0 / (uint64_t)Vp;
     }
-    dst[(a * Vp + vp) * (size_t)Up + up] = from_f32<S>(val);
+    Four f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int u = up - kPadDet, v = vp - vpad;
+        float val = 0.0f;
+        if (i0 + k < n && u >= 0 && u < U && v >= 0 && v < V) {
+            val = src[(a * V + v) * stride + u] * scale;
+            if (row_scale) val *= row_scale[a];
+        }
+        f.v[k] = from_f32<S>(val);
+        if (++up == Up) { // next destination row
+            up = 0;
+            if (++vp == Vp) {
+                vp = 0;
+                ++a;
+            }
+        }
+    }
+    if (i0 + 4 <= n) {
+        *reinterpret_cast<Four *>(dst + i0) = f;
+    } else {
+        for (uint64_t k = 0; i0 + k < n; ++k) dst[i0 + k] = f.v[k];
+    }
 }
 
 // Ramp filtering of projection rows through complex FFTs of row pairs (the filter's response
@@ -366,11 +401,14 @@ CMI_API cmi_status cmi_stage_projections(int device, const float *src, uint64_t 
     if (row_scale) CMI_TRY(check_device_ptr(row_scale, device, "row_scale"));
     const hipStream_t s = (hipStream_t)stream;
     const int Up = (int)U + 2 * kPadDet, Vp = (int)V + 2 * (int)vpad;
-    dim3 b(kThreads), g((unsigned)((Up + kThreads - 1) / kThreads), (unsigned)Vp, (unsigned)A);
+    const uint64_t n = (uint64_t)Up * Vp * A;
+    if ((n + 4 * kThreads - 1) / (4 * kThreads) > 0x7fffffffull)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "stage_projections: too many elements");
+    const dim3 b(kThreads), g((unsigned)((n + 4 * kThreads - 1) / (4 * kThreads)));
     auto launch = [&](auto tag) {
         using S = decltype(tag);
         hipLaunchKernelGGL((k_stage_projections<S>), g, b, 0, s, src, src_stride, (int)V,
-                           (int)U, (int)vpad, row_scale, scale, (typename S::raw *)dst);
+                           (int)U, (int)vpad, row_scale, scale, n, (typename S::raw *)dst);
     };
     if (dst_dtype == CMI_F32) launch(F32{});
     else if (dst_dtype == CMI_F16) launch(F16{});
