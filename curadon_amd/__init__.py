@@ -1,6 +1,6 @@
 """curadon_amd -- MI355X (gfx950) projection operators with curadon's Python API.
 
-Drop-in names (python/standalone/src/curadon/__init__.py:1-15): FanGeometry, ConeGeometry,
+Drop-in names (python/standalone/src/curadon/__init__.py:1-12): FanGeometry, ConeGeometry,
 forward, backward, TorchForward, TorchBackprojection, plan_2d, forward_2d, backward_2d,
 forward_3d, backward_3d. Additions: adjoint / adjoint_2d / TorchAdjoint (matched transpose),
 fbp / fbp_filter (filtered backprojection, FDK for cone beam). Stream and event pools under

@@ -5,8 +5,8 @@ Names, argument order and error types follow the reference binding:
 ``plan_2d(device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec, det_count,
 det_spacing, det_offset, DSO, DSD, angles, det_pitch, COR)`` with r/w ``forward_block_x`` /
 ``forward_block_y`` (python/standalone/src/plan.cu:30-59), ``forward_2d(volume, sinogram,
-plan)`` (forward.cu:158-210), ``backward_2d(volume, sinogram, plan)`` (backward.cu:299-351),
-``forward_3d`` / ``backward_3d`` with 15 arguments (forward.cu:89-141, backward.cu:230-282).
+plan)`` (forward.cu:92-136), ``backward_2d(volume, sinogram, plan)`` (backward.cu:89-133),
+``forward_3d`` / ``backward_3d`` with 15 arguments (forward.cu:23-75, backward.cu:20-72).
 
 Arrays are torch tensors on a ROCm device; work is enqueued on the tensor device's current
 torch stream. There is no CPU fallback: without the native library, importing this module
@@ -164,7 +164,7 @@ def check(status: int):
         return
     msg = last_error()
     if status == ERR_DEVICE:
-        raise AttributeError(msg)        # nb::attribute_error, forward.cu:161-171
+        raise AttributeError(msg)        # nb::attribute_error, forward.cu:95-105
     if status == ERR_DTYPE:
         raise TypeError(msg)
     if status == ERR_INVALID_ARGUMENT:
@@ -198,7 +198,7 @@ def precision_code(prec) -> int:
             return F32 if v == 32 else F16
         if v == BF16:
             return BF16
-    raise AttributeError("Precision must either be f32, f16 or bf16")  # plan.cu:27
+    raise AttributeError("Precision must either be f32, f16 or bf16")  # plan.cu:26
 
 
 def dtype_of_code(code: int) -> torch.dtype:
@@ -207,7 +207,7 @@ def dtype_of_code(code: int) -> torch.dtype:
 
 def tensor_code(t: torch.Tensor) -> int:
     if t.dtype not in _DTYPE_CODE:
-        raise TypeError("Only float32, float16 and bfloat16 are supported")  # forward.cu:200
+        raise TypeError("Only float32, float16 and bfloat16 are supported")  # forward.cu:134
     return _DTYPE_CODE[t.dtype]
 
 
@@ -341,7 +341,7 @@ def _check_pair_2d(volume, sinogram, plan, which):
     _require_device_tensor(volume, "volume")
     _require_device_tensor(sinogram, "sinogram")
     if volume.device != sinogram.device:
-        raise AttributeError("Volume and sinogram must be on the same device")   # forward.cu:161
+        raise AttributeError("Volume and sinogram must be on the same device")   # forward.cu:95-97
     if volume.device.index != plan.device:
         raise AttributeError("Volume and plan are not on the same device")         # :165
     vc, sc = tensor_code(volume), tensor_code(sinogram)
@@ -398,12 +398,59 @@ def adjoint_2d(volume, sinogram, plan):
 
 # ----------------------------------------------------------------------------- 3D -------
 
+def _cone_desc(vol_shape, vol_spacing, vol_offset, phi, theta, psi, det_shape, det_spacing,
+               det_offset, det_rotation, DSO, DSD, COR):
+    """A cmi_cone3d_desc and the FP64 angle arrays it points to (keep them alive with it)."""
+    phi = _f64(phi)
+    A = phi.size
+    theta, psi = _f64(theta, A), _f64(psi, A)
+    d = Cone3DDesc()
+    vs = [int(v) for v in np.asarray(vol_shape).reshape(-1)]
+    ds = [int(v) for v in np.asarray(det_shape).reshape(-1)]
+    if len(vs) != 3 or len(ds) != 2:
+        raise AttributeError("vol_shape needs 3 values and det_shape 2")
+    d.vol_shape[:] = vs
+    d.vol_spacing[:] = list(_f64(vol_spacing, 3))
+    d.vol_offset[:] = list(_f64(vol_offset, 3))
+    d.det_shape[:] = ds
+    d.det_spacing[:] = list(_f64(det_spacing, 2))
+    d.det_offset[:] = list(_f64(det_offset, 2))
+    d.det_rotation[:] = list(_f64(det_rotation, 3))
+    d.DSO, d.DSD, d.COR = float(DSO), float(DSD), float(COR)
+    d.nangles = A
+    d.phi = phi.ctypes.data_as(_dp)
+    d.theta = theta.ctypes.data_as(_dp)
+    d.psi = psi.ctypes.data_as(_dp)
+    return d, (phi, theta, psi)
+
+
+def _bp_matrices_of(desc, mode, with_layout):
+    out = np.zeros((int(desc.nangles), 3, 4), dtype=np.float32)
+    yinv = C.c_int()
+    check(_lib.cmi_cone3d_bp_matrices(C.byref(desc), _BP3D_MODES[mode],
+                                      out.ctypes.data_as(C.POINTER(C.c_float)),
+                                      C.byref(yinv)))
+    return (out, bool(yinv.value)) if with_layout else out
+
+
+def cone3d_bp_matrices(vol_shape, vol_spacing, vol_offset, phi, theta, psi, det_shape,
+                       det_spacing, det_offset, det_rotation, DSO, DSD, COR, mode="fdk",
+                       with_layout=False):
+    """The backprojection matrices of `mode` for a cone-beam description (arguments as
+    plan_3d), computed on the host: no device plan and no synchronisation."""
+    d, keep = _cone_desc(vol_shape, vol_spacing, vol_offset, phi, theta, psi, det_shape,
+                         det_spacing, det_offset, det_rotation, DSO, DSD, COR)
+    r = _bp_matrices_of(d, mode, with_layout)
+    del keep
+    return r
+
+
 class plan_3d:
     """Per-geometry state of the 3D cone-beam operators. The reference has no 3D plan (it
     rebuilds its tables on every call, forward_3d.hpp:214-274); this one holds the per-angle
     geometry and the backprojection matrices on ``device``.
 
-    Arguments follow curadon_ext.forward_3d (forward.cu:89-104) in Python axis order:
+    Arguments follow curadon_ext.forward_3d (forward.cu:23-38) in Python axis order:
     vol_shape / vol_spacing / vol_offset (z, y, x), angles phi / theta / psi (A,),
     det_shape / det_spacing / det_offset (v, u), det_rotation (pitch, roll, yaw)."""
 
@@ -413,30 +460,12 @@ class plan_3d:
         self.device = int(device)
         self.vol_prec = precision_code(vol_prec)
         self.det_prec = precision_code(det_prec)
-        self._phi = _f64(phi)
-        A = self._phi.size
-        self._theta = _f64(theta, A)
-        self._psi = _f64(psi, A)
-        d = Cone3DDesc()
-        vs = [int(v) for v in np.asarray(vol_shape).reshape(-1)]
-        ds = [int(v) for v in np.asarray(det_shape).reshape(-1)]
-        if len(vs) != 3 or len(ds) != 2:
-            raise AttributeError("vol_shape needs 3 values and det_shape 2")
-        d.vol_shape[:] = vs
-        d.vol_spacing[:] = list(_f64(vol_spacing, 3))
-        d.vol_offset[:] = list(_f64(vol_offset, 3))
-        d.det_shape[:] = ds
-        d.det_spacing[:] = list(_f64(det_spacing, 2))
-        d.det_offset[:] = list(_f64(det_offset, 2))
-        d.det_rotation[:] = list(_f64(det_rotation, 3))
-        d.DSO, d.DSD, d.COR = float(DSO), float(DSD), float(COR)
-        d.nangles = A
-        d.phi = self._phi.ctypes.data_as(_dp)
-        d.theta = self._theta.ctypes.data_as(_dp)
-        d.psi = self._psi.ctypes.data_as(_dp)
-        self._desc = d
-        self.vol_shape = tuple(vs)
-        self.sino_shape = (A, ds[0], ds[1])
+        d, keep = _cone_desc(vol_shape, vol_spacing, vol_offset, phi, theta, psi, det_shape,
+                             det_spacing, det_offset, det_rotation, DSO, DSD, COR)
+        self._desc, self._keep = d, keep
+        A = int(d.nangles)
+        self.vol_shape = tuple(int(v) for v in d.vol_shape)
+        self.sino_shape = (A, int(d.det_shape[0]), int(d.det_shape[1]))
         self.nangles = A
         h = _vp()
         check(_lib.cmi_plan3d_create(self.device, self.vol_prec, self.det_prec, C.byref(d),
@@ -451,12 +480,7 @@ class plan_3d:
         """(A, 3, 4) FP32 projection matrices cmi_backward_3d uses in `mode`; with_layout also
         returns whether their U and W rows are free of y (the backprojectors then run along
         y)."""
-        out = np.zeros((self.nangles, 3, 4), dtype=np.float32)
-        yinv = C.c_int()
-        check(_lib.cmi_cone3d_bp_matrices(C.byref(self._desc), _BP3D_MODES[mode],
-                                          out.ctypes.data_as(C.POINTER(C.c_float)),
-                                          C.byref(yinv)))
-        return (out, bool(yinv.value)) if with_layout else out
+        return _bp_matrices_of(self._desc, mode, with_layout)
 
     def info(self) -> dict:
         i = Plan3DInfo()
@@ -473,7 +497,7 @@ def _check_pair_3d(volume, sinogram, plan, which):
     _require_device_tensor(volume, "volume")
     _require_device_tensor(sinogram, "sinogram")
     if volume.device != sinogram.device:
-        raise AttributeError("Volume and sinogram must be on the same device")   # forward.cu:106
+        raise AttributeError("Volume and sinogram must be on the same device")   # forward.cu:40-42
     if volume.device.index != plan.device:
         raise AttributeError("Volume and plan are not on the same device")
     vc, sc = tensor_code(volume), tensor_code(sinogram)
@@ -548,7 +572,7 @@ def _plan3d_for(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi
 
 def forward_3d(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
                det_spacing, det_offset, det_rotation, DSO, DSD, COR):
-    """curadon_ext.forward_3d with its 15 arguments (forward.cu:89-141); the plan for the
+    """curadon_ext.forward_3d with its 15 arguments (forward.cu:23-75); the plan for the
     geometry is built once and cached."""
     p = _plan3d_for(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
                     det_spacing, det_offset, det_rotation, DSO, DSD, COR)
@@ -557,7 +581,7 @@ def forward_3d(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi,
 
 def backward_3d(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
                 det_spacing, det_offset, det_rotation, DSO, DSD, COR):
-    """curadon_ext.backward_3d with its 15 arguments (backward.cu:230-282), reference mode."""
+    """curadon_ext.backward_3d with its 15 arguments (backward.cu:20-72), reference mode."""
     p = _plan3d_for(x, vol_shape, vol_spacing, vol_offset, sinogram, phi, theta, psi, sino_shape,
                     det_spacing, det_offset, det_rotation, DSO, DSD, COR)
     backward_3d_plan(x, sinogram, p)

@@ -26,6 +26,7 @@
 #include "bp_tiles.hpp"
 #include "cmi_internal.hpp"
 #include "geometry3d.hpp"
+#include "stage_index.hpp"
 
 using namespace cmi;
 using cmi::cone3d::FwdGeom;
@@ -153,7 +154,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage3d_t(const typename S::raw *
 
 // ppad[b][a][vp][up] = sino[b][a][vp-2][up-2], 0 in the border. Flat over the n destination
 // elements, four consecutive ones per thread with one vector store (its offset is a multiple
-// of four elements); the source row is followed element by element.
+// of four elements; the scratch stack is allocated with at least that alignment); the source
+// row is followed element by element.
 template <class S>
 __global__ __launch_bounds__(kMaxBlock) void k_stage_panel(const typename S::raw *__restrict__ sino,
                                                            typename S::raw *__restrict__ ppad,
@@ -165,34 +167,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_stage_panel(const typename S::raw
     const uint64_t i0 = ((uint64_t)blockIdx.x * kMaxBlock + threadIdx.x) * 4;
     if (i0 >= n) return;
     const int Up = U + 2 * kPad, Vp = V + 2 * kPad;
-    int up, vp;
-    uint64_t ab;
-    if (n <= 0xffffffffull) { // 32-bit divisions
-        const uint32_t j = (uint32_t)i0, r0 = j / (uint32_t)Up; // (b * A + a) * Vp + vp
-        up = (int)(j - r0 * (uint32_t)Up);
-        vp = (int)(r0 % (uint32_t)Vp);
-        ab = r0 / (uint32_t)Vp;
-    } else {
-        const uint64_t r0 = i0 / (uint64_t)Up;
-        up = (int)(i0 - r0 * Up);
-        vp = (int)(r0 % (uint64_t)Vp);
-        ab = r0 / (uint64_t)Vp;
-    }
+    StageIndex at = stage_index(i0, Up, Vp, n <= 0xffffffffull); // at.a = b * A + a
     Four f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int u = up - kPad, v = vp - kPad;
+        const int u = at.up - kPad, v = at.vp - kPad;
         raw val = (raw)0.0f;
         if (i0 + k < n && u >= 0 && u < U && v >= 0 && v < V)
-            val = sino[(ab * V + v) * (size_t)U + u];
+            val = sino[(at.a * V + v) * (size_t)U + u];
         f.v[k] = val;
-        if (++up == Up) { // next destination row
-            up = 0;
-            if (++vp == Vp) {
-                vp = 0;
-                ++ab;
-            }
-        }
+        stage_next(at, Up, Vp);
     }
     if (i0 + 4 <= n) {
         *reinterpret_cast<Four *>(ppad + i0) = f;

@@ -128,7 +128,7 @@ struct Ray {
     int64_t n;      // number of samples; 0 for a missed ray
 };
 
-// Slab clip against [-1, n] per axis (include/curadon/detail/intersection.h:96-112), the miss
+// Slab clip against [-1, n] per axis (include/curadon/detail/intersection.h:30-46), the miss
 // test tmin > tmax - 1e-6 (forward_2d.hpp:58) and the march set-up (forward_2d.hpp:63-77).
 __host__ __device__ inline Ray setup_ray(const AngleGeom &ag, int64_t nx, int64_t ny, double sx,
                                          double sy, int64_t k) {

@@ -136,7 +136,7 @@ inline BpGeom bp_angle(const Desc &g, double phi, double theta, double psi) {
 }
 
 // Ray (iu, iv) of one angle (forward_3d.hpp:57-90): direction from the source to the pixel,
-// sample budget ceil(|dir|), slab clip against [-1, n+1]^3 (intersection.h:78-94), nst =
+// sample budget ceil(|dir|), slab clip against [-1, n+1]^3 (intersection.h:11-28), nst =
 // ceil((tmax - tmin) * budget) steps of dt; nst + 1 samples t_i = tmin + i dt. hit = false for
 // a ray that misses the clip box.
 struct Ray3 {

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "bp_tiles.hpp"
+#include "stage_index.hpp"
 #include "cmi_internal.hpp"
 
 using namespace cmi;
@@ -68,9 +69,10 @@ uint64_t tiles_of(const cmi_bp_job &j) {
 // ---------------------------------------------------------------- staging -----------------
 
 // dst[a][vp][up] = scale * row_scale[a] * src[a][vp - vpad][up - 2], 0 in the border. Flat
-// over the n destination elements, four consecutive ones per thread with one vector store (its
-// offset is a multiple of four elements); the source row is followed element by element.
-template <class S>
+// over the n destination elements, four consecutive ones per thread; the source row is
+// followed element by element. VEC: the four are written with one vector store, which needs
+// dst aligned to four elements (the host checks; otherwise element stores).
+template <class S, bool VEC>
 __global__ __launch_bounds__(kThreads) void k_stage_projections(
     const float *__restrict__ src, uint64_t stride, int V, int U, int vpad,
     const float *__restrict__ row_scale, float scale, uint64_t n,
@@ -82,41 +84,23 @@ __global__ __launch_bounds__(kThreads) void k_stage_projections(
     const uint64_t i0 = ((uint64_t)blockIdx.x * kThreads + threadIdx.x) * 4;
     if (i0 >= n) return;
     const int Up = U + 2 * kPadDet, Vp = V + 2 * vpad;
-    int up, vp;
-    uint64_t a;
-    if (n <= 0xffffffffull) { // 32-bit divisions
-        const uint32_t j = (uint32_t)i0, r0 = j / (uint32_t)Up; // a * Vp + vp
-        up = (int)(j - r0 * (uint32_t)Up);
-        vp = (int)(r0 % (uint32_t)Vp);
-        a = r0 / (uint32_t)Vp;
-    } else {
-        const uint64_t r0 = i0 / (uint64_t)Up;
-        up = (int)(i0 - r0 * Up);
-        vp = (int)(r0 % (uint64_t)Vp);
-        a = r0 / (uint64_t)Vp;
-    }
+    StageIndex at = stage_index(i0, Up, Vp, n <= 0xffffffffull);
     Four f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int u = up - kPadDet, v = vp - vpad;
+        const int u = at.up - kPadDet, v = at.vp - vpad;
         float val = 0.0f;
         if (i0 + k < n && u >= 0 && u < U && v >= 0 && v < V) {
-            val = src[(a * V + v) * stride + u] * scale;
-            if (row_scale) val *= row_scale[a];
+            val = src[(at.a * V + v) * stride + u] * scale;
+            if (row_scale) val *= row_scale[at.a];
         }
         f.v[k] = from_f32<S>(val);
-        if (++up == Up) { // next destination row
-            up = 0;
-            if (++vp == Vp) {
-                vp = 0;
-                ++a;
-            }
-        }
+        stage_next(at, Up, Vp);
     }
-    if (i0 + 4 <= n) {
+    if (VEC && i0 + 4 <= n) {
         *reinterpret_cast<Four *>(dst + i0) = f;
     } else {
-        for (uint64_t k = 0; i0 + k < n; ++k) dst[i0 + k] = f.v[k];
+        for (uint64_t k = 0; k < 4 && i0 + k < n; ++k) dst[i0 + k] = f.v[k];
     }
 }
 
@@ -405,10 +389,13 @@ CMI_API cmi_status cmi_stage_projections(int device, const float *src, uint64_t 
     if ((n + 4 * kThreads - 1) / (4 * kThreads) > 0x7fffffffull)
         return fail(CMI_ERR_INVALID_ARGUMENT, "stage_projections: too many elements");
     const dim3 b(kThreads), g((unsigned)((n + 4 * kThreads - 1) / (4 * kThreads)));
+    // one vector store per four elements needs dst aligned to four elements
+    const bool vec = (uintptr_t)dst % (4 * dtype_size(dst_dtype)) == 0;
     auto launch = [&](auto tag) {
         using S = decltype(tag);
-        hipLaunchKernelGGL((k_stage_projections<S>), g, b, 0, s, src, src_stride, (int)V,
-                           (int)U, (int)vpad, row_scale, scale, n, (typename S::raw *)dst);
+        auto kern = vec ? k_stage_projections<S, true> : k_stage_projections<S, false>;
+        hipLaunchKernelGGL(kern, g, b, 0, s, src, src_stride, (int)V, (int)U, (int)vpad,
+                           row_scale, scale, n, (typename S::raw *)dst);
     };
     if (dst_dtype == CMI_F32) launch(F32{});
     else if (dst_dtype == CMI_F16) launch(F16{});

@@ -17,13 +17,25 @@ from typing import Any, List, Optional, Tuple, Union
 import numpy as np
 
 
+def _as_float(val: Any) -> float:
+    """float(val); a one-element array goes through .item() first (float() of an array with
+    ndim > 0 is deprecated in NumPy)."""
+    if isinstance(val, np.ndarray) and val.size == 1:
+        val = val.item()
+    return float(val)
+
+
+def _fp(a) -> bytes:
+    """Bytes of an array-like as FP64, for plan fingerprints."""
+    a = np.asarray(a, dtype=np.float64)
+    return bytes(repr(a.shape), "ascii") + a.tobytes()
+
+
 def is_convertible_to_float(val: Any):
     try:
-        float(val)
+        _as_float(val)
         return True
-    except ValueError:
-        return False
-    except TypeError:
+    except (ValueError, TypeError):
         return False
 
 
@@ -112,7 +124,7 @@ class FanGeometry:
         self.nangles = self.angles.shape[0]
 
         if is_convertible_to_float(DSD) is True:
-            self.DSD = np.full(self.nangles, float(DSD))
+            self.DSD = np.full(self.nangles, _as_float(DSD))
         else:
             self.DSD = np.asarray(DSD, dtype=np.float64)
             if self.DSD.ndim != 1 or self.DSD.size != self.nangles:
@@ -121,7 +133,7 @@ class FanGeometry:
                     f"(is {self.DSD.size}, expected {self.nangles})")
 
         if is_convertible_to_float(DSO) is True:
-            self.DSO = np.full(self.nangles, float(DSO))
+            self.DSO = np.full(self.nangles, _as_float(DSO))
         else:
             self.DSO = np.asarray(DSO, dtype=np.float64)
             if self.DSO.ndim != 1 or self.DSO.size != self.nangles:
@@ -135,26 +147,26 @@ class FanGeometry:
 
         if is_convertible_to_float(det_shape) is False:
             raise AttributeError(f"det_shape must to be float, got {type(det_shape)}")
-        self.det_shape = int(det_shape)
+        self.det_shape = int(_as_float(det_shape))
 
         if det_spacing is not None:
             if is_convertible_to_float(det_spacing) is False:
                 raise TypeError(f"Expected det_spacing to be float, got {type(det_spacing)}")
-            self.det_spacing = float(det_spacing)
+            self.det_spacing = _as_float(det_spacing)
         else:
             self.det_spacing = 1.0
 
         if det_offset is not None:
             if is_convertible_to_float(det_offset) is False:
                 raise TypeError(f"Expected det_offset to be float, got {type(det_offset)}")
-            self.det_offset = float(det_offset)
+            self.det_offset = _as_float(det_offset)
         else:
             self.det_offset = 0.0
 
         if det_rotation is not None:
             if is_convertible_to_float(det_rotation) is False:
                 raise TypeError(f"Expected det_rotation to be float, got {type(det_rotation)}")
-            self.det_rotation = float(det_rotation)
+            self.det_rotation = _as_float(det_rotation)
         else:
             self.det_rotation = 0.0
 
@@ -179,23 +191,38 @@ class FanGeometry:
         self.sino_prec = sino_prec
         self.vecs = None  # ASTRA-like views when built by from_vec
         self._plan = None
+        self._plan_key = None
+
+    def fingerprint(self):
+        """Every attribute the native plan is built from, as a hashable tuple: the plan is
+        rebuilt when any of them changes after it was made."""
+        return (_fp(self.angles), _fp(self.DSD), _fp(self.DSO), _fp(self.vol_shape),
+                _fp(self.vol_spacing), _fp(self.vol_offset), int(self.det_shape),
+                _as_float(self.det_spacing), _as_float(self.det_offset),
+                _as_float(self.det_rotation), int(self.device), str(self.vol_prec),
+                str(self.sino_prec), None if self.vecs is None else _fp(self.vecs))
 
     @property
     def plan(self):
-        """The native plan (``curadon_amd._lib.plan_2d``), built on first access: from the
-        views for a geometry made by ``from_vec``, else from angles / DSO / DSD with COR 0, as
-        the reference passes it (geometry.py:226-228)."""
-        if self._plan is None:
+        """The native plan (``curadon_amd._lib.plan_2d``), built on first access and rebuilt
+        whenever an attribute it depends on has changed since (``fingerprint``): from the views
+        for a geometry made by ``from_vec``, else from angles / DSO / DSD with COR 0, as the
+        reference passes it (geometry.py:226-228)."""
+        key = self.fingerprint()
+        if self._plan is None or self._plan_key != key:
             from . import _lib
+            self._plan = None
             if self.vecs is not None:
                 self._plan = _lib.plan_2d.from_views(
                     self.device, self.vol_prec, self.vol_shape, self.vol_spacing,
                     self.vol_offset, self.sino_prec, self.det_shape, self.vecs)
-                return self._plan
-            self._plan = _lib.plan_2d(self.device, self.vol_prec, self.vol_shape,
-                                      self.vol_spacing, self.vol_offset, self.sino_prec,
-                                      self.det_shape, self.det_spacing, self.det_offset,
-                                      self.DSO, self.DSD, self.angles, self.det_rotation, 0.0)
+            else:
+                self._plan = _lib.plan_2d(self.device, self.vol_prec, self.vol_shape,
+                                          self.vol_spacing, self.vol_offset, self.sino_prec,
+                                          self.det_shape, self.det_spacing, self.det_offset,
+                                          self.DSO, self.DSD, self.angles, self.det_rotation,
+                                          0.0)
+            self._plan_key = key
         return self._plan
 
     def sinogram_shape(self):
@@ -216,10 +243,10 @@ class ConeGeometry:
                  vol_prec=32, sino_prec=32):
         if is_convertible_to_float(DSD) is False:
             raise TypeError(f"Expected DSD to be float, got {type(DSD)}")
-        self.DSD = float(DSD)
+        self.DSD = _as_float(DSD)
         if is_convertible_to_float(DSO) is False:
             raise TypeError(f"Expected DSO to be float, got {type(DSO)}")
-        self.DSO = float(DSO)
+        self.DSO = _as_float(DSO)
 
         self.det_shape = np.asarray(det_shape)
         if self.det_shape.shape != (2,):
@@ -261,7 +288,7 @@ class ConeGeometry:
         if COR is not None:
             if is_convertible_to_float(COR) is False:
                 raise TypeError(f"Expected COR to be float, got {type(COR)}")
-            self.COR = float(COR)
+            self.COR = _as_float(COR)
         else:
             self.COR = 0.0
         self.vol_spacing = vec(vol_spacing, 3, "vol_spacing", np.ones(3))
@@ -271,18 +298,41 @@ class ConeGeometry:
         self.vol_prec = vol_prec
         self.sino_prec = sino_prec
         self._plan = None
+        self._plan_key = None
+
+    def fingerprint(self):
+        """Every attribute the native plan is built from, as a hashable tuple: the plan is
+        rebuilt when any of them changes after it was made (the reference reads them on every
+        call, python/standalone/src/curadon/forward.py:18-26)."""
+        return (_fp(self.angles), _as_float(self.DSD), _as_float(self.DSO), _fp(self.vol_shape),
+                _fp(self.vol_spacing), _fp(self.vol_offset), _fp(self.det_shape),
+                _fp(self.det_spacing), _fp(self.det_offset), _fp(self.det_rotation),
+                _as_float(self.COR), int(self.device), str(self.vol_prec), str(self.sino_prec))
+
+    def _plan_args(self):
+        return (self.vol_shape, self.vol_spacing, self.vol_offset, self.angles[0],
+                self.angles[1], self.angles[2], self.det_shape, self.det_spacing,
+                self.det_offset, self.det_rotation, self.DSO, self.DSD, self.COR)
 
     @property
     def plan(self):
-        """The native 3D plan, built on first access and reused by every call."""
-        if self._plan is None:
+        """The native 3D plan, built on first access and reused by every call until an
+        attribute it depends on changes (``fingerprint``)."""
+        key = self.fingerprint()
+        if self._plan is None or self._plan_key != key:
             from . import _lib
+            self._plan = None
             self._plan = _lib.plan_3d(self.device, self.vol_prec, self.sino_prec,
-                                      self.vol_shape, self.vol_spacing, self.vol_offset,
-                                      self.angles[0], self.angles[1], self.angles[2],
-                                      self.det_shape, self.det_spacing, self.det_offset,
-                                      self.det_rotation, self.DSO, self.DSD, self.COR)
+                                      *self._plan_args())
+            self._plan_key = key
         return self._plan
+
+    def bp_matrices(self, mode="fdk", with_layout=False):
+        """(A, 3, 4) FP32 backprojection matrices of ``mode`` for the current attributes, made
+        on the host without building a device plan (cmi_cone3d_bp_matrices); with_layout also
+        returns whether their U and W rows are free of y."""
+        from . import _lib
+        return _lib.cone3d_bp_matrices(*self._plan_args(), mode=mode, with_layout=with_layout)
 
     def sinogram_shape(self):
         return (self.nangles, *[int(v) for v in self.det_shape])

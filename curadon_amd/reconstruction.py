@@ -25,6 +25,7 @@ fftshift(|linspace(-1, 1, D)|), times pi / (2 A)), for callers that want its out
 from __future__ import annotations
 
 import functools
+import threading
 import math
 
 import torch
@@ -140,6 +141,27 @@ def _toeplitz(n_det: int, window, frequency_scaling: float, device) -> torch.Ten
     return h[(k - j) % L].to(torch.float32).to(device).contiguous()
 
 
+_MM_LOCK = threading.Lock()
+
+
+def _mm_ieee(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """a @ b with full FP32 products whatever the caller's TF32 setting. The precision is
+    chosen when the product is enqueued; it is switched and restored through one API family
+    (torch.backends.cuda.matmul.fp32_precision), which leaves the caller's state as it was in
+    every combination of the legacy and new settings, and under a lock, so concurrent callers
+    of this function do not restore each other's values."""
+    m = torch.backends.cuda.matmul
+    with _MM_LOCK:
+        old = m.fp32_precision
+        if old == "ieee":
+            return torch.mm(a, b)
+        m.fp32_precision = "ieee"
+        try:
+            return torch.mm(a, b)
+        finally:
+            m.fp32_precision = old
+
+
 @functools.lru_cache(maxsize=64)
 def _full_response(n_det: int, window, frequency_scaling: float, device) -> torch.Tensor:
     H = _ramp_response_dev(n_det, window, frequency_scaling, device)
@@ -163,14 +185,7 @@ def filter_rows_weighted(p: torch.Tensor, cw: torch.Tensor, n_det: int, window="
     if D == int(n_det) and D <= TOEPLITZ_MAX:
         T = _toeplitz(D, window, float(frequency_scaling), p.device)
         w = (p.reshape(-1, cw.shape[0], D) * cw).reshape(-1, D)
-        # full FP32 products whatever the caller's TF32 setting
-        tf32 = torch.backends.cuda.matmul.allow_tf32
-        torch.backends.cuda.matmul.allow_tf32 = False
-        try:
-            y = torch.mm(w, T)
-        finally:
-            torch.backends.cuda.matmul.allow_tf32 = tf32
-        return y.reshape(p.shape)
+        return _mm_ieee(w, T).reshape(p.shape)
     L = fft_length(int(n_det))
     if L > PAIR_FFT_MAX:
         H = _ramp_response_dev(int(n_det), window, float(frequency_scaling), p.device)

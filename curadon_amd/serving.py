@@ -139,10 +139,12 @@ class FBPServer:
 
     # ------------------------------------------------------------------ prep ----------
     def _filter_parts(self, g, window, fs):
-        key = (id(g), window, fs)
+        # keyed by the geometry's attributes, so a geometry changed after an earlier job gets
+        # new weights and tables
+        key = (type(g).__name__, g.fingerprint(), window, fs)
         hit = self._cache.get(key)
-        if hit is not None and hit[0] is g:
-            return hit[1]
+        if hit is not None:
+            return hit
         dev = self.device
         if isinstance(g, FanGeometry):
             cw = cosine_weights_fan(g, dev)
@@ -153,12 +155,13 @@ class FBPServer:
         else:
             cw = cosine_weights_cone(g, dev)
             scale = (math.pi / g.nangles) * g.DSD / (float(g.det_spacing[1]) * g.DSO)
-            m, yinv = g.plan.bp_matrices("fdk", with_layout=True)
+            # host-side matrices: no device plan, no synchronisation on the serving loop
+            m, yinv = g.bp_matrices("fdk", with_layout=True)
             table = torch.from_numpy(m).to(dev)
             parts = (cw, None, scale, (table, yinv))
         if len(self._cache) > 256:
             self._cache.clear()
-        self._cache[key] = (g, parts)
+        self._cache[key] = parts
         return parts
 
     def _prepare(self, job: FBPJob) -> _Active:

@@ -8,7 +8,7 @@
  * creation except for the launch-shape tunables, so one plan may be used from several host
  * threads and streams at once.
  *
- * Layouts are the reference's (python/standalone/src/forward.cu:143-156, 113-138):
+ * Layouts are the reference's (python/standalone/src/forward.cu:47-58, 85-87):
  *   2D image     (ny, nx)            x fastest          -- batch of B images: (B, ny, nx)
  *   2D sinogram  (A, D)              detector fastest   -- batch: (B, A, D)
  *   3D volume    (nz, ny, nx)        x fastest          -- batch: (B, nz, ny, nx)
@@ -40,7 +40,7 @@ extern "C" {
 /* Status codes. The message of the last failure on the calling host thread is returned by
  * cmi_last_error(). Replaces gpuErrchk -> exit() (include/curadon/detail/error.h:6-18) and
  * the nanobind attribute_error / type_error raised by the binding
- * (python/standalone/src/forward.cu:106-108, 161-201; plan.cu:20-28). */
+ * (python/standalone/src/forward.cu:95-105, 107-135; plan.cu:20-28). */
 typedef enum cmi_status {
     CMI_OK = 0,
     CMI_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad shape, size or geometry value */
@@ -131,9 +131,9 @@ CMI_API cmi_status cmi_plan2d_get_info(const cmi_plan2d *plan, cmi_plan2d_info *
 
 /*
  * Ray-driven forward projection, sinogram := A volume, for `batch` images.
- * Replaces curadon_ext.forward_2d (python/standalone/src/forward.cu:158-202 ->
+ * Replaces curadon_ext.forward_2d (python/standalone/src/forward.cu:92-136 ->
  * curad::fp::forward_2d, include/curadon/detail/forward_2d.hpp:82-139).
- * vol_dtype must equal the plan's volume precision (forward.cu:173-189); sino_dtype must
+ * vol_dtype must equal the plan's volume precision (forward.cu:107-123); sino_dtype must
  * equal the plan's detector precision (not checked by the reference). Enqueued on `stream`;
  * does not block. Scratch: the call stages each image into zero-padded row-major and
  * transposed copies from a stream-ordered allocation that is released on the same stream.
@@ -143,7 +143,7 @@ CMI_API cmi_status cmi_forward_2d(const cmi_plan2d *plan, const void *volume, in
 
 /*
  * Voxel-driven backprojection with the reference's weighting, volume += B sinogram.
- * Replaces curadon_ext.backward_2d (python/standalone/src/backward.cu:299-343 ->
+ * Replaces curadon_ext.backward_2d (python/standalone/src/backward.cu:89-133 ->
  * curad::bp::backproject_2d, include/curadon/detail/backward_2d.hpp:103-166). All angles
  * are processed in one launch (no 1024-angle chunking). Enqueued on `stream`.
  */
@@ -176,7 +176,7 @@ CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_
 
 /*
  * Cone-beam description, in the argument order and axis order of curadon_ext.forward_3d /
- * backward_3d (python/standalone/src/forward.cu:89-104, backward.cu:230-245): vol_shape /
+ * backward_3d (python/standalone/src/forward.cu:23-38, backward.cu:20-35): vol_shape /
  * vol_spacing / vol_offset in (z, y, x), det_shape / det_spacing / det_offset in (v, u),
  * det_rotation as (pitch, roll, yaw), per-angle phi / theta / psi, scalar DSO, DSD, COR.
  */
@@ -217,7 +217,7 @@ CMI_API cmi_status cmi_plan3d_get_info(const cmi_plan3d *plan, cmi_plan3d_info *
 
 /*
  * Cone-beam forward projection, sinogram := A volume (unscaled sum of trilinear samples, as
- * forward_3d.hpp:87-93). Replaces curadon_ext.forward_3d (forward.cu:89-141 ->
+ * forward_3d.hpp:87-93). Replaces curadon_ext.forward_3d (forward.cu:23-75 ->
  * curad::fp::forward_3d, forward_3d.hpp:214-274). Rays that miss the volume write 0 (the
  * reference leaves them unwritten, forward_3d.hpp:74-76).
  */
@@ -226,7 +226,7 @@ CMI_API cmi_status cmi_forward_3d(const cmi_plan3d *plan, const void *volume, in
 
 /*
  * Voxel-driven backprojection, volume += B sinogram. Replaces curadon_ext.backward_3d
- * (backward.cu:230-282 -> curad::bp::backproject_3d, backward_3d.hpp:181-253).
+ * (backward.cu:20-72 -> curad::bp::backproject_3d, backward_3d.hpp:181-253).
  * mode CMI_BP3D_REFERENCE reproduces the reference's projection (detector plane at
  * z = DSO - DSD, integer detector centre, detector spacing ignored; backward_3d.hpp:79-94).
  * mode CMI_BP3D_MATCHED projects each voxel through the forward model's geometry instead.
@@ -271,7 +271,9 @@ CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *desc, int mode,
  * for vp - vpad in [0, V) and up - 2 in [0, U), and 0 elsewhere; dst is [A][V + 2 vpad][U + 4].
  * Fan-beam jobs pass V = 1, vpad = 0 ([A][D + 4]); cone-beam jobs V = panel rows, vpad = 2.
  * `src` is FP32 device memory (for instance the output of an inverse real FFT of length
- * src_stride >= U); row_scale may be null (all ones). Enqueued on `stream`.
+ * src_stride >= U); row_scale may be null (all ones). `dst` may have any element alignment
+ * (a view with a storage offset, say): when it is aligned to four elements, four are written
+ * with one vector store, otherwise one at a time. Enqueued on `stream`.
  */
 CMI_API cmi_status cmi_stage_projections(int device, const float *src, uint64_t src_stride,
                                          uint64_t A, uint64_t V, uint64_t U, uint32_t vpad,

@@ -6,7 +6,7 @@
  *
  * Reading conventions taken from the reference and kept here:
  *  - the slab clip starts from t in [FLT_MIN, FLT_MAX] and combines the per-axis bounds with
- *    IEEE minNum/maxNum (intersection.h:81-90, 99-108), so a 0/0 = NaN bound is ignored;
+ *    IEEE minNum/maxNum (intersection.h:19-25, 37-43), so a 0/0 = NaN bound is ignored;
  *  - the 2D miss test is tmin > tmax - 1e-6 with the constant in double
  *    (forward_2d.hpp:58);
  *  - the 2D sample count is round-half-even of the L-inf chord (forward_2d.hpp:66);
@@ -198,7 +198,7 @@ int64_t orc_ray2d(const orc_fan2d *g, const double *src, const double *origin,
     const double rd[2] = {origin[2 * a] + (double)k * delta[2 * a],
                           origin[2 * a + 1] + (double)k * delta[2 * a + 1]}; /* fwd:52 */
     const double hi[2] = {(double)g->nx, (double)g->ny};
-    double tmin = FLT_MIN, tmax = FLT_MAX; /* intersection.h:99-100 */
+    double tmin = FLT_MIN, tmax = FLT_MAX; /* intersection.h:33-34 */
     for (int d = 0; d < 2; ++d) {
         const double t1 = (-1.0 - ro[d]) / rd[d];
         const double t2 = (hi[d] - ro[d]) / rd[d];
@@ -534,7 +534,7 @@ int64_t orc_ray3d(const orc_cone3d *g, const double *origin, const double *du,
     }
     const double nsamples = ceil(sqrt(nrm2)); /* accuracy 1 (:68, :232) */
     const double hi[3] = {(double)g->nx + 1.0, (double)g->ny + 1.0, (double)g->nz + 1.0};
-    double tmin = FLT_MIN, tmax = FLT_MAX; /* intersection.h:81-90 against [-1, n+1]^3 */
+    double tmin = FLT_MIN, tmax = FLT_MAX; /* intersection.h:15-16, 19-25 against [-1, n+1]^3 */
     for (int d = 0; d < 3; ++d) {
         const double t1 = (-1.0 - s[d]) / dir[d];
         const double t2 = (hi[d] - s[d]) / dir[d];

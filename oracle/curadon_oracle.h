@@ -51,7 +51,7 @@ typedef struct {
  * all in texel coordinates (plan_2d.hpp:17-80, 189-241). Outputs are [A][2]. */
 void orc_plan2d(const orc_fan2d *g, double *src, double *origin, double *delta);
 
-/* Ray (a, k) after the slab clip (intersection.h:96-112) and the march set-up of
+/* Ray (a, k) after the slab clip (intersection.h:30-46) and the march set-up of
  * kernel_forward_2d (forward_2d.hpp:52-77): entry point e, unit-L-inf step v, sample count
  * N = rint(L-inf chord) and the world step length. Returns N; 0 for a missed ray. */
 int64_t orc_ray2d(const orc_fan2d *g, const double *src, const double *origin,
@@ -95,7 +95,7 @@ double orc_trilerp(const double *vol, int64_t nx, int64_t ny, int64_t nz, double
 /* ----------------------------------------------------------------------------------------
  * 3D cone beam, flat panel, circular or general (phi, theta, psi) trajectory.
  * Field order follows device_measurement / device_volume after the binding's axis reversal
- * (python/standalone/src/forward.cu:113-138): volume (x, y, z), detector (u, v).
+ * (python/standalone/src/forward.cu:47-58): volume (x, y, z), detector (u, v).
  * ------------------------------------------------------------------------------------- */
 typedef struct {
     int64_t nx, ny, nz;

@@ -115,7 +115,7 @@ def test_vertical_ray_downward_closed_form(oracle):
 def test_vertical_ray_upward_closed_form(oracle):
     """theta = pi: the ray enters at y = -1 and takes N = n + 1 samples at y = -1 .. n - 1, so
     the clip box [-1, n] leaves the last row half weighted:
-    sum_{r < n-1} row_r + 1/2 row_{n-1} (intersection.h:104-105, forward_2d.hpp:63-75)."""
+    sum_{r < n-1} row_r + 1/2 row_{n-1} (intersection.h:38-39, forward_2d.hpp:63-75)."""
     n = 8
     img = rng(3).random((n, n))
     g = _axis_geom(n, np.pi)
