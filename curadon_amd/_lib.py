@@ -125,6 +125,8 @@ SIGNATURES = {
     "cmi_backward_2d_weighted": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_adjoint_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_plan3d_create": (_i32, [_i32, _i32, _i32, C.POINTER(Cone3DDesc), C.POINTER(_vp)]),
+    "cmi_plan3d_create_vec": (_i32, [_i32, _i32, _i32, _u64p, _dp, _dp, _u64p, _dp, _u64,
+                                     C.POINTER(_vp)]),
     "cmi_plan3d_destroy": (_i32, [_vp]),
     "cmi_plan3d_get_info": (_i32, [_vp, C.POINTER(Plan3DInfo)]),
     "cmi_forward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
@@ -343,11 +345,11 @@ def _check_pair_2d(volume, sinogram, plan, which):
     if volume.device != sinogram.device:
         raise AttributeError("Volume and sinogram must be on the same device")   # forward.cu:95-97
     if volume.device.index != plan.device:
-        raise AttributeError("Volume and plan are not on the same device")         # :165
+        raise AttributeError("Volume and plan are not on the same device")  # forward.cu:99-101
     vc, sc = tensor_code(volume), tensor_code(sinogram)
     if vc != plan.vol_prec:
         raise AttributeError(
-            f"Plan precision does not match volume precision ({_CODE_NAME[vc]})")  # :174-176
+            f"Plan precision does not match volume precision ({_CODE_NAME[vc]})")  # forward.cu:108-110
     if sc != plan.det_prec:
         raise AttributeError(
             f"Plan precision does not match sinogram precision ({_CODE_NAME[sc]})")
@@ -472,6 +474,40 @@ class plan_3d:
                                      C.byref(h)))
         self._h = h
 
+    @classmethod
+    def from_views(cls, device, vol_prec, det_prec, vol_shape, vol_spacing, vol_offset,
+                   det_shape, vecs):
+        """Plan from ASTRA-like cone views ([A][12]: source, panel centre, u-step, v-step in
+        world (x, y, z); cmi_plan3d_create_vec). Shapes in Python axis order: vol_shape /
+        vol_spacing / vol_offset (z, y, x), det_shape (v, u)."""
+        self = cls.__new__(cls)
+        self._h = None
+        v = np.ascontiguousarray(np.asarray(vecs, dtype=np.float64))
+        if v.ndim != 2 or v.shape[1] != 12:
+            raise AttributeError("vecs must have shape (nangles, 12)")
+        vs = np.ascontiguousarray(np.asarray(vol_shape, dtype=np.uint64).reshape(-1))
+        ds = np.ascontiguousarray(np.asarray(det_shape, dtype=np.uint64).reshape(-1))
+        if vs.size != 3 or ds.size != 2:
+            raise AttributeError("vol_shape needs 3 values and det_shape 2")
+        sp = _f64(vol_spacing, 3)
+        of = _f64(vol_offset, 3)
+        self.device = int(device)
+        self.vol_prec = precision_code(vol_prec)
+        self.det_prec = precision_code(det_prec)
+        self.vol_shape = tuple(int(x) for x in vs)
+        A = int(v.shape[0])
+        self.sino_shape = (A, int(ds[0]), int(ds[1]))
+        self.nangles = A
+        self._desc = None
+        self._keep = (v, vs, ds, sp, of)
+        h = _vp()
+        check(_lib.cmi_plan3d_create_vec(
+            self.device, self.vol_prec, self.det_prec, vs.ctypes.data_as(_u64p),
+            sp.ctypes.data_as(_dp), of.ctypes.data_as(_dp), ds.ctypes.data_as(_u64p),
+            v.ctypes.data_as(_dp), A, C.byref(h)))
+        self._h = h
+        return self
+
     @property
     def handle(self):
         return self._h
@@ -479,7 +515,9 @@ class plan_3d:
     def bp_matrices(self, mode="fdk", with_layout=False):
         """(A, 3, 4) FP32 projection matrices cmi_backward_3d uses in `mode`; with_layout also
         returns whether their U and W rows are free of y (the backprojectors then run along
-        y)."""
+        y). Plans from views have no descriptor and raise."""
+        if self._desc is None:
+            raise NotImplementedError("backprojection matrices of a plan from views")
         return _bp_matrices_of(self._desc, mode, with_layout)
 
     def info(self) -> dict:

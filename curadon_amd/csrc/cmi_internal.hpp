@@ -61,6 +61,17 @@ struct DeviceGuard {
     ~DeviceGuard();
 };
 
+// The 2D forward in parts (fan2d.hip), shared by cmi_forward_2d (stream-ordered scratch) and
+// cmi_forward_2d_noalloc (the library's static scratch): host validation; the scratch bytes of a
+// call; staging and march into caller-provided scratch, with the plan's device current.
+cmi_status fan2d_forward_check(const cmi_plan2d *plan, const void *volume, int vol_dtype,
+                               const void *sinogram, int sino_dtype, uint64_t batch);
+size_t fan2d_forward_scratch_bytes(const cmi_plan2d *plan, int vol_dtype, uint64_t batch);
+int fan2d_plan_device(const cmi_plan2d *plan);
+cmi_status fan2d_forward_enqueue(const cmi_plan2d *plan, const void *volume, int vol_dtype,
+                                 void *sinogram, int sino_dtype, uint64_t batch, void *scratch,
+                                 hipStream_t s);
+
 // ------------------------------------------------------------------ storage types ------
 
 // Storage tags. Arithmetic is always FP32; these only say how an element is stored.

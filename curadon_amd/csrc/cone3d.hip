@@ -46,6 +46,7 @@ struct cmi_plan3d {
     float *d_mmat = nullptr;
     float *d_mfdk = nullptr;
     bool ycols[3] = {false, false, false}; // per cmi_bp3d_mode: U and W rows free of y
+    bool views = false; // built from ASTRA-like views: reference mode = matched, no FDK mode
 };
 
 namespace {
@@ -569,24 +570,22 @@ bool y_invariant(const std::vector<float> &m, int64_t A) {
 //              v); the lower-tap coordinate is u - 0.5.
 //   matched:   the forward frame: voxel P = src + t (org + iu du + iv dv - src) solved for
 //              (t, t iu, t iv); the lower-tap coordinate of pixel centre iu is iu.
-void bp_matrices(const cone3d::Desc &g, const double *phi, const double *theta, const double *psi,
-                 int64_t A, std::vector<float> &mref, std::vector<float> &mmat,
-                 std::vector<float> &mfdk) {
-    mref.assign(12 * A, 0.0f);
-    mmat.assign(12 * A, 0.0f);
-    mfdk.assign(12 * A, 0.0f);
+// Rows are scaled by `fixed_scale` if it is positive, else so that W = 1 at the volume centre.
+void finish_matrix(const cone3d::Desc &g, double rows[3][4], float *out, double fixed_scale) {
     const double c[3] = {((double)g.nx - 1.0) / 2.0, ((double)g.ny - 1.0) / 2.0,
                          ((double)g.nz - 1.0) / 2.0};
-    // fixed_scale > 0: multiply the rows by it instead of normalising W to 1 at the centre
-    auto finish = [&](double rows[3][4], float *out, double fixed_scale) {
-        for (int r = 0; r < 3; ++r)
-            rows[r][3] += rows[r][0] * c[0] + rows[r][1] * c[1] + rows[r][2] * c[2];
-        const double w = rows[2][3];
-        const double sc = fixed_scale > 0.0 ? fixed_scale
-                                            : ((w != 0.0 && std::isfinite(w)) ? 1.0 / w : 1.0);
-        for (int r = 0; r < 3; ++r)
-            for (int k = 0; k < 4; ++k) out[4 * r + k] = (float)(rows[r][k] * sc);
-    };
+    for (int r = 0; r < 3; ++r)
+        rows[r][3] += rows[r][0] * c[0] + rows[r][1] * c[1] + rows[r][2] * c[2];
+    const double w = rows[2][3];
+    const double sc = fixed_scale > 0.0 ? fixed_scale
+                                        : ((w != 0.0 && std::isfinite(w)) ? 1.0 / w : 1.0);
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 4; ++k) out[4 * r + k] = (float)(rows[r][k] * sc);
+}
+
+void reference_matrices(const cone3d::Desc &g, const double *phi, const double *theta,
+                        const double *psi, int64_t A, std::vector<float> &mref) {
+    mref.assign(12 * A, 0.0f);
     const double hu = (double)(g.nu / 2), hv = (double)(g.nv / 2);
     for (int64_t a = 0; a < A; ++a) {
         const cone3d::BpGeom bg = cone3d::bp_angle(g, phi[a], theta[a], psi[a]);
@@ -602,9 +601,19 @@ void bp_matrices(const cone3d::Desc &g, const double *phi, const double *theta, 
             rr[1][k] = cc * rowY[k] + (bg.src[1] + hv - 0.5) * rowW[k];
             rr[2][k] = rowW[k];
         }
-        finish(rr, mref.data() + 12 * a, 0.0);
+        finish_matrix(g, rr, mref.data() + 12 * a, 0.0);
+    }
+}
 
-        const FwdGeom fg = cone3d::fwd_angle(g, phi[a], theta[a], psi[a]);
+// Matched matrices from the forward frames; with `mfdk`, also the FDK ones: W = t (the voxel's
+// parameter on its ray, 1 at the panel) times DSD / DSO, so that W = depth / DSO.
+void matched_matrices(const cone3d::Desc &g, const std::vector<FwdGeom> &fgeo,
+                      std::vector<float> &mmat, std::vector<float> *mfdk) {
+    const int64_t A = (int64_t)fgeo.size();
+    mmat.assign(12 * A, 0.0f);
+    if (mfdk) mfdk->assign(12 * A, 0.0f);
+    for (int64_t a = 0; a < A; ++a) {
+        const FwdGeom &fg = fgeo[a];
         double Mc[3][3]; // columns org - src, du, dv
         for (int d = 0; d < 3; ++d) {
             Mc[d][0] = fg.org[d] - fg.src[d];
@@ -626,20 +635,27 @@ void bp_matrices(const cone3d::Desc &g, const double *phi, const double *theta, 
         inv[2][2] = (Mc[0][0] * Mc[1][1] - Mc[0][1] * Mc[1][0]) / det;
         // rows ordered (t iu, t iv, t) = (U, V, W)
         const int order[3] = {1, 2, 0};
+        double rr[3][4], rk[3][4];
         for (int r = 0; r < 3; ++r) {
             const double *ir = inv[order[r]];
             rr[r][0] = ir[0];
             rr[r][1] = ir[1];
             rr[r][2] = ir[2];
             rr[r][3] = -(ir[0] * fg.src[0] + ir[1] * fg.src[1] + ir[2] * fg.src[2]);
-        }
-        double rk[3][4];
-        for (int r = 0; r < 3; ++r)
             for (int k = 0; k < 4; ++k) rk[r][k] = rr[r][k];
-        finish(rr, mmat.data() + 12 * a, 0.0);
-        // W = t (the voxel's parameter on its ray, 1 at the panel); depth = t DSD
-        finish(rk, mfdk.data() + 12 * a, g.dsd / g.dso);
+        }
+        finish_matrix(g, rr, mmat.data() + 12 * a, 0.0);
+        if (mfdk) finish_matrix(g, rk, mfdk->data() + 12 * a, g.dsd / g.dso);
     }
+}
+
+void bp_matrices(const cone3d::Desc &g, const double *phi, const double *theta, const double *psi,
+                 int64_t A, std::vector<float> &mref, std::vector<float> &mmat,
+                 std::vector<float> &mfdk) {
+    reference_matrices(g, phi, theta, psi, A, mref);
+    std::vector<FwdGeom> fgeo(A);
+    for (int64_t a = 0; a < A; ++a) fgeo[a] = cone3d::fwd_angle(g, phi[a], theta[a], psi[a]);
+    matched_matrices(g, fgeo, mmat, &mfdk);
 }
 
 // Python order (z, y, x) / (v, u) -> (x, y, z) / (u, v), as forward.cu:113-124
@@ -736,6 +752,79 @@ cmi_status check_call(const cmi_plan3d *p, const void *vol, int vol_dtype, const
     return CMI_OK;
 }
 
+// Shared tail of the plan constructors: per-angle fast axes and volume copies, matrix layouts,
+// device tables and the total sample count (k_count3d). `mfdk` is empty for plans from views.
+cmi_status finish_plan3d(std::unique_ptr<cmi_plan3d> p, const std::vector<FwdGeom> &fgeo,
+                         const std::vector<float> &mref, const std::vector<float> &mmat,
+                         const std::vector<float> &mfdk, cmi_plan3d **plan) {
+    const cone3d::Desc &g = p->g;
+    const int64_t A = p->A;
+    std::vector<int> fast(A);
+    for (int64_t a = 0; a < A; ++a) {
+        const double *v = fgeo[a].dv;
+        const int f = std::fabs(v[1]) >= std::fabs(v[0]) && std::fabs(v[1]) >= std::fabs(v[2]) ? 1
+                      : std::fabs(v[2]) > std::fabs(v[0])                                   ? 2
+                                                                                            : 0;
+        fast[a] = f;
+        if (p->copy_slot[f] < 0) p->copy_slot[f] = p->ncopies++;
+    }
+    p->ycols[CMI_BP3D_REFERENCE] = y_invariant(mref, A);
+    p->ycols[CMI_BP3D_MATCHED] = y_invariant(mmat, A);
+    p->ycols[CMI_BP3D_FDK] = !mfdk.empty() && y_invariant(mfdk, A);
+
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(p->device));
+    struct Cleanup {
+        cmi_plan3d *p;
+        bool armed = true;
+        ~Cleanup() {
+            if (!armed) return;
+            (void)hipFree(p->d_fgeo);
+            (void)hipFree(p->d_fast);
+            (void)hipFree(p->d_mref);
+            (void)hipFree(p->d_mmat);
+            (void)hipFree(p->d_mfdk);
+        }
+    } cleanup{p.get()};
+    CMI_HIP_TRY(hipMalloc(&p->d_fgeo, sizeof(FwdGeom) * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_fast, sizeof(int) * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_mref, sizeof(float) * 12 * A));
+    CMI_HIP_TRY(hipMalloc(&p->d_mmat, sizeof(float) * 12 * A));
+    if (!mfdk.empty()) CMI_HIP_TRY(hipMalloc(&p->d_mfdk, sizeof(float) * 12 * A));
+    p->table_bytes = (sizeof(FwdGeom) + sizeof(int) + (mfdk.empty() ? 2 : 3) * 12 * sizeof(float)) * A;
+    unsigned long long *d_total = nullptr;
+    CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
+    std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
+        d_total, [](unsigned long long *q) { (void)hipFree(q); });
+    hipStream_t s;
+    CMI_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<ihipStream_t, void (*)(hipStream_t)> stream_guard(
+        s, [](hipStream_t q) { (void)hipStreamDestroy(q); });
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_fgeo, fgeo.data(), sizeof(FwdGeom) * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_fast, fast.data(), sizeof(int) * A, hipMemcpyHostToDevice,
+                               s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_mref, mref.data(), sizeof(float) * 12 * A,
+                               hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemcpyAsync(p->d_mmat, mmat.data(), sizeof(float) * 12 * A,
+                               hipMemcpyHostToDevice, s));
+    if (!mfdk.empty())
+        CMI_HIP_TRY(hipMemcpyAsync(p->d_mfdk, mfdk.data(), sizeof(float) * 12 * A,
+                                   hipMemcpyHostToDevice, s));
+    CMI_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(unsigned long long), s));
+    dim3 blk(256), grd((unsigned)((g.nu + 255) / 256), (unsigned)g.nv, (unsigned)A);
+    hipLaunchKernelGGL(k_count3d, grd, blk, 0, s, p->d_fgeo, (int)g.nu, (int)g.nv, (int)g.nx,
+                       (int)g.ny, (int)g.nz, d_total);
+    CMI_HIP_TRY(hipGetLastError());
+    unsigned long long total = 0;
+    CMI_HIP_TRY(hipMemcpyAsync(&total, d_total, sizeof total, hipMemcpyDeviceToHost, s));
+    CMI_HIP_TRY(hipStreamSynchronize(s));
+    p->total_samples = total;
+    cleanup.armed = false;
+    *plan = p.release();
+    return CMI_OK;
+}
+
 } // namespace
 
 // ===================================================================== C-ABI ===============
@@ -766,74 +855,94 @@ CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
     const cone3d::Desc &g = p->g;
     p->A = (int64_t)d->nangles;
     const int64_t A = p->A;
-
     std::vector<FwdGeom> fgeo(A);
-    std::vector<int> fast(A);
-    for (int64_t a = 0; a < A; ++a) {
-        fgeo[a] = cone3d::fwd_angle(g, d->phi[a], d->theta[a], d->psi[a]);
-        const double *v = fgeo[a].dv;
-        const int f = std::fabs(v[1]) >= std::fabs(v[0]) && std::fabs(v[1]) >= std::fabs(v[2]) ? 1
-                      : std::fabs(v[2]) > std::fabs(v[0])                                   ? 2
-                                                                                            : 0;
-        fast[a] = f;
-        if (p->copy_slot[f] < 0) p->copy_slot[f] = p->ncopies++;
-    }
+    for (int64_t a = 0; a < A; ++a) fgeo[a] = cone3d::fwd_angle(g, d->phi[a], d->theta[a], d->psi[a]);
     std::vector<float> mref, mmat, mfdk;
-    bp_matrices(g, d->phi, d->theta, d->psi, A, mref, mmat, mfdk);
-    p->ycols[CMI_BP3D_REFERENCE] = y_invariant(mref, A);
-    p->ycols[CMI_BP3D_MATCHED] = y_invariant(mmat, A);
-    p->ycols[CMI_BP3D_FDK] = y_invariant(mfdk, A);
+    reference_matrices(g, d->phi, d->theta, d->psi, A, mref);
+    matched_matrices(g, fgeo, mmat, &mfdk);
+    return finish_plan3d(std::move(p), fgeo, mref, mmat, mfdk, plan);
+}
 
-    DeviceGuard guard;
-    CMI_TRY(guard.enter(device));
-    struct Cleanup {
-        cmi_plan3d *p;
-        bool armed = true;
-        ~Cleanup() {
-            if (!armed) return;
-            (void)hipFree(p->d_fgeo);
-            (void)hipFree(p->d_fast);
-            (void)hipFree(p->d_mref);
-            (void)hipFree(p->d_mmat);
-            (void)hipFree(p->d_mfdk);
-        }
-    } cleanup{p.get()};
-    CMI_HIP_TRY(hipMalloc(&p->d_fgeo, sizeof(FwdGeom) * A));
-    CMI_HIP_TRY(hipMalloc(&p->d_fast, sizeof(int) * A));
-    CMI_HIP_TRY(hipMalloc(&p->d_mref, sizeof(float) * 12 * A));
-    CMI_HIP_TRY(hipMalloc(&p->d_mmat, sizeof(float) * 12 * A));
-    CMI_HIP_TRY(hipMalloc(&p->d_mfdk, sizeof(float) * 12 * A));
-    p->table_bytes = (sizeof(FwdGeom) + sizeof(int) + 3 * 12 * sizeof(float)) * A;
-    unsigned long long *d_total = nullptr;
-    CMI_HIP_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
-    std::unique_ptr<unsigned long long, void (*)(unsigned long long *)> total_guard(
-        d_total, [](unsigned long long *q) { (void)hipFree(q); });
-    hipStream_t s;
-    CMI_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    std::unique_ptr<ihipStream_t, void (*)(hipStream_t)> stream_guard(
-        s, [](hipStream_t q) { (void)hipStreamDestroy(q); });
-    CMI_HIP_TRY(hipMemcpyAsync(p->d_fgeo, fgeo.data(), sizeof(FwdGeom) * A,
-                               hipMemcpyHostToDevice, s));
-    CMI_HIP_TRY(hipMemcpyAsync(p->d_fast, fast.data(), sizeof(int) * A, hipMemcpyHostToDevice,
-                               s));
-    CMI_HIP_TRY(hipMemcpyAsync(p->d_mref, mref.data(), sizeof(float) * 12 * A,
-                               hipMemcpyHostToDevice, s));
-    CMI_HIP_TRY(hipMemcpyAsync(p->d_mmat, mmat.data(), sizeof(float) * 12 * A,
-                               hipMemcpyHostToDevice, s));
-    CMI_HIP_TRY(hipMemcpyAsync(p->d_mfdk, mfdk.data(), sizeof(float) * 12 * A,
-                               hipMemcpyHostToDevice, s));
-    CMI_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(unsigned long long), s));
-    dim3 blk(256), grd((unsigned)((g.nu + 255) / 256), (unsigned)g.nv, (unsigned)A);
-    hipLaunchKernelGGL(k_count3d, grd, blk, 0, s, p->d_fgeo, (int)g.nu, (int)g.nv, (int)g.nx,
-                       (int)g.ny, (int)g.nz, d_total);
-    CMI_HIP_TRY(hipGetLastError());
-    unsigned long long total = 0;
-    CMI_HIP_TRY(hipMemcpyAsync(&total, d_total, sizeof total, hipMemcpyDeviceToHost, s));
-    CMI_HIP_TRY(hipStreamSynchronize(s));
-    p->total_samples = total;
-    cleanup.armed = false;
-    *plan = p.release();
-    return CMI_OK;
+CMI_API cmi_status cmi_plan3d_create_vec(int device, int vol_prec, int det_prec,
+                                         const uint64_t vol_shape[3], const double vol_spacing[3],
+                                         const double vol_offset[3], const uint64_t det_shape[2],
+                                         const double *vecs, uint64_t nangles,
+                                         cmi_plan3d **plan) {
+    clear_error();
+    if (!plan) return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create_vec: output pointer is null");
+    *plan = nullptr;
+    if (!vol_shape || !vol_spacing || !vol_offset || !det_shape || !vecs)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create_vec: null geometry array");
+    if (!dtype_ok(vol_prec) || !dtype_ok(det_prec))
+        return fail(CMI_ERR_DTYPE, "plan3d_create_vec: precision must be f32, f16 or bf16");
+    // the shape, spacing and size limits of cmi_plan3d_create, through a descriptor
+    const double zero3[3] = {0.0, 0.0, 0.0};
+    cmi_cone3d_desc d{};
+    for (int i = 0; i < 3; ++i) {
+        d.vol_shape[i] = vol_shape[i];
+        d.vol_spacing[i] = vol_spacing[i];
+        d.vol_offset[i] = vol_offset[i];
+    }
+    for (int i = 0; i < 2; ++i) {
+        d.det_shape[i] = det_shape[i];
+        d.det_spacing[i] = 1.0;
+    }
+    d.DSO = d.DSD = 1.0;
+    d.nangles = nangles;
+    d.phi = d.theta = d.psi = zero3; // not read past index 0 below; angles come from the views
+    if (nangles == 0 || nangles > 65535)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create_vec: nangles must be in [1, 65535]");
+    {
+        cmi_cone3d_desc d1 = d;
+        d1.nangles = 1;
+        CMI_TRY(check_desc(&d1, "plan3d_create_vec"));
+    }
+    for (int i = 0; i < 3; ++i)
+        if (!std::isfinite(vol_offset[i]))
+            return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create_vec: non-finite volume offset");
+    const uint64_t padvol = (vol_shape[0] + 2 * kPad) * (vol_shape[1] + 2 * kPad) *
+                            (vol_shape[2] + 2 * kPad);
+    const uint64_t padpan = (det_shape[0] + 2 * kPad) * (det_shape[1] + 2 * kPad) * nangles;
+    if (padvol * 4 >= (1ull << 31) || padpan * 4 >= (1ull << 31))
+        return fail(CMI_ERR_INVALID_ARGUMENT,
+                    "plan3d_create_vec: volume or projection stack exceeds 2 GiB");
+    for (uint64_t a = 0; a < nangles; ++a) {
+        const double *v = vecs + 12 * a;
+        for (int i = 0; i < 12; ++i)
+            if (!std::isfinite(v[i]))
+                return fail(CMI_ERR_INVALID_ARGUMENT, "plan3d_create_vec: non-finite view %llu",
+                            (unsigned long long)a);
+        // the panel normal u x v must be non-zero and the source off the panel plane
+        const double n[3] = {v[7] * v[11] - v[8] * v[10], v[8] * v[9] - v[6] * v[11],
+                             v[6] * v[10] - v[7] * v[9]};
+        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        const double L = (n[0] * (v[3] - v[0]) + n[1] * (v[4] - v[1]) + n[2] * (v[5] - v[2]));
+        const double ul = std::sqrt(v[6] * v[6] + v[7] * v[7] + v[8] * v[8]);
+        const double vl = std::sqrt(v[9] * v[9] + v[10] * v[10] + v[11] * v[11]);
+        if (!(nn > 1e-12 * ul * vl) || !(std::fabs(L) > 1e-9 * nn))
+            return fail(CMI_ERR_INVALID_ARGUMENT,
+                        "plan3d_create_vec: view %llu has parallel or zero u / v steps or its "
+                        "source on the panel plane",
+                        (unsigned long long)a);
+    }
+    auto p = std::make_unique<cmi_plan3d>();
+    p->device = device;
+    p->vol_prec = vol_prec;
+    p->det_prec = det_prec;
+    p->views = true;
+    p->g = desc_of(&d);
+    p->g.du = std::sqrt(vecs[6] * vecs[6] + vecs[7] * vecs[7] + vecs[8] * vecs[8]);
+    p->g.dv = std::sqrt(vecs[9] * vecs[9] + vecs[10] * vecs[10] + vecs[11] * vecs[11]);
+    p->g.dso = p->g.dsd = 0.0; // no circular-orbit distances: no FDK weighting
+    p->A = (int64_t)nangles;
+    const int64_t A = p->A;
+    std::vector<FwdGeom> fgeo(A);
+    for (int64_t a = 0; a < A; ++a) fgeo[a] = cone3d::fwd_view(p->g, vecs + 12 * a);
+    std::vector<float> mmat;
+    matched_matrices(p->g, fgeo, mmat, nullptr);
+    // reference mode of a plan from views projects like matched mode (the reference's own
+    // backprojection frame is defined by Euler angles only)
+    return finish_plan3d(std::move(p), fgeo, mmat, mmat, {}, plan);
 }
 
 CMI_API cmi_status cmi_cone3d_bp_matrices(const cmi_cone3d_desc *d, int mode, float *out,
@@ -948,6 +1057,10 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
     CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "backward_3d"));
     if (mode != CMI_BP3D_REFERENCE && mode != CMI_BP3D_MATCHED && mode != CMI_BP3D_FDK)
         return fail(CMI_ERR_INVALID_ARGUMENT, "backward_3d: unknown mode %d", mode);
+    if (mode == CMI_BP3D_FDK && plan->views)
+        return fail(CMI_ERR_NOT_SUPPORTED,
+                    "backward_3d: FDK weighting needs a circular orbit (DSO, DSD); this plan "
+                    "was built from views");
     DeviceGuard guard;
     CMI_TRY(guard.enter(plan->device));
     const hipStream_t s = (hipStream_t)stream;

@@ -930,6 +930,51 @@ cmi_status finish_plan2d(std::unique_ptr<cmi_plan2d> p, const std::vector<AngleG
 
 } // namespace
 
+// ------------------------------------------------------ forward, shared by the entry points --
+
+namespace cmi {
+
+cmi_status fan2d_forward_check(const cmi_plan2d *plan, const void *volume, int vol_dtype,
+                               const void *sinogram, int sino_dtype, uint64_t batch) {
+    return check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "forward_2d");
+}
+
+int fan2d_plan_device(const cmi_plan2d *plan) { return plan->device; }
+
+// Two zero-bordered copies of each image of the batch (row-major and transposed).
+size_t fan2d_forward_scratch_bytes(const cmi_plan2d *plan, int vol_dtype, uint64_t batch) {
+    const size_t Wp = (size_t)plan->g.nx + 2 * kPad, Hp = (size_t)plan->g.ny + 2 * kPad;
+    return 2 * Wp * Hp * dtype_size(vol_dtype) * batch;
+}
+
+// Stages the batch into `scratch` (at least fan2d_forward_scratch_bytes bytes of device memory
+// on the plan's device, which is current) and launches the march, on `s`. Every staged element,
+// border included, is written, so the scratch needs no clearing between calls.
+cmi_status fan2d_forward_enqueue(const cmi_plan2d *plan, const void *volume, int vol_dtype,
+                                 void *sinogram, int sino_dtype, uint64_t batch, void *scratch,
+                                 hipStream_t s) {
+    const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad;
+    return dispatch2(vol_dtype, sino_dtype, [&](auto si, auto ss) -> cmi_status {
+        using SI = decltype(si);
+        using SS = decltype(ss);
+        using RI = typename SI::raw;
+        dim3 sb(32, 8), sg((unsigned)((Wp + 31) / 32), (unsigned)((Hp + 31) / 32),
+                           (unsigned)batch);
+        hipLaunchKernelGGL((k_stage2d<SI>), sg, sb, 0, s, (const RI *)volume, (RI *)scratch, nx,
+                           ny, Wp, Hp);
+        // block_x * block_y (at most kFwdRays) adjacent cells of one angle per workgroup
+        const unsigned nt = std::min<unsigned>(plan->block_x * plan->block_y, kFwdRays);
+        dim3 fb(nt), fg((unsigned)((D + nt - 1) / nt), (unsigned)A, (unsigned)batch);
+        hipLaunchKernelGGL((k_fwd2d<SI, SS>), fg, fb, 0, s, (const RI *)scratch, Wp, Hp,
+                           plan->d_ev, plan->d_ns, (typename SS::raw *)sinogram, D, A);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+}
+
+} // namespace cmi
+
 // ===================================================================== C-ABI ===============
 
 extern "C" {
@@ -1092,32 +1137,14 @@ CMI_API cmi_status cmi_forward_2d(const cmi_plan2d *plan, const void *volume, in
                                   void *sinogram, int sino_dtype, uint64_t batch,
                                   void *stream) {
     clear_error();
-    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch, "forward_2d"));
+    CMI_TRY(fan2d_forward_check(plan, volume, vol_dtype, sinogram, sino_dtype, batch));
     DeviceGuard guard;
     CMI_TRY(guard.enter(plan->device));
     const hipStream_t s = (hipStream_t)stream;
-    const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
-    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad;
-    const size_t esz = dtype_size(vol_dtype);
-    const size_t one = (size_t)Wp * Hp * esz; // bytes of one padded copy
     void *scratch = nullptr;
-    CMI_HIP_TRY(hipMallocAsync(&scratch, 2 * one * batch, s));
-    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto si, auto ss) -> cmi_status {
-        using SI = decltype(si);
-        using SS = decltype(ss);
-        using RI = typename SI::raw;
-        dim3 sb(32, 8), sg((unsigned)((Wp + 31) / 32), (unsigned)((Hp + 31) / 32),
-                           (unsigned)batch);
-        hipLaunchKernelGGL((k_stage2d<SI>), sg, sb, 0, s, (const RI *)volume, (RI *)scratch, nx,
-                           ny, Wp, Hp);
-        // block_x * block_y (at most kFwdRays) adjacent cells of one angle per workgroup
-        const unsigned nt = std::min<unsigned>(plan->block_x * plan->block_y, kFwdRays);
-        dim3 fb(nt), fg((unsigned)((D + nt - 1) / nt), (unsigned)A, (unsigned)batch);
-        hipLaunchKernelGGL((k_fwd2d<SI, SS>), fg, fb, 0, s, (const RI *)scratch, Wp, Hp,
-                           plan->d_ev, plan->d_ns, (typename SS::raw *)sinogram, D, A);
-        CMI_HIP_TRY(hipGetLastError());
-        return CMI_OK;
-    });
+    CMI_HIP_TRY(hipMallocAsync(&scratch, fan2d_forward_scratch_bytes(plan, vol_dtype, batch), s));
+    const cmi_status st =
+        fan2d_forward_enqueue(plan, volume, vol_dtype, sinogram, sino_dtype, batch, scratch, s);
     const hipError_t fe = hipFreeAsync(scratch, s);
     if (st != CMI_OK) return st;
     CMI_HIP_TRY(fe);

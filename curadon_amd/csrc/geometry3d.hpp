@@ -96,6 +96,29 @@ inline FwdGeom fwd_angle(const Desc &g, double phi, double theta, double psi) {
     return r;
 }
 
+// ASTRA-like cone view: v = (source x, y, z, detector centre x, y, z, u-step x, y, z, v-step x,
+// y, z) in world coordinates (the volume's centre at the volume offset), the u-step going from
+// pixel (iu, iv) to (iu + 1, iv) and the v-step to (iu, iv + 1). No counterpart in the
+// reference, whose cone geometry takes Euler angles only and lists projection matrices as
+// future work (README.md:58-59). Pixel (0, 0)'s centre is c - (U - 1)/2 u - (V - 1)/2 v, the
+// reference's detector origin (forward_3d.hpp:123-125) for a centred panel; the voxel frame
+// follows as in fwd_angle (translation by vol_size/2 - spacing/2 - offset, then / spacing).
+inline FwdGeom fwd_view(const Desc &g, const double v[12]) {
+    const double sp[3] = {g.sx, g.sy, g.sz}, off[3] = {g.ox, g.oy, g.oz};
+    const double n[3] = {(double)g.nx, (double)g.ny, (double)g.nz};
+    const double hu = ((double)g.nu - 1.0) / 2.0, hv = ((double)g.nv - 1.0) / 2.0;
+    FwdGeom r{};
+    for (int d = 0; d < 3; ++d) {
+        const double tr = n[d] * sp[d] / 2.0 - sp[d] / 2.0;
+        const double o = v[3 + d] - hu * v[6 + d] - hv * v[9 + d];
+        r.org[d] = (o - off[d] + tr) / sp[d];
+        r.du[d] = v[6 + d] / sp[d];
+        r.dv[d] = v[9 + d] / sp[d];
+        r.src[d] = (v[d] - off[d] + tr) / sp[d];
+    }
+    return r;
+}
+
 // Backprojector frame of the reference (backward_3d.hpp:110-178, 218-227): rotated position of
 // voxel (0,0,0), the three voxel steps, and the source.
 struct BpGeom {

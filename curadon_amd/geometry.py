@@ -25,6 +25,25 @@ def _as_float(val: Any) -> float:
     return float(val)
 
 
+def _rot_yzy(phi, theta, psi):
+    """The trajectory rotation of the reference's cone beam, the matrix written in
+    include/curadon/detail/rotation.hpp:18-21."""
+    cf, sf, ct, st, cp, sp = (np.cos(phi), np.sin(phi), np.cos(theta), np.sin(theta),
+                              np.cos(psi), np.sin(psi))
+    return np.array([[cf * cp * ct - sf * sp, -st * cf, sf * cp + sp * cf * ct],
+                     [st * cp, ct, sp * st],
+                     [-sf * cp * ct - sp * cf, sf * st, -sf * sp * ct + cp * cf]])
+
+
+def _rot_rpy(roll, pitch, yaw):
+    """The detector rotation of the reference's cone beam (rotation.hpp:31-48)."""
+    cr, sr, cp, sp, cy, sy = (np.cos(roll), np.sin(roll), np.cos(pitch), np.sin(pitch),
+                              np.cos(yaw), np.sin(yaw))
+    return np.array([[cr * cp, cr * sp * sy - sr * cy, cr * sp * cy + sr * sy],
+                     [sr * cp, sr * sp * sy + cr * cy, sr * sp * cy - cr * sy],
+                     [-sp, cp * sy, cp * cy]])
+
+
 def _fp(a) -> bytes:
     """Bytes of an array-like as FP64, for plan fingerprints."""
     a = np.asarray(a, dtype=np.float64)
@@ -297,8 +316,67 @@ class ConeGeometry:
         self.device = int(device)
         self.vol_prec = vol_prec
         self.sino_prec = sino_prec
+        self.vecs = None  # ASTRA-like views when built by from_vec
         self._plan = None
         self._plan_key = None
+
+    def to_vec(self):
+        """ASTRA-like cone views, one row per projection: (source x, y, z, panel centre x, y,
+        z, u-step x, y, z, v-step x, y, z) in world coordinates (the volume's centre at
+        vol_offset), the steps going from pixel (iu, iv) to (iu + 1, iv) and to (iu, iv + 1).
+        For a geometry from Euler angles these are the points of the reference's forward set-up
+        (forward_3d.hpp:118-190: detector roll / pitch / yaw, the shift by DSD - DSO, the
+        trajectory rotation and the COR shift) before its translation into voxel coordinates,
+        so ``from_vec(to_vec())`` projects like this geometry. The reference has no vector form
+        for cone beam (its README.md:58-59 lists projection matrices as future work). A geometry
+        made by ``from_vec`` returns its views."""
+        if self.vecs is not None:
+            return np.array(self.vecs)
+        du, dv = float(self.det_spacing[1]), float(self.det_spacing[0])
+        V, U = (int(v) for v in self.det_shape)
+        pitch, roll, yaw = (float(v) for v in self.det_rotation)
+        dod = self.DSD - self.DSO
+        p00 = np.array([-du * (U / 2.0) + du * 0.5, -dv * (V / 2.0) + dv * 0.5, 0.0])
+        pts = [p00, p00 + [du, 0.0, 0.0], p00 + [0.0, dv, 0.0]]
+        Rd = _rot_rpy(roll, pitch, yaw)
+        pts = [Rd @ p + [0.0, 0.0, dod] for p in pts]
+        src0 = np.array([0.0, 0.0, -self.DSO])
+        out = np.zeros((self.nangles, 12))
+        for a in range(self.nangles):
+            phi, theta, psi = (float(v) for v in self.angles[:, a])
+            R = _rot_yzy(phi, theta, psi)
+            cor = self.COR * np.array([np.sin(phi), 0.0, np.cos(phi)])
+            q = [R @ p + cor for p in pts]
+            s = R @ src0 + cor
+            u, v = q[1] - q[0], q[2] - q[0]
+            c = q[0] + (U - 1) / 2.0 * u + (V - 1) / 2.0 * v
+            out[a] = np.concatenate([s, c, u, v])
+        return out
+
+    @staticmethod
+    def from_vec(vol_shape, det_shape, vecs, **kwargs):
+        """Build a ConeGeometry from ASTRA-like cone views (see ``to_vec``): any trajectory,
+        panel position, tilt and pixel pitch per projection. The geometry keeps the views
+        (``vecs``) and its plan projects through them (cmi_plan3d_create_vec): forward,
+        adjoint and backward in 'matched' mode follow every view; 'reference' mode projects
+        like 'matched' (the reference's backprojection frame exists for Euler angles only) and
+        'fdk' is refused. A summary is set for display and for code that reads it: DSD =
+        |source - panel centre|, DSO = |source|, angles = (atan2(-s.x, -s.z), 0, 0), det_spacing
+        = (|v|, |u|) of the first view."""
+        vecs = np.asarray(vecs, dtype=np.float64)
+        if vecs.ndim != 2 or vecs.shape[1] != 12:
+            raise AttributeError(f"vecs must have shape (nangles, 12), got {vecs.shape}")
+        src, det = vecs[:, 0:3], vecs[:, 3:6]
+        dsd = float(np.linalg.norm(src[0] - det[0]))
+        dso = float(np.linalg.norm(src[0]))
+        phi = np.arctan2(-src[:, 0], -src[:, 2]) % (2 * np.pi)
+        spacing = (float(np.linalg.norm(vecs[0, 9:12])), float(np.linalg.norm(vecs[0, 6:9])))
+        kw = dict(kwargs)
+        kw.setdefault("det_spacing", spacing)
+        g = ConeGeometry(DSD=dsd, DSO=dso, angles=phi, vol_shape=vol_shape, det_shape=det_shape,
+                         **kw)
+        g.vecs = np.array(vecs)
+        return g
 
     def fingerprint(self):
         """Every attribute the native plan is built from, as a hashable tuple: the plan is
@@ -307,7 +385,8 @@ class ConeGeometry:
         return (_fp(self.angles), _as_float(self.DSD), _as_float(self.DSO), _fp(self.vol_shape),
                 _fp(self.vol_spacing), _fp(self.vol_offset), _fp(self.det_shape),
                 _fp(self.det_spacing), _fp(self.det_offset), _fp(self.det_rotation),
-                _as_float(self.COR), int(self.device), str(self.vol_prec), str(self.sino_prec))
+                _as_float(self.COR), int(self.device), str(self.vol_prec), str(self.sino_prec),
+                None if self.vecs is None else _fp(self.vecs))
 
     def _plan_args(self):
         return (self.vol_shape, self.vol_spacing, self.vol_offset, self.angles[0],
@@ -322,8 +401,13 @@ class ConeGeometry:
         if self._plan is None or self._plan_key != key:
             from . import _lib
             self._plan = None
-            self._plan = _lib.plan_3d(self.device, self.vol_prec, self.sino_prec,
-                                      *self._plan_args())
+            if self.vecs is not None:
+                self._plan = _lib.plan_3d.from_views(
+                    self.device, self.vol_prec, self.sino_prec, self.vol_shape,
+                    self.vol_spacing, self.vol_offset, self.det_shape, self.vecs)
+            else:
+                self._plan = _lib.plan_3d(self.device, self.vol_prec, self.sino_prec,
+                                          *self._plan_args())
             self._plan_key = key
         return self._plan
 
@@ -332,6 +416,8 @@ class ConeGeometry:
         on the host without building a device plan (cmi_cone3d_bp_matrices); with_layout also
         returns whether their U and W rows are free of y."""
         from . import _lib
+        if self.vecs is not None:
+            raise NotImplementedError("backprojection matrices of a geometry from views")
         return _lib.cone3d_bp_matrices(*self._plan_args(), mode=mode, with_layout=with_layout)
 
     def sinogram_shape(self):

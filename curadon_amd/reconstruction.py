@@ -237,11 +237,11 @@ def fbp_filter(sinogram: torch.Tensor, geom, window="ram-lak", frequency_scaling
     backprojection (`backward(..., mode="fbp")` for fan beam, mode="fdk" for cone beam)."""
     dev = sinogram.device
     dt = torch.float64 if sinogram.dtype == torch.float64 else torch.float32
+    if isinstance(geom, (FanGeometry, ConeGeometry)) and geom.vecs is not None:
+        raise NotImplementedError(
+            "fbp: geometries built from ASTRA-like views (from_vec) have no FBP / FDK "
+            "weighting; use backward / adjoint, or a circular geometry")
     if isinstance(geom, FanGeometry):
-        if geom.vecs is not None:
-            raise NotImplementedError(
-                "fbp: geometries built from ASTRA-like views (from_vec) have no fan-beam FBP "
-                "weighting; use backward / adjoint, or a circular FanGeometry")
         cw = cosine_weights_fan(geom, dev).to(dt)
         scale = (math.pi / geom.nangles) * torch.as_tensor(
             geom.DSO / geom.DSD, dtype=dt, device=dev)[:, None]

@@ -239,9 +239,9 @@ class FBPServer:
         for i, j in enumerate(jobs):
             if j.id < 0:
                 j.id = i
-            if isinstance(j.geom, FanGeometry) and j.geom.vecs is not None:
+            if getattr(j.geom, "vecs", None) is not None:
                 raise NotImplementedError(
-                    f"job {j.id}: fan geometries from ASTRA-like views (from_vec) are not served")
+                    f"job {j.id}: geometries from ASTRA-like views (from_vec) are not served")
             need = sum(footprint(j, self.storage))
             if need > self.hbm_budget:
                 raise MemoryError(f"job {j.id} needs {need} B, more than the server's budget "

@@ -202,6 +202,28 @@ typedef struct cmi_plan3d cmi_plan3d;
  * backward_3d.hpp:181-253). This plan holds the per-angle tables on `device` instead. */
 CMI_API cmi_status cmi_plan3d_create(int device, int vol_prec, int det_prec,
                                      const cmi_cone3d_desc *desc, cmi_plan3d **plan);
+
+/*
+ * A 3D plan from ASTRA-like cone views, one per projection: vecs[12 a .. 12 a + 11] = (source
+ * x, y, z, panel centre x, y, z, u-step x, y, z, v-step x, y, z) in world coordinates (x, y, z;
+ * the volume's centre sits at vol_offset, as in cmi_plan3d_create), the u-step going from pixel
+ * (iu, iv) to (iu + 1, iv) and the v-step to (iu, iv + 1); pixel (0, 0)'s centre is
+ * centre - (U - 1)/2 u - (V - 1)/2 v. Shapes and spacings in the Python axis order of
+ * cmi_cone3d_desc: vol_shape / vol_spacing / vol_offset (z, y, x), det_shape (v, u). Every
+ * view may move the source and place, tilt and scale the panel freely (any trajectory, a
+ * projection-matrix geometry given by its source and panel). No counterpart in the reference,
+ * whose cone geometry takes Euler angles only and lists projection matrices as future work
+ * (README.md:58-59). cmi_forward_3d, cmi_adjoint_3d and cmi_backward_3d in CMI_BP3D_MATCHED
+ * mode follow the views; CMI_BP3D_REFERENCE mode projects like CMI_BP3D_MATCHED (the
+ * reference's own backprojection frame exists only for Euler angles) and CMI_BP3D_FDK returns
+ * CMI_ERR_NOT_SUPPORTED (its weight needs a circular orbit's DSO and DSD).
+ */
+CMI_API cmi_status cmi_plan3d_create_vec(int device, int vol_prec, int det_prec,
+                                         const uint64_t vol_shape[3], const double vol_spacing[3],
+                                         const double vol_offset[3], const uint64_t det_shape[2],
+                                         const double *vecs, uint64_t nangles,
+                                         cmi_plan3d **plan);
+
 CMI_API cmi_status cmi_plan3d_destroy(cmi_plan3d *plan);
 
 /* Shape queries. total_samples = sum over rays that hit the clip box of the sample count

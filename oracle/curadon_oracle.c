@@ -478,7 +478,31 @@ void orc_rotate_rpy(const double v[3], double roll, double pitch, double yaw, do
 
 /* ------------------------------------------------------------- 3D forward (A.4) ------- */
 
+/* Views (world points and steps) into the voxel frame: w -> (w - off + n sp / 2 - sp / 2) / sp
+ * (forward_3d.hpp:158-169), steps divided by the spacing. */
+static void plan3d_views(const orc_cone3d *g, double *origin, double *du, double *dv,
+                         double *src) {
+    const double sp[3] = {g->sx, g->sy, g->sz}, off[3] = {g->ox, g->oy, g->oz};
+    const double n[3] = {(double)g->nx, (double)g->ny, (double)g->nz};
+    const double hu = ((double)g->nu - 1.0) / 2.0, hv = ((double)g->nv - 1.0) / 2.0;
+    for (int64_t a = 0; a < g->nangles; ++a) {
+        const double *v = g->vecs + 12 * a;
+        for (int d = 0; d < 3; ++d) {
+            const double tr = n[d] * sp[d] / 2.0 - sp[d] / 2.0;
+            const double p0 = v[3 + d] - hu * v[6 + d] - hv * v[9 + d]; /* pixel (0, 0) */
+            origin[3 * a + d] = (p0 - off[d] + tr) / sp[d];
+            du[3 * a + d] = v[6 + d] / sp[d];
+            dv[3 * a + d] = v[9 + d] / sp[d];
+            src[3 * a + d] = (v[d] - off[d] + tr) / sp[d];
+        }
+    }
+}
+
 void orc_plan3d_fwd(const orc_cone3d *g, double *origin, double *du, double *dv, double *src) {
+    if (g->vecs) {
+        plan3d_views(g, origin, du, dv, src);
+        return;
+    }
     const double dod = g->dsd - g->dso;
     const double sp[3] = {g->sx, g->sy, g->sz}, off[3] = {g->ox, g->oy, g->oz};
     const double n[3] = {(double)g->nx, (double)g->ny, (double)g->nz};
@@ -658,7 +682,14 @@ static inline double panel_bilerp(const double *p, int64_t nu, int64_t nv, doubl
            b * ((1.0 - a) * fetch_panel(p, nu, nv, i, j + 1) + a * fetch_panel(p, nu, nv, i + 1, j + 1));
 }
 
+static void bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int64_t z0,
+                         int64_t z1, int fdk);
+
 void orc_bp3d(const orc_cone3d *g, const double *sino, double *vol, int64_t z0, int64_t z1) {
+    if (g->vecs) { /* views: the reference's frame exists for Euler angles only */
+        bp3d_matched(g, sino, vol, z0, z1, 0);
+        return;
+    }
     const size_t n3 = (size_t)g->nangles * 3;
     double *O = malloc(n3 * sizeof(double)), *dx = malloc(n3 * sizeof(double));
     double *dy = malloc(n3 * sizeof(double)), *dz = malloc(n3 * sizeof(double));

@@ -108,6 +108,11 @@ typedef struct {
     double dso, dsd, cor;
     int64_t nangles;
     const double *phi, *theta, *psi; /* [A] each */
+    /* ASTRA-like views [A][12] = (source, panel centre, u-step, v-step) in world (x, y, z), or
+     * NULL. When set, they replace the Euler-angle set-up (phi / theta / psi, DSO, DSD, COR,
+     * detector offset and rotation are then unused), and the reference-mode backprojection
+     * projects like the matched one. */
+    const double *vecs;
 } orc_cone3d;
 
 /* Rotations of rotation.hpp:7-48, exported for the orthogonality / special-case tests. */
@@ -115,7 +120,9 @@ void orc_rotate_yzy(const double v[3], double phi, double theta, double psi, dou
 void orc_rotate_rpy(const double v[3], double roll, double pitch, double yaw, double out[3]);
 
 /* Forward set-up (forward_3d.hpp:97-211): per angle the detector pixel (0,0) position, the
- * u- and v-steps and the source, in voxel coordinates (voxel i centred at i). [A][3] each. */
+ * u- and v-steps and the source, in voxel coordinates (voxel i centred at i). [A][3] each.
+ * With views: pixel (0,0) at centre - (U-1)/2 u - (V-1)/2 v, every point taken into the voxel
+ * frame as the reference's steps 4-5 do (forward_3d.hpp:158-169). */
 void orc_plan3d_fwd(const orc_cone3d *g, double *origin, double *du, double *dv, double *src);
 
 /* Ray (a, iu, iv) of kernel_forward_3d (forward_3d.hpp:57-90): returns the number of

@@ -40,6 +40,7 @@ class Cone3D(C.Structure):
         ("off_u", C.c_double), ("off_v", C.c_double), ("pitch", C.c_double),
         ("roll", C.c_double), ("yaw", C.c_double), ("dso", C.c_double), ("dsd", C.c_double),
         ("cor", C.c_double), ("nangles", _i64), ("phi", _dp), ("theta", _dp), ("psi", _dp),
+        ("vecs", _dp),
     ]
 
 
@@ -244,19 +245,34 @@ class Cone:
     """Holds an ``orc_cone3d``. ``geom`` has ``ConeGeometry``'s attributes in Python axis
     order: ``vol_shape`` (z, y, x), ``det_shape`` (v, u), ``angles`` (3, A) as (phi, theta,
     psi), ``det_rotation`` (pitch, roll, yaw), scalar ``DSO``/``DSD``/``COR``. The axis
-    reversal and the angle/rotation slots follow python/standalone/src/forward.cu:113-138."""
+    reversal and the angle/rotation slots follow python/standalone/src/forward.cu:47-72.
+    A ``vecs`` attribute (A, 12) that is not None (ASTRA-like views in world (x, y, z):
+    source, panel centre, u-step, v-step) replaces the Euler-angle set-up; the other
+    geometry fields are then unused apart from the shapes, spacings and offsets."""
 
     def __init__(self, geom):
         vs = np.asarray(geom.vol_shape)
         sp = np.asarray(geom.vol_spacing, dtype=np.float64)
         of = np.asarray(geom.vol_offset, dtype=np.float64)
         ds = np.asarray(geom.det_shape)
-        dsp = np.asarray(geom.det_spacing, dtype=np.float64)
-        dof = np.asarray(geom.det_offset, dtype=np.float64)
-        rot = np.asarray(geom.det_rotation, dtype=np.float64)
-        ang = np.asarray(geom.angles, dtype=np.float64)
-        if ang.ndim == 1:
-            ang = np.vstack([ang, np.zeros_like(ang), np.zeros_like(ang)])
+        vecs = getattr(geom, "vecs", None)
+        self.vecs = None if vecs is None else _c64(np.asarray(vecs, dtype=np.float64).reshape(-1, 12))
+        if self.vecs is not None:
+            A = self.vecs.shape[0]
+            dsp = np.ones(2)
+            dof = np.zeros(2)
+            rot = np.zeros(3)
+            ang = np.zeros((3, A))
+            dso = dsd = cor = 0.0
+        else:
+            dsp = np.asarray(geom.det_spacing, dtype=np.float64)
+            dof = np.asarray(geom.det_offset, dtype=np.float64)
+            rot = np.asarray(geom.det_rotation, dtype=np.float64)
+            ang = np.asarray(geom.angles, dtype=np.float64)
+            if ang.ndim == 1:
+                ang = np.vstack([ang, np.zeros_like(ang), np.zeros_like(ang)])
+            dso, dsd = float(geom.DSO), float(geom.DSD)
+            cor = float(getattr(geom, "COR", 0.0))
         self.phi, self.theta, self.psi = _c64(ang[0]), _c64(ang[1]), _c64(ang[2])
         A = self.phi.size
         self.s = Cone3D(
@@ -266,8 +282,9 @@ class Cone:
             nu=int(ds[1]), nv=int(ds[0]), du=float(dsp[1]), dv=float(dsp[0]),
             off_u=float(dof[1]), off_v=float(dof[0]),
             pitch=float(rot[0]), roll=float(rot[1]), yaw=float(rot[2]),
-            dso=float(geom.DSO), dsd=float(geom.DSD), cor=float(getattr(geom, "COR", 0.0)),
+            dso=dso, dsd=dsd, cor=cor,
             nangles=A, phi=_d(self.phi), theta=_d(self.theta), psi=_d(self.psi),
+            vecs=_d(self.vecs) if self.vecs is not None else None,
         )
         self.shape_vol = (int(vs[0]), int(vs[1]), int(vs[2]))
         self.shape_sino = (A, int(ds[0]), int(ds[1]))
@@ -320,6 +337,8 @@ def bp3d(geom, sino, slabs=None, vol=None, matched=False, fdk=False):
     """FP64 3D backprojection: the reference's (default), geometry-matched, or matched with
     the FDK distance weight."""
     c = Cone(geom)
+    if fdk and c.vecs is not None:
+        raise ValueError("FDK weighting needs a circular orbit, not views")
     sino = _c64(sino)
     assert sino.shape == c.shape_sino
     out = np.zeros(c.shape_vol) if vol is None else _c64(vol).copy()

@@ -76,3 +76,29 @@ def as_cone_geometry(rec, **kw):
                            det_offset=rec.det_offset, det_rotation=rec.det_rotation,
                            vol_spacing=rec.vol_spacing, vol_offset=rec.vol_offset, COR=rec.COR,
                            **kw)
+
+
+def with_views(g, vecs):
+    """A copy of record `g` with ASTRA-like views (the oracle then uses them)."""
+    r = SimpleNamespace(**vars(g))
+    r.vecs = np.asarray(vecs, dtype=np.float64)
+    return r
+
+
+def helix_views(n_views, n, radius, pitch_per_turn, dod, U, V, du=1.0, dv=1.0, tilt=0.0):
+    """Helical views: the source on a helix about the y axis, the panel facing it at distance
+    radius + dod, tilted by `tilt` about its own normal; an explicit, non-circular trajectory."""
+    out = []
+    for k in range(n_views):
+        t = 2 * np.pi * k / n_views * 1.3
+        h = pitch_per_turn * t / (2 * np.pi) - 0.3 * n
+        s = np.array([-radius * np.sin(t), h, -radius * np.cos(t)])
+        d = np.array([np.sin(t), 0.0, np.cos(t)])      # from the source towards the axis
+        c = s + (radius + dod) * d
+        e_u = np.array([np.cos(t), 0.0, -np.sin(t)])
+        e_v = np.array([0.0, 1.0, 0.0])
+        ct, st = np.cos(tilt), np.sin(tilt)
+        u = du * (ct * e_u + st * e_v)
+        v = dv * (-st * e_u + ct * e_v)
+        out.append(np.concatenate([s, c, u, v]))
+    return np.array(out)

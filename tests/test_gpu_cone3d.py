@@ -212,6 +212,52 @@ def test_c3_backward_subset(oracle, big, gpu):
         check_close(np.stack(got), np.stack(ref), f"c3 bp3d {mode} (3 slices)")
 
 
+@pytest.mark.parametrize("a0,a1", [(0, 2), (89, 91), (200, 202)])
+def test_c3_adjoint_blocks(oracle, big, gpu, a0, a1):
+    """The matched adjoint at the 256^3 workload against the oracle's transposed march, on a
+    sinogram that is zero outside two projections (the adjoint of that block)."""
+    rec, geom, _ = big
+    y = np.zeros((360, 384, 384), dtype=np.float32)
+    y[a0:a1] = np.random.default_rng(30 + a0).random((a1 - a0, 384, 384))
+    got = host(cm.adjoint(dev(y, gpu), geom))
+    ref = oracle.adj3d(rec, y.astype(np.float64), angles=(a0, a1))
+    check_close(got, ref, f"c3 adj3d (projections {a0}..{a1 - 1})")
+
+
+@pytest.mark.parametrize("prec,tol", [(torch.float16, 1e-3), (torch.bfloat16, 6e-3)])
+def test_c3_half_storage(oracle, big, gpu, prec, tol):
+    """16-bit storage at the 256^3 workload: forward on four projections and the reference
+    backprojection on three slices against the FP64 oracle of the unrounded inputs and of the
+    stored ones."""
+    rec, _, vol = big
+    geom = as_cone_geometry(rec, vol_prec=prec, sino_prec=prec)
+    x = dev(vol, gpu, prec)
+    out = host(cm.forward(x, geom))
+    xs = host(x)
+    got, ref, refr = [], [], []
+    for a0 in (0, 45, 90, 200):
+        got.append(out[a0])
+        ref.append(oracle.fwd3d(rec, vol.astype(np.float64), angles=(a0, a0 + 1))[a0])
+        refr.append(oracle.fwd3d(rec, xs, angles=(a0, a0 + 1))[a0])
+    r, rr = rel_l2(np.stack(got), np.stack(ref)), rel_l2(np.stack(got), np.stack(refr))
+    print(f"c3 fwd3d {prec} (4 projections): rel-L2 {r:.3e} vs exact input, {rr:.3e} vs "
+          f"stored input", flush=True)
+    del out, x
+    y = np.random.default_rng(31).random((360, 384, 384)).astype(np.float32)
+    y16 = dev(y, gpu, prec)
+    bv = host(cm.backward(y16, geom))
+    ys = host(y16)
+    gb, rb, rbr = [], [], []
+    for z0 in (0, 128, 255):
+        gb.append(bv[z0])
+        rb.append(oracle.bp3d(rec, y.astype(np.float64), slabs=(z0, z0 + 1))[z0])
+        rbr.append(oracle.bp3d(rec, ys, slabs=(z0, z0 + 1))[z0])
+    b, br = rel_l2(np.stack(gb), np.stack(rb)), rel_l2(np.stack(gb), np.stack(rbr))
+    print(f"c3 bp3d {prec} (3 slices): rel-L2 {b:.3e} vs exact input, {br:.3e} vs stored "
+          f"input", flush=True)
+    assert max(r, b) <= tol and max(rr, br) <= tol
+
+
 def test_c3_dot_product(big, gpu):
     rec, geom, _ = big
     g = torch.Generator(device=gpu).manual_seed(11)
@@ -257,3 +303,22 @@ def test_circular_orbit_high_magnification(oracle, gpu, dso, maxabs):
         ref = oracle.bp3d(rec, y, matched=mode == "matched", fdk=mode == "fdk")
         check_close(host(cm.backward(dev(y, gpu), geom, mode=mode)), ref,
                     f"high-magnification (DSO {dso}) bp3d {mode}", maxabs=maxabs)
+
+
+@pytest.mark.parametrize("circular", [True, False])
+def test_odd_panel_every_mode(oracle, gpu, circular):
+    """A panel of 7 x 5 pixels and 3 projections: the staged panel stack holds
+    (7 + 4) (5 + 4) 3 = 297 elements, not a multiple of four, so the staging kernel's last
+    thread writes its tail element by element. Circular orbits take the y-run backprojector,
+    tilted ones the column backprojector; every mode against the oracle."""
+    if circular:
+        rec = cone(9, 7, 5, 3, 30.0, 50.0, nz=7, ny=8)
+    else:
+        ang = np.array([[0.2, 1.9, 4.0], [0.1, -0.2, 0.15], [0.0, 0.1, -0.1]])
+        rec = cone(9, 7, 5, 3, 30.0, 50.0, angles=ang, nz=7, ny=8)
+    geom = as_cone_geometry(rec)
+    y = np.random.default_rng(40).random((3, 5, 7))
+    for mode in ("reference", "matched", "fdk"):
+        ref = oracle.bp3d(rec, y, matched=mode == "matched", fdk=mode == "fdk")
+        check_close(host(cm.backward(dev(y, gpu), geom, mode=mode)), ref,
+                    f"odd panel ({'circular' if circular else 'tilted'}) bp3d {mode}")

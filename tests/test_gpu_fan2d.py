@@ -139,14 +139,20 @@ def test_c2_backward_subsets(oracle, g2, gpu):
     check_close(np.concatenate(got), np.concatenate(ref), "c2 backward (16 rows)")
 
 
-def test_c2_adjoint_angle_subset(oracle, g2, gpu):
+# Blocks of six angles where the fan's rays change dominant axis (the fan's half angle is
+# 13.3 degrees, so each block holds rays of both kinds near 45, 135, 225 and 315 degrees) and
+# where every ray has the same dominant axis (0, 90 degrees).
+ADJ_BLOCKS = [(0, 6), (87, 93), (177, 183), (264, 270), (447, 453), (627, 633)]
+
+
+@pytest.mark.parametrize("a0,a1", ADJ_BLOCKS)
+def test_c2_adjoint_angle_blocks(oracle, g2, gpu, a0, a1):
     rec, geom, _ = g2
     y = np.zeros((720, 1449))
-    a0, a1 = 87, 93
-    y[a0:a1] = np.random.default_rng(5).random((a1 - a0, 1449))
+    y[a0:a1] = np.random.default_rng(5 + a0).random((a1 - a0, 1449))
     ref = oracle.adj2d(rec, y, angles=(a0, a1))
     got = host(cm.adjoint(dev(y, gpu), geom))
-    check_close(got, ref, "c2 adjoint (6 angles)")
+    check_close(got, ref, f"c2 adjoint (angles {a0}..{a1 - 1})")
 
 
 def test_c2_dot_product(g2, gpu):
@@ -227,6 +233,41 @@ def test_half_storage_c1(oracle, gpu, prec, tol):
     rb = rel_l2(outb, refb)
     print(f"c1 backward {prec}: rel-L2 {rb:.3e}")
     assert rb <= tol
+
+
+# 16-bit storage at the 1024^2 workload: the error against the FP64 oracle of the unrounded
+# inputs (what a caller gets) and against the oracle of the rounded inputs (the operator's own
+# part: FP32 arithmetic and the rounding of its output).
+@pytest.mark.parametrize("prec,tol", [(torch.float16, 1e-3), (torch.bfloat16, 6e-3)])
+def test_half_storage_c2(oracle, g2, gpu, prec, tol):
+    rec, _, img = g2
+    geom = as_fan_geometry(rec, vol_prec=prec, sino_prec=prec)
+    x = dev(img, gpu, prec)
+    xr = host(x)  # the image as stored
+    out = host(cm.forward(x, geom))
+    got, ref, refr = [], [], []
+    for a0, a1 in ANGLE_BLOCKS[:4]:
+        got.append(out[a0:a1])
+        ref.append(oracle.fwd2d(rec, img, angles=(a0, a1))[a0:a1])
+        refr.append(oracle.fwd2d(rec, xr, angles=(a0, a1))[a0:a1])
+    got, ref, refr = np.concatenate(got), np.concatenate(ref), np.concatenate(refr)
+    r, rr = rel_l2(got, ref), rel_l2(got, refr)
+    print(f"c2 forward {prec} (24 angles): rel-L2 {r:.3e} vs exact input, {rr:.3e} vs "
+          f"stored input", flush=True)
+    sino = np.random.default_rng(8).random((720, 1449))
+    s16 = dev(sino, gpu, prec)
+    vol = host(cm.backward(s16, geom))
+    gb, rb, rbr = [], [], []
+    for y0, y1 in ROW_BLOCKS:
+        gb.append(vol[y0:y1])
+        rb.append(oracle.bp2d(rec, sino, rows=(y0, y1))[y0:y1])
+        rbr.append(oracle.bp2d(rec, host(s16), rows=(y0, y1))[y0:y1])
+    gb, rb, rbr = np.concatenate(gb), np.concatenate(rb), np.concatenate(rbr)
+    b, br = rel_l2(gb, rb), rel_l2(gb, rbr)
+    print(f"c2 backward {prec} (16 rows): rel-L2 {b:.3e} vs exact input, {br:.3e} vs "
+          f"stored input", flush=True)
+    assert max(r, b) <= tol
+    assert max(rr, br) <= tol
 
 
 def test_errors_like_reference(g1, gpu):

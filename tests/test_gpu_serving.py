@@ -230,3 +230,35 @@ def test_every_job_class_of_the_mix_against_fp64(oracle, gpu):
         print(f"{j.tag}: FP16 storage rel-L2 vs FP64 {r:.3e}", flush=True)
         worst = max(worst, r)
     assert worst <= 1e-3
+
+
+@pytest.mark.parametrize("storage", [torch.float32, torch.float16, torch.bfloat16])
+@pytest.mark.parametrize("offset", [0, 1, 3])
+def test_stage_projections_tail_and_unaligned_destination(gpu, storage, offset):
+    """Staging into a destination that starts `offset` elements into its buffer (no vector
+    stores there) and whose element count, A (D + 4) = 3 * 9 = 27 (fan) and
+    A (V + 4) (U + 4) = 3 * 7 * 9 = 189 (cone), is no multiple of four (the last thread's tail):
+    every element, borders included, equals the host-side expectation."""
+    from curadon_amd import _lib
+    g = torch.Generator().manual_seed(50 + offset)
+    for shape in ((3, 5), (3, 3, 5)):
+        src = torch.rand(shape, generator=g).to(gpu)
+        rs = torch.rand(shape[0], generator=g).to(gpu)
+        staged_shape = (3, 9) if len(shape) == 2 else (3, 7, 9)
+        n = int(np.prod(staged_shape))
+        buf = torch.full((n + offset + 4,), 7.0, dtype=storage, device=gpu)
+        dst = buf[offset:offset + n].view(staged_shape)
+        keep = _lib.stage_projections(src, dst, row_scale=rs, scale=0.5)
+        torch.cuda.synchronize()
+        del keep
+        # FP32 products as the kernel forms them, then one rounding to the storage type
+        exp = torch.zeros(staged_shape, dtype=torch.float32)
+        val = (src.cpu() * 0.5) * rs.cpu().reshape((-1,) + (1,) * (len(shape) - 1))
+        if len(shape) == 2:
+            exp[:, 2:7] = val
+        else:
+            exp[:, 2:5, 2:7] = val
+        exp = exp.to(storage)
+        assert torch.equal(dst.cpu(), exp), (shape, offset)
+        # nothing written before or after the destination
+        assert torch.all(buf[:offset].cpu() == 7.0) and torch.all(buf[offset + n:].cpu() == 7.0)
