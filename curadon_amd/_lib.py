@@ -43,6 +43,7 @@ ERR_NOT_SUPPORTED = 6
 F16, F32, BF16 = 16, 32, 1016
 BP3D_REFERENCE, BP3D_MATCHED, BP3D_FDK = 0, 1, 2
 BP2D_REFERENCE, BP2D_FBP = 0, 1
+PLAN2D_EXACT_GEOMETRY = 1
 _BP3D_MODES = {"reference": BP3D_REFERENCE, "matched": BP3D_MATCHED, "fdk": BP3D_FDK}
 _BP2D_MODES = {"reference": BP2D_REFERENCE, "fbp": BP2D_FBP}
 
@@ -116,6 +117,9 @@ SIGNATURES = {
                                  _dp, _dp, _dp, _u64, C.c_double, C.c_double, C.POINTER(_vp)]),
     "cmi_plan2d_create_vec": (_i32, [_i32, _i32, _u64p, _dp, _dp, _i32, _u64, _dp, _u64,
                                      C.POINTER(_vp)]),
+    "cmi_plan2d_create_ex": (_i32, [_i32, _i32, _u64p, _dp, _dp, _i32, _u64, C.c_double,
+                                    C.c_double, _dp, _dp, _dp, _u64, C.c_double, C.c_double,
+                                    C.c_uint32, C.POINTER(_vp)]),
     "cmi_plan2d_destroy": (_i32, [_vp]),
     "cmi_plan2d_set_block": (_i32, [_vp, C.c_uint32, C.c_uint32]),
     "cmi_plan2d_get_block": (_i32, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -242,7 +246,13 @@ class plan_2d:
     ``device``; the plan is immutable afterwards apart from the launch-shape tunables."""
 
     def __init__(self, device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec,
-                 det_count, det_spacing, det_offset, DSO, DSD, angles, det_pitch=0.0, COR=0.0):
+                 det_count, det_spacing, det_offset, DSO, DSD, angles, det_pitch=0.0, COR=0.0,
+                 exact_geometry=False):
+        """``exact_geometry`` (not in the reference): det_offset shifts the detector along its
+        axis, det_pitch turns it about its centre and COR moves the rotation centre, in the
+        forward, backprojection and adjoint alike (cmi_plan2d_create_ex); without it, the
+        reference's semantics (the forward moves the source by det_offset, the backprojection
+        ignores it, det_pitch and COR are unused)."""
         self._h = None
         angles = _f64(angles)
         A = angles.size
@@ -259,12 +269,14 @@ class plan_2d:
         self.vol_shape = (int(vs[0]), int(vs[1]))
         self.det_count = int(det_count)
         self.nangles = A
+        self.exact_geometry = bool(exact_geometry)
         h = _vp()
-        check(_lib.cmi_plan2d_create(
+        check(_lib.cmi_plan2d_create_ex(
             self.device, self.vol_prec, vs.ctypes.data_as(_u64p), sp.ctypes.data_as(_dp),
             of.ctypes.data_as(_dp), self.det_prec, self.det_count, float(det_spacing),
             float(det_offset), DSO.ctypes.data_as(_dp), DSD.ctypes.data_as(_dp),
-            angles.ctypes.data_as(_dp), A, float(det_pitch), float(COR), C.byref(h)))
+            angles.ctypes.data_as(_dp), A, float(det_pitch), float(COR),
+            PLAN2D_EXACT_GEOMETRY if exact_geometry else 0, C.byref(h)))
         self._h = h
 
     @classmethod

@@ -928,6 +928,44 @@ cmi_status finish_plan2d(std::unique_ptr<cmi_plan2d> p, const std::vector<AngleG
     return CMI_OK;
 }
 
+// Plans from ASTRA-like views (cmi_plan2d_create_vec, and cmi_plan2d_create_ex with exact
+// geometry, which builds its views first).
+cmi_status plan2d_from_views(int device, int vol_prec, const uint64_t vol_shape[2],
+                             const double vol_spacing[2], const double vol_offset[2],
+                             int det_prec, uint64_t det_count, const double *vecs,
+                             uint64_t nangles, const char *fn, cmi_plan2d **plan) {
+    CMI_TRY(check_plan_shape(vol_prec, det_prec, vol_shape, vol_spacing, det_count, nangles, fn));
+    for (uint64_t a = 0; a < nangles; ++a) {
+        const double *v = vecs + 6 * a;
+        for (int i = 0; i < 6; ++i)
+            if (!std::isfinite(v[i]))
+                return fail(CMI_ERR_INVALID_ARGUMENT, "%s: non-finite view %llu", fn,
+                            (unsigned long long)a);
+        const fan2d::BpView bv = fan2d::bp_view(v, (int64_t)det_count);
+        if (!(v[4] * v[4] + v[5] * v[5] > 0.0) || !(bv.L > 0.0))
+            return fail(CMI_ERR_INVALID_ARGUMENT,
+                        "%s: view %llu has a zero cell step or its source on the detector line",
+                        fn, (unsigned long long)a);
+    }
+    auto p = std::make_unique<cmi_plan2d>();
+    set_plan_shape(*p, device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec, det_count,
+                   nangles);
+    p->g.du = std::sqrt(vecs[4] * vecs[4] + vecs[5] * vecs[5]); // view 0's, for get_info
+    const int64_t A = p->A;
+    std::vector<AngleGeom> geo(A);
+    std::vector<float4> bpv(3 * A);
+    for (int64_t a = 0; a < A; ++a) {
+        geo[a] = fan2d::plan_angle_vec(p->g, vecs + 6 * a);
+        const fan2d::BpView bv = fan2d::bp_view(vecs + 6 * a, p->g.D);
+        bpv[3 * a] = make_float4((float)bv.n[0], (float)bv.n[1], (float)bv.n0, (float)bv.L);
+        // the cell offset carries the lerp's texel-centre shift (-0.5)
+        bpv[3 * a + 1] =
+            make_float4((float)bv.e[0], (float)bv.e[1], (float)bv.e0, (float)(bv.c0 - 0.5));
+        bpv[3 * a + 2] = make_float4((float)bv.inv_du, 0.0f, 0.0f, 0.0f);
+    }
+    return finish_plan2d(std::move(p), geo, {}, bpv, plan);
+}
+
 } // namespace
 
 // ------------------------------------------------------ forward, shared by the entry points --
@@ -1026,37 +1064,50 @@ CMI_API cmi_status cmi_plan2d_create_vec(int device, int vol_prec, const uint64_
     *plan = nullptr;
     if (!vol_shape || !vol_spacing || !vol_offset || !vecs)
         return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create_vec: null geometry array");
+    return plan2d_from_views(device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec,
+                             det_count, vecs, nangles, "plan2d_create_vec", plan);
+}
+
+CMI_API cmi_status cmi_plan2d_create_ex(int device, int vol_prec, const uint64_t vol_shape[2],
+                                        const double vol_spacing[2], const double vol_offset[2],
+                                        int det_prec, uint64_t det_count, double det_spacing,
+                                        double det_offset, const double *DSO, const double *DSD,
+                                        const double *angles, uint64_t nangles,
+                                        double det_pitch, double COR, uint32_t flags,
+                                        cmi_plan2d **plan) {
+    if ((flags & ~(uint32_t)CMI_PLAN2D_EXACT_GEOMETRY) != 0) {
+        clear_error();
+        if (plan) *plan = nullptr;
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create_ex: unknown flags 0x%x", flags);
+    }
+    if (!(flags & CMI_PLAN2D_EXACT_GEOMETRY))
+        return cmi_plan2d_create(device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec,
+                                 det_count, det_spacing, det_offset, DSO, DSD, angles, nangles,
+                                 det_pitch, COR, plan);
+    clear_error();
+    if (!plan) return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create_ex: output pointer is null");
+    *plan = nullptr;
+    if (!vol_shape || !vol_spacing || !vol_offset || !DSO || !DSD || !angles)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create_ex: null geometry array");
     CMI_TRY(check_plan_shape(vol_prec, det_prec, vol_shape, vol_spacing, det_count, nangles,
-                             "plan2d_create_vec"));
+                             "plan2d_create_ex"));
+    if (!(det_spacing > 0))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create_ex: spacings must be positive");
+    if (!std::isfinite(det_offset) || !std::isfinite(det_pitch) || !std::isfinite(COR))
+        return fail(CMI_ERR_INVALID_ARGUMENT, "plan2d_create_ex: non-finite detector geometry");
+    fan2d::Scalars g{};
+    g.du = det_spacing;
+    g.det_offset = det_offset;
+    std::vector<double> vecs(6 * nangles);
     for (uint64_t a = 0; a < nangles; ++a) {
-        const double *v = vecs + 6 * a;
-        for (int i = 0; i < 6; ++i)
-            if (!std::isfinite(v[i]))
-                return fail(CMI_ERR_INVALID_ARGUMENT,
-                            "plan2d_create_vec: non-finite view %llu", (unsigned long long)a);
-        const fan2d::BpView bv = fan2d::bp_view(v, (int64_t)det_count);
-        if (!(v[4] * v[4] + v[5] * v[5] > 0.0) || !(bv.L > 0.0))
+        if (!std::isfinite(angles[a]) || !std::isfinite(DSO[a]) || !std::isfinite(DSD[a]))
             return fail(CMI_ERR_INVALID_ARGUMENT,
-                        "plan2d_create_vec: view %llu has a zero cell step or its source on "
-                        "the detector line", (unsigned long long)a);
+                        "plan2d_create_ex: non-finite geometry at angle %llu",
+                        (unsigned long long)a);
+        fan2d::exact_view(g, angles[a], DSO[a], DSD[a], det_pitch, COR, vecs.data() + 6 * a);
     }
-    auto p = std::make_unique<cmi_plan2d>();
-    set_plan_shape(*p, device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec, det_count,
-                   nangles);
-    p->g.du = std::sqrt(vecs[4] * vecs[4] + vecs[5] * vecs[5]); // view 0's, for get_info
-    const int64_t A = p->A;
-    std::vector<AngleGeom> geo(A);
-    std::vector<float4> bpv(3 * A);
-    for (int64_t a = 0; a < A; ++a) {
-        geo[a] = fan2d::plan_angle_vec(p->g, vecs + 6 * a);
-        const fan2d::BpView bv = fan2d::bp_view(vecs + 6 * a, p->g.D);
-        bpv[3 * a] = make_float4((float)bv.n[0], (float)bv.n[1], (float)bv.n0, (float)bv.L);
-        // the cell offset carries the lerp's texel-centre shift (-0.5)
-        bpv[3 * a + 1] =
-            make_float4((float)bv.e[0], (float)bv.e[1], (float)bv.e0, (float)(bv.c0 - 0.5));
-        bpv[3 * a + 2] = make_float4((float)bv.inv_du, 0.0f, 0.0f, 0.0f);
-    }
-    return finish_plan2d(std::move(p), geo, {}, bpv, plan);
+    return plan2d_from_views(device, vol_prec, vol_shape, vol_spacing, vol_offset, det_prec,
+                             det_count, vecs.data(), nangles, "plan2d_create_ex", plan);
 }
 
 CMI_API cmi_status cmi_fan2d_bp_table(uint64_t nangles, const double *angles, const double *DSO,

@@ -85,6 +85,25 @@ inline AngleGeom plan_angle_vec(const Scalars &g, const double v[6]) {
     return r;
 }
 
+// The view of a circular plan with exact geometry (cmi_plan2d_create_ex, CMI_PLAN2D_EXACT_GEOMETRY;
+// no counterpart in the reference, which moves the source by det_offset in its forward only and
+// stores det_pitch and COR unused, plan_2d.hpp:76-80, 121-122): before the rotation by the
+// angle, the source sits at (-COR, DSO), the detector centre at (det_offset - COR,
+// -(DSD - DSO)) and the cell step is du (cos pitch, sin pitch); all three are then rotated
+// clockwise by the angle like the reference's points (rotation.hpp:51-61). So det_offset moves
+// the detector along its own axis, pitch turns it about its centre, and the rotation centre lies
+// COR along the detector axis from the line through the source and the detector centre.
+inline void exact_view(const Scalars &g, double angle, double dso, double dsd, double pitch,
+                       double cor, double v[6]) {
+    const double c = std::cos(angle), s = std::sin(angle), zero[2] = {0.0, 0.0};
+    const double s0[2] = {-cor, dso};
+    const double d0[2] = {g.det_offset - cor, -(dsd - dso)};
+    const double u0[2] = {g.du * std::cos(pitch), g.du * std::sin(pitch)};
+    rot_cw(s0, c, s, zero, v);
+    rot_cw(d0, c, s, zero, v + 2);
+    rot_cw(u0, c, s, zero, v + 4);
+}
+
 // Backprojection constants of a view (the reference's formula, backward_2d.hpp:58-99, for an
 // arbitrary view): with n the unit normal of the detector line pointing from the source towards
 // it and e = u / |u|, a pixel at world x has depth n . x + n0 (n0 = -n . s), magnification

@@ -92,6 +92,17 @@ class FanGeometry:
         if getattr(self, "vecs", None) is not None:
             return self.vecs.tolist()
         out = []
+        if getattr(self, "exact_geometry", False):
+            # the views of cmi_plan2d_create_ex with exact geometry
+            p = self.det_rotation
+            for alpha, dsd, dso in zip(self.angles, self.DSD, self.DSO):
+                c, s = np.cos(alpha), np.sin(alpha)
+                R = np.array([[c, s], [-s, c]])
+                src = R @ np.array([-self.COR, dso])
+                det = R @ np.array([self.det_offset - self.COR, -(dsd - dso)])
+                u = R @ (self.det_spacing * np.array([np.cos(p), np.sin(p)]))
+                out.append([*src.tolist(), *det.tolist(), *u.tolist()])
+            return out
         for alpha, dsd, dso in zip(self.angles, self.DSD, self.DSO):
             c, s = np.cos(alpha), np.sin(alpha)
             R = np.array([[c, s], [-s, c]])
@@ -136,7 +147,16 @@ class FanGeometry:
                  det_rotation: Optional[float] = None,
                  vol_spacing: Optional[Tuple[float, float]] = None,
                  vol_offset: Optional[Tuple[float, float]] = None, device: int = 0,
-                 vol_prec=32, sino_prec=32):
+                 vol_prec=32, sino_prec=32, *, COR: Optional[float] = None,
+                 exact_geometry: bool = False):
+        """``COR`` and ``exact_geometry`` are additions (keyword only). With
+        ``exact_geometry=True`` the detector geometry is honoured consistently by the forward,
+        backward and adjoint: det_offset shifts the detector along its axis, det_rotation
+        turns it about its centre, and COR is the distance along the detector axis from the
+        line through the source and the detector centre to the centre of rotation. Without it
+        (default) the reference's semantics hold: the forward moves the source by det_offset,
+        the backprojection ignores it, and det_rotation and COR are stored and unused
+        (plan_2d.hpp:76-80, 121-122)."""
         self.angles = np.asarray(angles, dtype=np.float64)
         if self.angles.ndim != 1:
             raise AttributeError(f"Angles must be 1D array, got {str(self.angles.shape)}")
@@ -205,6 +225,13 @@ class FanGeometry:
         else:
             self.vol_offset = np.zeros(self.vol_shape.shape)
 
+        if COR is not None:
+            if is_convertible_to_float(COR) is False:
+                raise TypeError(f"Expected COR to be float, got {type(COR)}")
+            self.COR = _as_float(COR)
+        else:
+            self.COR = 0.0
+        self.exact_geometry = bool(exact_geometry)
         self.device = int(device)
         self.vol_prec = vol_prec
         self.sino_prec = sino_prec
@@ -218,15 +245,17 @@ class FanGeometry:
         return (_fp(self.angles), _fp(self.DSD), _fp(self.DSO), _fp(self.vol_shape),
                 _fp(self.vol_spacing), _fp(self.vol_offset), int(self.det_shape),
                 _as_float(self.det_spacing), _as_float(self.det_offset),
-                _as_float(self.det_rotation), int(self.device), str(self.vol_prec),
-                str(self.sino_prec), None if self.vecs is None else _fp(self.vecs))
+                _as_float(self.det_rotation), _as_float(self.COR), bool(self.exact_geometry),
+                int(self.device), str(self.vol_prec), str(self.sino_prec),
+                None if self.vecs is None else _fp(self.vecs))
 
     @property
     def plan(self):
         """The native plan (``curadon_amd._lib.plan_2d``), built on first access and rebuilt
         whenever an attribute it depends on has changed since (``fingerprint``): from the views
-        for a geometry made by ``from_vec``, else from angles / DSO / DSD with COR 0, as the
-        reference passes it (geometry.py:226-228)."""
+        for a geometry made by ``from_vec``, else from angles / DSO / DSD, det_rotation and COR
+        (the reference passes COR 0, geometry.py:226-228, and its plan ignores both), with the
+        exact-geometry flag when ``exact_geometry`` is set."""
         key = self.fingerprint()
         if self._plan is None or self._plan_key != key:
             from . import _lib
@@ -240,9 +269,19 @@ class FanGeometry:
                                           self.vol_spacing, self.vol_offset, self.sino_prec,
                                           self.det_shape, self.det_spacing, self.det_offset,
                                           self.DSO, self.DSD, self.angles, self.det_rotation,
-                                          0.0)
+                                          self.COR, exact_geometry=self.exact_geometry)
             self._plan_key = key
         return self._plan
+
+    def is_centred_circular(self):
+        """True when the projector is the reference's centred circular fan beam with the
+        detector on its central ray: no views, and (with exact geometry) no detector offset,
+        rotation or COR. FBP weighting and the serving tables assume this."""
+        if self.vecs is not None:
+            return False
+        if self.exact_geometry:
+            return self.det_offset == 0.0 and self.det_rotation == 0.0 and self.COR == 0.0
+        return True
 
     def sinogram_shape(self):
         return (self.nangles, self.det_shape)

@@ -237,10 +237,12 @@ def fbp_filter(sinogram: torch.Tensor, geom, window="ram-lak", frequency_scaling
     backprojection (`backward(..., mode="fbp")` for fan beam, mode="fdk" for cone beam)."""
     dev = sinogram.device
     dt = torch.float64 if sinogram.dtype == torch.float64 else torch.float32
-    if isinstance(geom, (FanGeometry, ConeGeometry)) and geom.vecs is not None:
+    if (isinstance(geom, FanGeometry) and not geom.is_centred_circular()) or (
+            isinstance(geom, ConeGeometry) and geom.vecs is not None):
         raise NotImplementedError(
-            "fbp: geometries built from ASTRA-like views (from_vec) have no FBP / FDK "
-            "weighting; use backward / adjoint, or a circular geometry")
+            "fbp: geometries built from ASTRA-like views (from_vec), or fan geometries with "
+            "an exact detector offset, rotation or COR, have no FBP / FDK weighting here; use "
+            "backward / adjoint, or a centred circular geometry")
     if isinstance(geom, FanGeometry):
         cw = cosine_weights_fan(geom, dev).to(dt)
         scale = (math.pi / geom.nangles) * torch.as_tensor(

@@ -239,9 +239,11 @@ class FBPServer:
         for i, j in enumerate(jobs):
             if j.id < 0:
                 j.id = i
-            if getattr(j.geom, "vecs", None) is not None:
+            if getattr(j.geom, "vecs", None) is not None or (
+                    isinstance(j.geom, FanGeometry) and not j.geom.is_centred_circular()):
                 raise NotImplementedError(
-                    f"job {j.id}: geometries from ASTRA-like views (from_vec) are not served")
+                    f"job {j.id}: geometries from ASTRA-like views (from_vec) or with an exact "
+                    f"detector offset, rotation or COR are not served")
             need = sum(footprint(j, self.storage))
             if need > self.hbm_budget:
                 raise MemoryError(f"job {j.id} needs {need} B, more than the server's budget "

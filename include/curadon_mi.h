@@ -85,7 +85,8 @@ typedef struct cmi_plan2d cmi_plan2d;
  * detector precision, det_count, det_spacing, det_offset, per-angle DSO / DSD / angles,
  * det_pitch, COR. The reference takes FP32 geometry arrays; these take FP64 (FP32 values
  * widen exactly) because the per-angle and per-ray geometry is computed in FP64.
- * det_pitch and COR are stored and unused, as in the reference (plan_2d.hpp:121-122).
+ * det_pitch and COR are stored and unused, as in the reference (plan_2d.hpp:121-122); see
+ * cmi_plan2d_create_ex for a plan that honours them.
  * Builds the per-angle geometry on the host and the per-ray table on `device`.
  */
 CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vol_shape[2],
@@ -108,6 +109,28 @@ CMI_API cmi_status cmi_plan2d_create_vec(int device, int vol_prec, const uint64_
                                          const double vol_spacing[2], const double vol_offset[2],
                                          int det_prec, uint64_t det_count, const double *vecs,
                                          uint64_t nangles, cmi_plan2d **plan);
+
+/*
+ * cmi_plan2d_create with flags. flags = 0 is cmi_plan2d_create (the reference's semantics: the
+ * forward moves the source by det_offset, the backprojection ignores det_offset, and det_pitch
+ * and COR are unused; plan_2d.hpp:76-80, 121-122, backward_2d.hpp:58-92).
+ * CMI_PLAN2D_EXACT_GEOMETRY: every projection becomes an ASTRA-like view (the plan is then a
+ * plan from views, see cmi_plan2d_create_vec) in which, before the clockwise rotation by the
+ * angle (plan_2d.hpp:17-80), the source sits at (-COR, DSO), the detector centre at
+ * (det_offset - COR, -(DSD - DSO)) and the cell step is det_spacing (cos det_pitch,
+ * sin det_pitch): det_offset shifts the detector along its own axis, det_pitch turns it about
+ * its centre, and the rotation centre lies COR along the detector axis from the line through
+ * the source and the detector centre; forward, backprojection and adjoint all use this
+ * geometry. No counterpart in the reference.
+ */
+#define CMI_PLAN2D_EXACT_GEOMETRY 1u
+CMI_API cmi_status cmi_plan2d_create_ex(int device, int vol_prec, const uint64_t vol_shape[2],
+                                        const double vol_spacing[2], const double vol_offset[2],
+                                        int det_prec, uint64_t det_count, double det_spacing,
+                                        double det_offset, const double *DSO, const double *DSD,
+                                        const double *angles, uint64_t nangles,
+                                        double det_pitch, double COR, uint32_t flags,
+                                        cmi_plan2d **plan);
 
 CMI_API cmi_status cmi_plan2d_destroy(cmi_plan2d *plan);
 

@@ -112,13 +112,40 @@ def _c64(a):
 
 # ------------------------------------------------------------------------------ 2D -----
 
+def exact_views2d(geom):
+    """Views [A][6] of a circular fan geometry with exact detector geometry: before the clockwise
+    rotation by the angle (the reference's rotate, rotation.hpp:51-61), the source at
+    (-COR, DSO), the detector centre at (det_offset - COR, -(DSD - DSO)) and the cell step
+    det_spacing (cos r, sin r), r = det_rotation. The reference itself moves the source by
+    det_offset in its forward only and leaves det_pitch / COR unused (plan_2d.hpp:76-80,
+    121-122); this is the consistent geometry that the exact-geometry plans implement."""
+    ang = np.asarray(geom.angles, dtype=np.float64)
+    A = ang.size
+    dso = np.broadcast_to(np.asarray(geom.DSO, dtype=np.float64), (A,))
+    dsd = np.broadcast_to(np.asarray(geom.DSD, dtype=np.float64), (A,))
+    cor = float(getattr(geom, "COR", 0.0))
+    r = float(getattr(geom, "det_rotation", 0.0))
+    du, off = float(geom.det_spacing), float(geom.det_offset)
+    out = np.zeros((A, 6))
+    for a in range(A):
+        c, s = np.cos(ang[a]), np.sin(ang[a])
+
+        def rot(x, y):
+            return (x * c + y * s, -x * s + y * c)
+        out[a, 0:2] = rot(-cor, dso[a])
+        out[a, 2:4] = rot(off - cor, -(dsd[a] - dso[a]))
+        out[a, 4:6] = rot(du * np.cos(r), du * np.sin(r))
+    return out
+
+
 class Fan:
     """Holds an ``orc_fan2d`` and the arrays it points into.
 
     ``geom`` is any record with the attributes of ``FanGeometry``: ``vol_shape`` (y, x),
     ``vol_spacing``/``vol_offset`` (y, x), ``det_shape``, ``det_spacing``, ``det_offset``,
     ``angles``, ``DSO``, ``DSD`` (per angle); with a ``vecs`` attribute ([A][6] ASTRA-like
-    views) that is not None, the views define the geometry instead.
+    views) that is not None, the views define the geometry instead, and with a true
+    ``exact_geometry`` attribute the views of ``exact_views2d``.
     """
 
     def __init__(self, geom):
@@ -126,6 +153,8 @@ class Fan:
         sp = np.asarray(geom.vol_spacing, dtype=np.float64)
         of = np.asarray(geom.vol_offset, dtype=np.float64)
         vecs = getattr(geom, "vecs", None)
+        if vecs is None and getattr(geom, "exact_geometry", False):
+            vecs = exact_views2d(geom)
         self.vecs = None if vecs is None else _c64(np.asarray(vecs, dtype=np.float64).reshape(-1, 6))
         self.angles = _c64(geom.angles) if self.vecs is None else _c64(np.zeros(len(self.vecs)))
         A = self.angles.size

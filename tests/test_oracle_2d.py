@@ -397,3 +397,90 @@ def test_vector_dot_product(oracle):
     lhs = np.vdot(oracle.fwd2d(v, x), y)
     rhs = np.vdot(x, oracle.adj2d(v, y))
     assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+# ---------------------------------------------------------- exact detector geometry -------
+
+def _exact(rec, cor=0.0, rot=0.0):
+    from types import SimpleNamespace
+    r = SimpleNamespace(**vars(rec))
+    r.exact_geometry, r.COR, r.det_rotation = True, float(cor), float(rot)
+    return r
+
+
+@pytest.mark.parametrize("k", [3, -2])
+def test_exact_offset_is_a_shifted_wider_detector(oracle, k):
+    """An exact detector offset of k whole cells equals the centred detector with 2|k| more
+    cells, read k + |k| cells further on: the rays are the same lines (footprint check of the
+    offset's direction and size), in the forward and, for a sinogram that is zero outside the
+    shifted window, in the backprojection."""
+    D, Dw = 21, 21 + 2 * abs(k)
+    base = fan(16, D, 10, 40.0, 64.0, det_offset=float(k), spacing=(1.0, 1.0))
+    wide = fan(16, Dw, 10, 40.0, 64.0)
+    img = np.random.default_rng(3).random((16, 16))
+    so = oracle.fwd2d(_exact(base), img)
+    sw = oracle.fwd2d(wide, img)
+    j0 = k + abs(k)
+    np.testing.assert_allclose(so, sw[:, j0:j0 + D], rtol=1e-9, atol=1e-9)
+    y = np.random.default_rng(4).random((10, D))
+    yw = np.zeros((10, Dw))
+    yw[:, j0:j0 + D] = y
+    np.testing.assert_allclose(oracle.bp2d(_exact(base), y), oracle.bp2d(wide, yw), rtol=1e-9,
+                               atol=1e-9)
+    # without the exact flag the reference's semantics hold: the forward moves the source
+    assert not np.allclose(oracle.fwd2d(base, img), so)
+
+
+def test_exact_cor_moves_the_rotation_centre(oracle):
+    """For one projection, COR = c is the same as moving the image by c along the detector
+    axis u = (cos t, -sin t) (the reference's clockwise rotation of (1, 0))."""
+    t, c = 0.7, 1.6
+    rec = fan(14, 19, 1, 30.0, 52.0, angles=[t])
+    img = np.random.default_rng(5).random((14, 14))
+    a = oracle.fwd2d(_exact(rec, cor=c), img)
+    moved = fan(14, 19, 1, 30.0, 52.0, angles=[t], offset=(-c * np.sin(t), c * np.cos(t)))
+    b = oracle.fwd2d(_exact(moved), img)
+    np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9)
+
+
+def test_exact_rotation_turns_the_detector_about_its_centre(oracle):
+    """With the detector turned by r, the ray of cell k passes through the point
+    centre + (k - (D - 1)/2) du (cos r, sin r) before the rotation by the angle (a line check on
+    the planned rays), and the centre cell's ray does not move."""
+    r_ = 0.2
+    rec = fan(12, 15, 4, 30.0, 50.0, du=1.3)
+    ex = _exact(rec, rot=r_)
+    f = oracle.Fan(ex)
+    pl = f.plan()
+    for a, t in enumerate(rec.angles):
+        c, s = np.cos(t), np.sin(t)
+        R = np.array([[c, s], [-s, c]])
+        src = R @ np.array([0.0, 30.0])
+        for k in (0, 7, 14):
+            p = R @ (np.array([0.0, -20.0]) + (k - 7) * 1.3 * np.array([np.cos(r_), np.sin(r_)]))
+            n, e, v, sc = f.ray(a, k, pl)
+            # the ray (texel frame = world + n/2 per axis at spacing 1, texel i centred at
+            # i + 0.5) is the line src -> p
+            ew = e - 12 / 2
+            d = p - src
+
+            def cross(a, b):
+                return a[0] * b[1] - a[1] * b[0]
+            assert abs(cross(d, ew - src)) < 1e-9 * np.linalg.norm(d) ** 2
+            assert abs(cross(d, v)) < 1e-12 * np.linalg.norm(d) * np.linalg.norm(v)
+    img = np.random.default_rng(6).random((12, 12))
+    np.testing.assert_allclose(oracle.fwd2d(ex, img)[:, 7], oracle.fwd2d(_exact(rec), img)[:, 7],
+                               rtol=1e-12)
+
+
+def test_exact_geometry_adjoint_is_transpose(oracle):
+    rec = _exact(fan(6, 9, 5, 20.0, 33.0, det_offset=0.7, du=1.2), cor=0.9, rot=0.1)
+    cols = []
+    for j in range(36):
+        e = np.zeros(36)
+        e[j] = 1.0
+        cols.append(oracle.fwd2d(rec, e.reshape(6, 6)).ravel())
+    M = np.stack(cols, axis=1)
+    y = np.random.default_rng(7).random((5, 9))
+    np.testing.assert_allclose(oracle.adj2d(rec, y).ravel(), M.T @ y.ravel(), rtol=1e-12,
+                               atol=1e-12)
