@@ -780,3 +780,120 @@ void orc_bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int6
 void orc_bp3d_fdk(const orc_cone3d *g, const double *sino, double *vol, int64_t z0, int64_t z1) {
     bp3d_matched(g, sino, vol, z0, z1, 1);
 }
+
+/* ------------------------------------------------ 3D FP32 ports (CPU baseline only) --- */
+
+static inline float fetch3f(const float *v, int64_t nx, int64_t ny, int64_t nz, int64_t i,
+                            int64_t j, int64_t k) {
+    return (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) ? v[(k * ny + j) * nx + i]
+                                                                       : 0.0f;
+}
+
+static inline float trilerpf(const float *v, int64_t nx, int64_t ny, int64_t nz, float x,
+                             float y, float z) {
+    const float xb = x - 0.5f, yb = y - 0.5f, zb = z - 0.5f;
+    const float fx = floorf(xb), fy = floorf(yb), fz = floorf(zb);
+    const float a = xb - fx, b = yb - fy, c = zb - fz;
+    const int64_t i = (int64_t)fx, j = (int64_t)fy, k = (int64_t)fz;
+    float acc = 0.0f;
+    for (int dk = 0; dk < 2; ++dk) {
+        const float wz = dk ? c : 1.0f - c;
+        for (int dj = 0; dj < 2; ++dj) {
+            const float wy = dj ? b : 1.0f - b;
+            const float r = (1.0f - a) * fetch3f(v, nx, ny, nz, i, j + dj, k + dk) +
+                            a * fetch3f(v, nx, ny, nz, i + 1, j + dj, k + dk);
+            acc += wz * wy * r;
+        }
+    }
+    return acc;
+}
+
+/* kernel_forward_3d (forward_3d.hpp:57-93) in FP32 over projections [a0, a1): the per-angle
+ * frame from orc_plan3d_fwd rounded to FP32, the reference's float ray set-up, slab clip and
+ * float-carried march. */
+void orc_fwd3d_f32(const orc_cone3d *g, const float *vol, float *sino, int64_t a0, int64_t a1) {
+    const size_t n3 = (size_t)g->nangles * 3;
+    double *org = malloc(n3 * sizeof(double)), *du = malloc(n3 * sizeof(double));
+    double *dv = malloc(n3 * sizeof(double)), *src = malloc(n3 * sizeof(double));
+    orc_plan3d_fwd(g, org, du, dv, src);
+    const float hi[3] = {(float)g->nx + 1.0f, (float)g->ny + 1.0f, (float)g->nz + 1.0f};
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+    for (int64_t a = a0; a < a1; ++a) {
+        for (int64_t iv = 0; iv < g->nv; ++iv) {
+            for (int64_t iu = 0; iu < g->nu; ++iu) {
+                float s[3], d[3], nrm2 = 0.0f;
+                for (int k = 0; k < 3; ++k) {
+                    const float q = (float)org[3 * a + k] + (float)iu * (float)du[3 * a + k] +
+                                    (float)iv * (float)dv[3 * a + k];
+                    s[k] = (float)src[3 * a + k];
+                    d[k] = q - s[k];
+                    nrm2 += d[k] * d[k];
+                }
+                const float nsamples = ceilf(sqrtf(nrm2));
+                float tmin = FLT_MIN, tmax = FLT_MAX;
+                for (int k = 0; k < 3; ++k) {
+                    const float t1 = (-1.0f - s[k]) / d[k], t2 = (hi[k] - s[k]) / d[k];
+                    tmin = fminf(fmaxf(t1, tmin), fmaxf(t2, tmin));
+                    tmax = fmaxf(fminf(t1, tmax), fminf(t2, tmax));
+                }
+                float acc = 0.0f;
+                if (tmin <= tmax) {
+                    const float nst = ceilf((tmax - tmin) * nsamples);
+                    const float dt = nst > 0.0f ? (tmax - tmin) / nst : 1.0f;
+                    for (float t = tmin; t <= tmax; t += dt)
+                        acc += trilerpf(vol, g->nx, g->ny, g->nz, s[0] + t * d[0] + 0.5f,
+                                        s[1] + t * d[1] + 0.5f, s[2] + t * d[2] + 0.5f);
+                }
+                sino[(a * g->nv + iv) * g->nu + iu] = acc;
+            }
+        }
+    }
+    free(org);
+    free(du);
+    free(dv);
+    free(src);
+}
+
+/* kernel_backprojection_3d (backward_3d.hpp:28-108) in FP32 over slices [z0, z1): the frame of
+ * orc_plan3d_bp rounded to FP32, projection onto z = DSO - DSD, bilinear layer sample,
+ * unweighted sum. */
+void orc_bp3d_f32(const orc_cone3d *g, const float *sino, float *vol, int64_t z0, int64_t z1) {
+    const size_t n3 = (size_t)g->nangles * 3;
+    double *O = malloc(n3 * sizeof(double)), *dx = malloc(n3 * sizeof(double));
+    double *dy = malloc(n3 * sizeof(double)), *dz = malloc(n3 * sizeof(double));
+    double sd[3];
+    orc_plan3d_bp(g, O, dx, dy, dz, sd);
+    const float s[3] = {(float)sd[0], (float)sd[1], (float)sd[2]};
+    const float plane = (float)(g->dso - g->dsd);
+    const float hu = (float)(g->nu / 2), hv = (float)(g->nv / 2);
+    const int64_t UV = g->nu * g->nv;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t iz = z0; iz < z1; ++iz)
+        for (int64_t iy = 0; iy < g->ny; ++iy)
+            for (int64_t ix = 0; ix < g->nx; ++ix) {
+                float acc = 0.0f;
+                for (int64_t a = 0; a < g->nangles; ++a) {
+                    float dir[3];
+                    for (int k = 0; k < 3; ++k)
+                        dir[k] = (float)O[3 * a + k] + (float)ix * (float)dx[3 * a + k] +
+                                 (float)iy * (float)dy[3 * a + k] +
+                                 (float)iz * (float)dz[3 * a + k] - s[k];
+                    const float t = (plane - s[2]) / dir[2];
+                    const float u = dir[0] * t + s[0] + hu - 0.5f;
+                    const float v = dir[1] * t + s[1] + hv - 0.5f;
+                    const float fu = floorf(u), fv = floorf(v);
+                    const float al = u - fu, be = v - fv;
+                    const int64_t i = (int64_t)fu, j = (int64_t)fv;
+                    const float *p = sino + a * UV;
+#define PF(ii, jj) (((ii) >= 0 && (ii) < g->nu && (jj) >= 0 && (jj) < g->nv) ? p[(jj) * g->nu + (ii)] : 0.0f)
+                    acc += (1.0f - be) * ((1.0f - al) * PF(i, j) + al * PF(i + 1, j)) +
+                           be * ((1.0f - al) * PF(i, j + 1) + al * PF(i + 1, j + 1));
+#undef PF
+                }
+                vol[(iz * g->ny + iy) * g->nx + ix] += acc;
+            }
+    free(O);
+    free(dx);
+    free(dy);
+    free(dz);
+}

@@ -65,6 +65,8 @@ def lib():
             "orc_bp2d": (None, [F2, _dp, _dp, _i64, _i64]),
             "orc_bp2d_w": (None, [F2, _dp, _dp, _i64, _i64, C.c_int]),
             "orc_bp3d_fdk": (None, [F3, _dp, _dp, _i64, _i64]),
+            "orc_fwd3d_f32": (None, [F3, _fp, _fp, _i64, _i64]),
+            "orc_bp3d_f32": (None, [F3, _fp, _fp, _i64, _i64]),
             "orc_fwd2d_f32": (None, [F2, _fp, _fp, _i64, _i64]),
             "orc_bp2d_f32": (None, [F2, _fp, _fp, _i64, _i64]),
             "orc_num_threads": (C.c_int, []),
@@ -374,4 +376,24 @@ def bp3d(geom, sino, slabs=None, vol=None, matched=False, fdk=False):
     z0, z1 = (0, c.shape_vol[0]) if slabs is None else slabs
     fn = lib().orc_bp3d_fdk if fdk else (lib().orc_bp3d_matched if matched else lib().orc_bp3d)
     fn(C.byref(c.s), _d(sino), _d(out), z0, z1)
+    return out
+
+
+def fwd3d_f32(geom, vol, angles=None):
+    """FP32 CPU port of the reference's 3D forward (timing only) over projections (a0, a1)."""
+    c = Cone(geom)
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    a0, a1 = (0, c.A) if angles is None else angles
+    sino = np.zeros(c.shape_sino, dtype=np.float32)
+    lib().orc_fwd3d_f32(C.byref(c.s), _f(vol), _f(sino), a0, a1)
+    return sino
+
+
+def bp3d_f32(geom, sino, slabs=None):
+    """FP32 CPU port of the reference's 3D backprojection (timing only) over slices (z0, z1)."""
+    c = Cone(geom)
+    sino = np.ascontiguousarray(sino, dtype=np.float32)
+    out = np.zeros(c.shape_vol, dtype=np.float32)
+    z0, z1 = (0, c.shape_vol[0]) if slabs is None else slabs
+    lib().orc_bp3d_f32(C.byref(c.s), _f(sino), _f(out), z0, z1)
     return out

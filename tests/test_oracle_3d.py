@@ -336,3 +336,17 @@ def test_views_adjoint_and_matched_backprojection(oracle):
     np.testing.assert_array_equal(oracle.bp3d(gv, y), oracle.bp3d(gv, y, matched=True))
     with pytest.raises(ValueError):
         oracle.bp3d(gv, y, fdk=True)
+
+
+def test_fp32_ports_follow_the_fp64_oracle(oracle):
+    """The FP32 CPU ports timed as the CPU baseline compute the same operators: within FP32
+    rounding (and the float-carried march's step count) of the FP64 oracle."""
+    g = cone(16, 20, 18, 6, 40.0, 64.0)
+    vol = rng(12).random((16, 16, 16))
+    ref = oracle.fwd3d(g, vol)
+    got = oracle.fwd3d_f32(g, vol)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+    y = rng(13).random((6, 18, 20))
+    ref = oracle.bp3d(g, y)
+    got = oracle.bp3d_f32(g, y)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
