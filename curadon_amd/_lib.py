@@ -125,6 +125,7 @@ SIGNATURES = {
     "cmi_plan2d_get_block": (_i32, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "cmi_plan2d_get_info": (_i32, [_vp, C.POINTER(Plan2DInfo)]),
     "cmi_forward_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_forward_2d_noalloc": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_backward_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_backward_2d_weighted": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_adjoint_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
@@ -386,6 +387,21 @@ def forward_2d(volume, sinogram, plan):
         return
     check(_lib.cmi_forward_2d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc, b,
                               stream_of(volume)))
+
+
+# Bytes of the static staging scratch of forward_2d_noalloc (CMI_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def forward_2d_noalloc(volume, sinogram, plan):
+    """forward_2d through cmi_forward_2d_noalloc: the same result without any device allocation
+    (the staging scratch is the library's static array of NOALLOC_SCRATCH_BYTES per GPU; calls
+    that need more raise). Calls that overlap on one GPU race on that array."""
+    vc, sc, b = _check_pair_2d(volume, sinogram, plan, "forward_2d_noalloc")
+    if b == 0:
+        return
+    check(_lib.cmi_forward_2d_noalloc(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(),
+                                      sc, b, stream_of(volume)))
 
 
 def backward_2d(volume, sinogram, plan, weighting="reference"):

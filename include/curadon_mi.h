@@ -165,6 +165,22 @@ CMI_API cmi_status cmi_forward_2d(const cmi_plan2d *plan, const void *volume, in
                                   void *sinogram, int sino_dtype, uint64_t batch, void *stream);
 
 /*
+ * cmi_forward_2d without any allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same arguments, validation and result as cmi_forward_2d (the same
+ * kernels, bit-identical output); the staging scratch (two zero-bordered copies per image,
+ * 2 (nx + 4) (ny + 4) bytes-per-element per image) is a static device array of the library of
+ * CMI_NOALLOC_SCRATCH_BYTES, which each GPU holds for the life of the process. A call that needs
+ * more returns CMI_ERR_INVALID_ARGUMENT (for FP32, single images up to about 2890 x 2890).
+ * All calls on one GPU share that GPU's array: calls that overlap on one GPU (several streams
+ * or host threads) race, so callers serialise them.
+ */
+#define CMI_NOALLOC_SCRATCH_BYTES (64ull << 20)
+CMI_API cmi_status cmi_forward_2d_noalloc(const cmi_plan2d *plan, const void *volume,
+                                          int vol_dtype, void *sinogram, int sino_dtype,
+                                          uint64_t batch, void *stream);
+
+/*
  * Voxel-driven backprojection with the reference's weighting, volume += B sinogram.
  * Replaces curadon_ext.backward_2d (python/standalone/src/backward.cu:89-133 ->
  * curad::bp::backproject_2d, include/curadon/detail/backward_2d.hpp:103-166). All angles
