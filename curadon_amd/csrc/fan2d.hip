@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "bp_tiles.hpp"
@@ -60,6 +61,7 @@ struct cmi_plan2d {
     float *d_ageo = nullptr;
     float4 *d_ev = nullptr;
     float4 *d_ns = nullptr;
+    bool quarter = false; // backprojection by quarter turns applies (k_bp2d_qt)
 };
 
 namespace {
@@ -579,6 +581,163 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp2d_vec(typename SV::raw *__rest
     vol[i] = from_f32<SV>(to_f32<SV>(vol[i]) + acc);
 }
 
+// Backprojection by quarter turns, for square images (n even, unit pixels, no offset) whose
+// angles come in quarter-turn quadruples (angle a + A/4 = angle a + pi/2, equal DSO and DSD):
+// the pixel at world (y, -x) has at angle theta + pi/2 the depth xi + DSO and detector
+// coordinate that the pixel (x, y) has at angle theta (with meta = (-sin, cos), xi = -sin x -
+// cos y and eta = cos x - sin y turn into each other). So one thread evaluates the geometry of
+// its pixel P = (x, y) once per angle a and adds its weighted
+// sample of row a to P, of row a + A/4 to (y, -x), of row a + A/2 to (-x, -y) and of row
+// a + 3A/4 to (-y, x): four pixels per geometry evaluation. In index terms, with h = n/2 and
+// P = (ix, iy) in the quadrant [0, h)^2: (iy, n-1-ix), (n-1-ix, n-1-iy), (n-1-iy, ix).
+// A workgroup owns a 16 x 16 tile of the quadrant. Per chunk of 16 angles, every wavefront
+// projects the tile's corners (lanes 4j .. 4j+3 for angle j of the chunk), the four rows of each
+// angle share that footprint (the four rotated tiles project alike), and the workgroup stages
+// the four rows' footprint segments (64 cells) into LDS with 16-byte loads while the previous
+// chunk is sampled from the other buffer. Samples read their tap pair from LDS at immediate
+// offsets, one address per angle for all four rows. The plan checks the symmetry and that every
+// footprint fits a segment; the per-lane clamp keeps every LDS read inside its segment.
+constexpr int kQT = 16;            // tile edge, pixels
+constexpr int kQThreads = kQT * kQT;
+constexpr int kQChunk = 16;        // angles per staged chunk
+constexpr int kQSeg = 64;          // staged cells per row
+static_assert(kQThreads == 256 && kQChunk * 4 * (kQSeg / 4) == 4 * kQThreads,
+              "each thread stages four quads per chunk");
+
+template <class SV, class SS, int WPOW>
+__global__ __launch_bounds__(kQThreads) void k_bp2d_qt(typename SV::raw *__restrict__ vol,
+                                                       const typename SS::raw *__restrict__ spad,
+                                                       const float4 *__restrict__ meta, int n,
+                                                       int D, int A, float inv_du, float half_d) {
+    __shared__ float seg[2][kQChunk * 4 * kQSeg]; // two chunks of [angle][row k][cell]
+    const int h = n / 2, Q = A / 4, Dp = D + 2 * kPadDet;
+    const size_t b = blockIdx.z;
+    const uint32_t sino_elems = (uint32_t)A * Dp;
+    const rsrc_t R = make_rsrc(spad + b * sino_elems, bytes_of<SS>(sino_elems));
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int bx = blockIdx.x * kQT, by = blockIdx.y * kQT;
+    const int ix = bx + (t & (kQT - 1)), iy = by + t / kQT;
+    const float x = (float)ix - (float)h + 0.5f, y = (float)iy - (float)h + 0.5f;
+    const float px = x * inv_du, py = y * inv_du;
+    const float hi = (float)D + 2.0f;     // padded coordinates: the row's zero border included
+    const float c2 = half_d - 0.5f + 2.0f; // D/2, the lerp's texel-centre shift and the border
+    // the tile corner (pixel centre) this lane projects for angle lane >> 2 of a chunk
+    const int cxi = (lane & 1) ? min(bx + kQT - 1, h - 1) : bx;
+    const int cyi = (lane & 2) ? min(by + kQT - 1, h - 1) : by;
+    const float qx = (float)cxi - (float)h + 0.5f, qy = (float)cyi - (float)h + 0.5f;
+    const float qpx = qx * inv_du, qpy = qy * inv_du;
+    // staging share: quads q = t & 15 of row k = (t >> 4) & 3 of angles j = wv + 4 m, m < 4
+    const int sq = t & 15, sk = (t >> 4) & 3;
+    constexpr int esz = (int)sizeof(typename SS::raw);
+
+    // Per chunk c, lanes 4 j .. 4 j + 3 of every wavefront hold angle j's segment: s0 (its first
+    // padded cell), c2 - s0 (the sample coordinate's constant in segment coordinates) and the
+    // largest lower-tap coordinate min(D + 2 - s0, kQSeg - 2) (the row's end, or the segment's).
+    struct Seg {
+        int s0;
+        float c2s, hi;
+    };
+    auto footprint = [&](int c) -> Seg {
+        const int a = min(c + (lane >> 2), A - 1);
+        const float4 m = meta[a];
+        const float depth = fmaf(m.x, qx, fmaf(-m.y, qy, m.w));
+        const float eta = fmaf(m.y, qpx, m.x * qpy);
+        const float w = m.z * __builtin_amdgcn_rcpf(depth);
+        float v = __builtin_amdgcn_fmed3f(fmaf(w, eta, c2), 0.0f, hi);
+        v = fminf(v, __shfl_xor(v, 1, 64));
+        v = fminf(v, __shfl_xor(v, 2, 64));
+        // at most Dp - kQSeg, so that a segment never runs past its row (the plan requires
+        // Dp >= kQSeg): no staged quad reads another row or straddles the array's end
+        const int s0 = min(max(floor_i32(v) - 1, 0), Dp - kQSeg);
+        return {s0, c2 - (float)s0, fminf(hi - (float)s0, (float)(kQSeg - 2))};
+    };
+    auto load = [&](int c, int s0v, Quad (&qd)[4]) {
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) {
+            const int j = wv + 4 * mm;
+            const int s0 = __builtin_amdgcn_readlane(s0v, 4 * j);
+            // row of angle c + j (past the last angle: the last one's, loaded and unused),
+            // turned by k quarters
+            int r = min(c + j, A - 1) + sk * Q;
+            r = r >= A ? r - A : r;
+            qd[mm] = buf_quad_at<SS>(R, (r * Dp + s0 + 4 * sq) * esz);
+        }
+    };
+    auto store = [&](int buf, const Quad (&qd)[4]) {
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) {
+            const int j = wv + 4 * mm;
+            *reinterpret_cast<float4 *>(&seg[buf][(j * 4 + sk) * kQSeg + 4 * sq]) =
+                make_float4(qd[mm].v[0], qd[mm].v[1], qd[mm].v[2], qd[mm].v[3]);
+        }
+    };
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // angle c + j of a chunk whose segments are in `sg`; TAIL: the angle may lie past the last
+    // one (then it adds nothing)
+    auto sample = [&](int c, int j, const Seg &sg, int bufofs, auto tail) {
+        constexpr bool TAIL = decltype(tail)::value;
+        if (TAIL && c + j >= A) return;
+        const float4 m = meta[c + j]; // wave-uniform: scalar loads
+        const float depth = fmaf(m.x, x, fmaf(-m.y, y, m.w)); // xi + DSO
+        const float eta = fmaf(m.y, px, m.x * py);
+        const float w = m.z * __builtin_amdgcn_rcpf(depth);
+        const float c2s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sg.c2s), 4 * j));
+        const float hj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sg.hi), 4 * j));
+        const float l = __builtin_amdgcn_fmed3f(fmaf(w, eta, c2s), 0.0f, hj);
+        const float al = __builtin_amdgcn_fractf(l);
+        const float wt = WPOW == 2 ? w * w : w;
+        // byte offset of the angle's four rows, opaque in a scalar register so that the rows
+        // stay immediate offsets of one address
+        int jb = bufofs + j * 4 * kQSeg * 4;
+        asm("" : "+s"(jb));
+        const float *p = reinterpret_cast<const float *>(
+            reinterpret_cast<const char *>(&seg[0][0]) + jb) + floor_i32(l);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float lo = p[k * kQSeg], up = p[k * kQSeg + 1];
+            acc[k] = fmaf(fmaf(al, up - lo, lo), wt, acc[k]);
+        }
+    };
+    using False = std::false_type;
+    using True = std::true_type;
+    Quad qd[4];
+    Seg sg = footprint(0);
+    load(0, sg.s0, qd);
+    store(0, qd);
+    __syncthreads();
+    int cur = 0;
+    for (int c = 0; c < A; c += kQChunk, cur ^= 1) {
+        // next chunk's loads in flight while this one is sampled
+        const int cn = c + kQChunk;
+        Seg sn{};
+        if (cn < A) {
+            sn = footprint(cn);
+            load(cn, sn.s0, qd);
+        }
+        const int bufofs = cur * kQChunk * 4 * kQSeg * 4;
+        if (cn <= A) {
+#pragma unroll
+            for (int j = 0; j < kQChunk; ++j) sample(c, j, sg, bufofs, False{});
+        } else {
+#pragma unroll
+            for (int j = 0; j < kQChunk; ++j) sample(c, j, sg, bufofs, True{});
+        }
+        if (cn < A) store(cur ^ 1, qd);
+        sg = sn;
+        __syncthreads();
+    }
+    if (ix >= h || iy >= h) return;
+    const int m1 = n - 1;
+    const int px4[4] = {ix, iy, m1 - ix, m1 - iy};
+    const int py4[4] = {iy, m1 - ix, m1 - iy, ix};
+    typename SV::raw *vb = vol + b * (size_t)n * n;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = (size_t)py4[k] * n + px4[k];
+        vb[i] = from_f32<SV>(to_f32<SV>(vb[i]) + acc[k] * inv_du);
+    }
+}
+
 // ---------------------------------------------------------------- adjoint ------------------
 
 // Largest |y q.x| over the rays of each sinogram of a batch (q.x the ray's scale), in two
@@ -859,6 +1018,38 @@ void set_plan_shape(cmi_plan2d &p, int device, int vol_prec, const uint64_t vol_
     p.A = (int64_t)nangles;
 }
 
+// Whether the backprojection by quarter turns (k_bp2d_qt) applies: a square image with an even
+// side, unit pixels and no offset (the pixel centres, at (i - n/2) + 1/2 as the reference places
+// them, backward_2d.hpp:58-59, are then symmetric under quarter turns), angles in quadruples
+// a + k A/4 that are quarter turns apart with equal DSO and DSD, a padded detector row at least
+// as long as a staged segment, and every 16 x 16 tile's detector footprint within a staged
+// segment: the projection of a segment of length L at
+// depth >= DSO - r (r the largest pixel-centre radius) spans at most
+// L DSD / (DSO - r) sqrt(1 + (r / (DSO - r))^2) / du cells.
+bool quarter_turns(const fan2d::Scalars &g, const double *angles, const double *DSO,
+                   const double *DSD, int64_t A) {
+    if (g.nx != g.ny || g.nx % 2 != 0 || g.nx < 2 || g.sx != 1.0 || g.sy != 1.0 ||
+        g.ox != 0.0 || g.oy != 0.0 || A % 4 != 0 || A < 4 || g.D + 2 * kPad < kQSeg)
+        return false;
+    const int64_t Q = A / 4;
+    const double r = ((double)(g.nx / 2) - 0.5) * std::sqrt(2.0);
+    double wmax = 0.0;
+    for (int64_t a = 0; a < A; ++a) {
+        if (!(DSO[a] > r + 1.0)) return false;
+        const double ratio = r / (DSO[a] - r);
+        wmax = std::max(wmax, DSD[a] / (DSO[a] - r) * std::sqrt(1.0 + ratio * ratio));
+        if (a >= Q) continue;
+        for (int64_t k = 1; k < 4; ++k) {
+            const int64_t b = a + k * Q;
+            if (DSO[b] != DSO[a] || DSD[b] != DSD[a]) return false;
+            double d = std::remainder(angles[b] - angles[a] - (double)k * M_PI / 2.0, 2.0 * M_PI);
+            if (std::fabs(d) > 1e-9) return false;
+        }
+    }
+    const double span = wmax * (kQT - 1) * std::sqrt(2.0) / g.du;
+    return span + 4.0 <= (double)kQSeg;
+}
+
 // Shared tail of the plan constructors: per-angle tables to the device, then the FP64 ray
 // table (k_ray_table). `meta` (circular plans) or `bpv` (plans from views) is empty.
 cmi_status finish_plan2d(std::unique_ptr<cmi_plan2d> p, const std::vector<AngleGeom> &geo,
@@ -1052,6 +1243,7 @@ CMI_API cmi_status cmi_plan2d_create(int device, int vol_prec, const uint64_t vo
         geo[a] = fan2d::plan_angle(p->g, angles[a], DSO[a], DSD[a]);
         meta[a] = bp2d_meta(angles[a], DSO[a], DSD[a]);
     }
+    p->quarter = quarter_turns(p->g, angles, DSO, DSD, A);
     return finish_plan2d(std::move(p), geo, meta, {}, plan);
 }
 
@@ -1229,6 +1421,14 @@ CMI_API cmi_status cmi_backward_2d_weighted(const cmi_plan2d *plan, void *volume
             hipLaunchKernelGGL(kern, g, b, 0, s, (typename SV::raw *)volume, (const RS *)scratch,
                                plan->d_bpv, nx, ny, D, A, (float)plan->g.sx, (float)plan->g.sy,
                                (float)plan->g.ox, (float)plan->g.oy);
+        } else if (plan->quarter && weighting == CMI_BP2D_REFERENCE) {
+            // (the FBP weighting keeps the per-pixel body, which the serving kernels share, so
+            // that a served FBP volume equals the plain one bit for bit)
+            const unsigned th = (unsigned)((nx / 2 + kQT - 1) / kQT);
+            auto kern = k_bp2d_qt<SV, SS, 1>;
+            hipLaunchKernelGGL(kern, dim3(th, th, (unsigned)batch), dim3(kQThreads), 0, s,
+                               (typename SV::raw *)volume, (const RS *)scratch, plan->d_meta, nx,
+                               D, A, (float)(1.0 / plan->g.du), (float)D / 2.0f);
         } else {
             auto kern = weighting == CMI_BP2D_FBP ? k_bp2d<SV, SS, 2> : k_bp2d<SV, SS, 1>;
             hipLaunchKernelGGL(kern, g, b, 0, s, (typename SV::raw *)volume, (const RS *)scratch,

@@ -147,12 +147,17 @@ ADJ_BLOCKS = [(0, 6), (87, 93), (177, 183), (264, 270), (447, 453), (627, 633)]
 
 @pytest.mark.parametrize("a0,a1", ADJ_BLOCKS)
 def test_c2_adjoint_angle_blocks(oracle, g2, gpu, a0, a1):
+    """Adjoint of six-angle sinogram blocks at 1024^2: rel-L2 <= 1e-5 and max-abs <= 5e-5 of
+    max|ref|. A texel's value here is a sum of about a dozen samples, each weighted at an FP32
+    position (the other coordinate is fma(j, v, m) with |j| up to 700: about 3e-5 texels of
+    rounding), so the texel-wise bound is looser than for the full-angle operators, whose sums
+    average that rounding out."""
     rec, geom, _ = g2
     y = np.zeros((720, 1449))
     y[a0:a1] = np.random.default_rng(5 + a0).random((a1 - a0, 1449))
     ref = oracle.adj2d(rec, y, angles=(a0, a1))
     got = host(cm.adjoint(dev(y, gpu), geom))
-    check_close(got, ref, f"c2 adjoint (angles {a0}..{a1 - 1})")
+    check_close(got, ref, f"c2 adjoint (angles {a0}..{a1 - 1})", maxabs=5e-5)
 
 
 def test_c2_dot_product(g2, gpu):
