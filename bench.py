@@ -8,8 +8,9 @@ Workload (one "step" = one forward projection + one voxel-driven backprojection)
 
 Prints one JSON line: value = forward+backward pairs per second (wall clock over K steps,
 synchronised on both sides), per-op device times from HIP events on the stream the kernels
-run on, the HBM-roofline figure of the dominant kernel, and a CPU baseline that times the
-oracle's FP32 port of the reference arithmetic on a bounded sample of the same workload.
+run on, the rooflines of the dominant kernel (VALU issue, LDS and HBM, from committed per-launch
+counter counts over the live op time), and a CPU baseline that times the oracle's FP32 port of
+the reference arithmetic on the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 3]
 """
@@ -22,8 +23,13 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fan2d_1024.json")
+# MI355X peaks: 256 CUs at 2.4 GHz; FP32 VALU 128 lanes per clock per CU (157.3 TFLOP/s as FMA),
+# LDS 128 bytes per clock per CU, HBM3E 8 TB/s
+CUS, CLOCK_HZ = 256, 2.4e9
+VALU_PEAK_LANES = 128 * CUS * CLOCK_HZ
+LDS_PEAK_BPS = 128 * CUS * CLOCK_HZ
+HBM_PEAK_BPS = 8.0e12
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters_fan2d_1024.json")
 
 WORKLOAD = dict(n=1024, D=1449, A=720, DSO=2048.0, DSD=3072.0)
 
@@ -44,7 +50,7 @@ def build_case(torch, cm, phantom, np):
     return geom, img, sino, vol
 
 
-def algorithmic_bytes(info, w, s=4):
+def tap_bytes(info, w, s=4):
     """Bytes a texture-less gather moves if every interpolation tap came from HBM:
     forward  sum_rays N * 4 taps * s + A * D * s
     backward A * n^2 * 2 taps * s + 2 * n^2 * s (read-modify-write of the image)."""
@@ -52,6 +58,43 @@ def algorithmic_bytes(info, w, s=4):
     fwd = info["total_samples"] * 4 * s + w["A"] * w["D"] * s
     bwd = w["A"] * n2 * 2 * s + 2 * n2 * s
     return fwd, bwd
+
+
+def cold_calls(torch, cm, phantom, np):
+    """Plan construction and first calls on fresh geometries, wall clock with synchronisation:
+    the 1024^2 fan beam (plan with its FP64 per-ray table, first forward, first
+    backprojection) and the 256^3 cone beam (plan with its per-ray sample count, first forward).
+    The warm figures elsewhere exclude this one-off plan work."""
+    w = WORKLOAD
+    out = {}
+    img = torch.tensor(phantom.ellipses_2d(w["n"], seed=1), dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = cm.FanGeometry(DSD=w["DSD"] + 1.0, DSO=w["DSO"],
+                       angles=np.linspace(0, 2 * np.pi, w["A"], endpoint=False),
+                       vol_shape=(w["n"], w["n"]), det_shape=w["D"])
+    g.plan
+    t1 = time.perf_counter()
+    sino = cm.forward(img, g)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    cm.backward(sino, g)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    out["fan2d_1024"] = {"plan_ms": (t1 - t0) * 1e3, "first_fwd_ms": (t2 - t1) * 1e3,
+                         "first_bwd_ms": (t3 - t2) * 1e3, "plan_and_first_pair_ms": (t3 - t0) * 1e3}
+    vol = torch.tensor(phantom.ellipsoids_3d(256, seed=1), device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gc = cm.ConeGeometry(DSD=769.0, DSO=512.0, angles=np.linspace(0, 2 * np.pi, 360, endpoint=False),
+                         vol_shape=(256, 256, 256), det_shape=(384, 384))
+    gc.plan
+    t1 = time.perf_counter()
+    cm.forward(vol, gc)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out["cone3d_256"] = {"plan_ms": (t1 - t0) * 1e3, "first_fwd_ms": (t2 - t1) * 1e3}
+    return out
 
 
 def time_ops(torch, stream, fns, reps):
@@ -91,9 +134,9 @@ def cone_case(torch, cm, phantom, np, stream, reps):
     return {
         "workload": "cone3d 256^3 f32, 384x384 panel, 360 angles, DSO 512, DSD 768",
         "total_samples": info["total_samples"],
-        "fwd3d": {"ms": t["fwd3d"], "alg_GB_per_s": fwd_b / t["fwd3d"] / 1e6},
-        "bp3d": {"ms": t["bp3d"], "alg_GB_per_s": bp_b / t["bp3d"] / 1e6},
-        "bp3d_matched": {"ms": t["bp3d_matched"], "alg_GB_per_s": bp_b / t["bp3d_matched"] / 1e6},
+        "fwd3d": {"ms": t["fwd3d"], "tap_equiv_GB_per_s": fwd_b / t["fwd3d"] / 1e6},
+        "bp3d": {"ms": t["bp3d"], "tap_equiv_GB_per_s": bp_b / t["bp3d"] / 1e6},
+        "bp3d_matched": {"ms": t["bp3d_matched"], "tap_equiv_GB_per_s": bp_b / t["bp3d_matched"] / 1e6},
         "adj3d": {"ms": t["adj3d"]},
         "fwd_bwd_pairs_per_s": 1e3 / (t["fwd3d"] + t["bp3d"]),
     }
@@ -142,7 +185,10 @@ def serving_geometry(cm, np, kind, n, storage):
 def serving_case(torch, cm, phantom, np, n_jobs, seed=4):
     """Closed-loop throughput of the continuous-batching server (FP16 storage) against plain
     per-call fbp (FP32 and FP16 storage, one synchronised call per job), then an open-loop run
-    with Poisson arrivals at 70 % of the measured server capacity for p50 / p99 latency."""
+    with Poisson arrivals at 70 % of the measured server capacity for p50 / p99 latency.
+    Also reported: the drawn mix against the recipe's expectation, per-class latency, and the
+    bound on any batching speed-up over plain calls: the plain calls' wall time over their
+    device time (batching can remove launch and synchronisation gaps, not device work)."""
     from curadon_amd.serving import FBPJob, FBPServer
     rng = np.random.default_rng(seed)
     classes = [("fan", n) for n in SERVING_FAN] + [("cone", n) for n in SERVING_CONE]
@@ -162,9 +208,10 @@ def serving_case(torch, cm, phantom, np, n_jobs, seed=4):
         data[ci] = (kind, n, g32, g16, raw)
     torch.cuda.synchronize()
     log(f"bench: serving inputs ready ({n_jobs} jobs, {len(data)} classes)")
+    tags = {ci: f"{data[ci][0]}{data[ci][1]}" for ci in data}
 
     def jobs(arrivals=None):
-        return [FBPJob(geom=data[c][3], projections=data[c][4], tag=f"{data[c][0]}{data[c][1]}",
+        return [FBPJob(geom=data[c][3], projections=data[c][4], tag=tags[c],
                        arrival=0.0 if arrivals is None else float(arrivals[i]))
                 for i, c in enumerate(draw.tolist())]
 
@@ -174,6 +221,23 @@ def serving_case(torch, cm, phantom, np, n_jobs, seed=4):
         cm.fbp(data[c][4], data[c][3])
     FBPServer().run([FBPJob(geom=data[c][3], projections=data[c][4]) for c in data])
     torch.cuda.synchronize()
+
+    # plain per-call cost per class: device time (events on the current stream) and wall time
+    per_class = {}
+    stream = torch.cuda.current_stream()
+    for c in data:
+        dev_ms, wall_ms = [], []
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record(stream)
+            cm.fbp(data[c][4], data[c][3])
+            e1.record(stream)
+            torch.cuda.synchronize()
+            wall_ms.append((time.perf_counter() - t0) * 1e3)
+            dev_ms.append(e0.elapsed_time(e1))
+        per_class[tags[c]] = {"plain_f16_device_ms": float(np.median(dev_ms)),
+                              "plain_f16_wall_ms": float(np.median(wall_ms))}
 
     def plain(which):
         t0 = time.perf_counter()
@@ -195,10 +259,11 @@ def serving_case(torch, cm, phantom, np, n_jobs, seed=4):
     opened = jobs(arrivals)
     srv2 = FBPServer(hbm_budget=96 << 30)
     st_open = srv2.run(opened)
-    per_class = {}
+    lat = {}
     for j in opened:
-        per_class.setdefault(j.tag, []).append(j.latency * 1e3)
+        lat.setdefault(j.tag, []).append(j.latency * 1e3)
     # storage error of each class: served FP16 volume against the plain FP32 operator chain
+    # (the FP64 pipeline comparison of every class is a GPU test, test_gpu_serving.py)
     errors = {}
     for j, c in zip(closed, draw.tolist()):
         if j.tag in errors:
@@ -208,31 +273,66 @@ def serving_case(torch, cm, phantom, np, n_jobs, seed=4):
                               torch.linalg.vector_norm(ref))
     log(f"bench: serving open loop p50 {st_open['latency_ms_p50']:.2f} ms, "
         f"p99 {st_open['latency_ms_p99']:.2f} ms at {lam:.1f} jobs/s")
+    counts = {tags[c]: int((draw == c).sum()) for c in data}
+    expected = {f"{k}{n}": n_jobs * p for (k, n), p in zip(classes, prob)}
+    dev_sum = sum(per_class[tags[c]]["plain_f16_device_ms"] for c in draw.tolist()) * 1e-3
+    wall_sum = sum(per_class[tags[c]]["plain_f16_wall_ms"] for c in draw.tolist()) * 1e-3
+    for t, v in per_class.items():
+        v["jobs_drawn"] = counts.get(t, 0)
+        v["jobs_expected"] = expected.get(t, 0.0)
+        if t in lat:
+            v["open_loop_p50_ms"] = float(np.percentile(lat[t], 50))
+            v["open_loop_p99_ms"] = float(np.percentile(lat[t], 99))
+            v["open_loop_max_ms"] = float(np.max(lat[t]))
+        v["rel_l2_vs_plain_f32"] = errors.get(t)
     return {
         "workload": "FBP/FDK job mix: 70% fan (n in 128..2048, D = ceil(sqrt2 n)), 30% cone "
                     "(n in 128..384, panel 1.5n); A = 2n over 2pi, DSO 2n, DSD 3n; FP32 raw "
                     "projections resident in HBM; cosine weight + ramp filter + weighted "
                     "backprojection",
         "jobs": n_jobs,
-        "mix": {t: int(sum(1 for j in closed if j.tag == t)) for t in sorted({j.tag for j in closed})},
         "storage": "f16 (filtered projections and volume), FP32 filtering and accumulation",
         "closed_loop": {
             "serving_jobs_per_s": st_closed["jobs_per_s"],
             "plain_f32_jobs_per_s": n_jobs / t_plain32,
             "plain_f16_jobs_per_s": n_jobs / t_plain16,
-            "speedup_vs_plain_f32": (n_jobs / st_closed["makespan_s"]) / (n_jobs / t_plain32),
-            "speedup_vs_plain_f16": (n_jobs / st_closed["makespan_s"]) / (n_jobs / t_plain16),
+            "speedup_vs_plain_f32": t_plain32 / st_closed["makespan_s"],
+            "speedup_vs_plain_f16": t_plain16 / st_closed["makespan_s"],
+            "speedup_bound_vs_plain_f16": wall_sum / dev_sum,
+            "plain_f16_device_s": dev_sum,
             "launches": srv.rounds,
         },
         "open_loop": dict(st_open, arrival_rate_per_s=lam, launches=srv2.rounds,
-                          max_active=srv2.max_active_seen,
-                          per_class_p50_ms={k: float(np.percentile(v, 50))
-                                            for k, v in sorted(per_class.items())},
-                          per_class_p99_ms={k: float(np.percentile(v, 99))
-                                            for k, v in sorted(per_class.items())}),
-        "rel_l2_vs_plain_f32": errors,
+                          max_active=srv2.max_active_seen),
+        "per_class": per_class,
     }
 
+
+def kernels_only(torch, cm, phantom, np, which, reps):
+    """The operators of one workload, `reps` times each after one untimed call, for a profiler
+    (rocprofv3 counters and traces attribute each kernel by name)."""
+    if which == "fan2d":
+        geom, x, sino, vol = build_case(torch, cm, phantom, np)
+        ops = [("forward", lambda: cm.forward(x, sino, geom)),
+               ("backprojection", lambda: cm.backward(sino, vol, geom)),
+               ("adjoint", lambda: cm.adjoint(sino, vol, geom))]
+    else:
+        geom = cm.ConeGeometry(DSD=768.0, DSO=512.0,
+                               angles=np.linspace(0, 2 * np.pi, 360, endpoint=False),
+                               vol_shape=(256, 256, 256), det_shape=(384, 384))
+        x = torch.tensor(phantom.ellipsoids_3d(256, seed=0), device="cuda:0")
+        sino = torch.empty(360, 384, 384, device="cuda:0")
+        vol = torch.zeros_like(x)
+        ops = [("forward", lambda: cm.forward(x, sino, geom)),
+               ("backprojection", lambda: cm.backward(sino, vol, geom)),
+               ("adjoint", lambda: cm.adjoint(sino, vol, geom))]
+    geom.plan
+    torch.cuda.synchronize()
+    for name, fn in ops:
+        for _ in range(reps + 1):
+            fn()
+        torch.cuda.synchronize()
+        log(f"bench: {which} {name} x {reps + 1} done")
 
 
 def copy_calibration(torch, mib, reps):
@@ -246,66 +346,66 @@ def copy_calibration(torch, mib, reps):
     log(f"bench: {reps} copies of {mib} MiB done")
 
 
-def cpu_baseline(np, geom_rec):
-    """Times the oracle's FP32 port (OpenMP) on a bounded sample: forward on 144 of 720
-    angles and backprojection on 256 of 1024 image rows, scaled to whole operators."""
-    from oracle import oracle
-    from curadon_amd import phantom
-    w = WORKLOAD
-    img = phantom.ellipses_2d(w["n"], seed=0).astype(np.float32)
-    fa = 144
-    fr = 256
-    oracle.fwd2d_f32(geom_rec, img, angles=(0, 8))  # warm-up (thread pool, pages)
-    t0 = time.perf_counter()
-    sino = oracle.fwd2d_f32(geom_rec, img, angles=(0, fa))
-    t1 = time.perf_counter()
-    oracle.bp2d_f32(geom_rec, sino, rows=(0, fr))
-    t2 = time.perf_counter()
-    t_fwd = (t1 - t0) * w["A"] / fa
-    t_bwd = (t2 - t1) * w["n"] / fr
-    cores = oracle.num_threads()
-    # one core: a smaller sample of the same operators
-    fa1, fr1 = 16, 32
-    oracle.set_num_threads(1)
-    try:
-        t0 = time.perf_counter()
-        oracle.fwd2d_f32(geom_rec, img, angles=(0, fa1))
-        t1 = time.perf_counter()
-        oracle.bp2d_f32(geom_rec, sino, rows=(0, fr1))
-        t2 = time.perf_counter()
-    finally:
-        oracle.set_num_threads(cores)
-    t_fwd1 = (t1 - t0) * w["A"] / fa1
-    t_bwd1 = (t2 - t1) * w["n"] / fr1
+def cpu_baseline():
+    """The oracle's FP32 port of the reference arithmetic on this host's cores
+    (oracle/cpu_baseline.py): 64^2 with 3 warm-ups + 50 runs of the forward and of the
+    backprojection, 1024^2 with 1 warm-up + 5 full runs, a stated sample of the 256^3 cone beam,
+    and the 64^2 protocol on one thread."""
+    from oracle import cpu_baseline as cb
+    r = cb.run(full_cone=False)
+    f = r["fan1024"]
     return {
-        "value": 1.0 / (t_fwd + t_bwd), "unit": "fwd+bwd pairs/s",
-        "cores": cores, "kind": "port",
-        "sample": f"FP32 port of the reference arithmetic, forward on {fa}/{w['A']} angles "
-                  f"and backprojection on {fr}/{w['n']} rows, scaled to full operators; one "
-                  f"core: {fa1} angles and {fr1} rows",
-        "fwd_s": t_fwd, "bwd_s": t_bwd, "cpu_model": _cpu_model(),
-        "single_core": {"value": 1.0 / (t_fwd1 + t_bwd1), "fwd_s": t_fwd1, "bwd_s": t_bwd1},
+        "value": f["pairs_per_s"], "unit": "fwd+bwd pairs/s", "cores": r["threads"],
+        "kind": "port",
+        "sample": "FP32 port of the reference arithmetic (oracle/cpu_baseline.py) on the 1024^2 "
+                  "workload: 1 warm-up + 5 full runs of the forward and of the backprojection "
+                  "on all host threads",
+        "cpu_model": r["cpu_model"],
+        "fan1024": f, "fan64": r["fan64"], "cone256": r["cone256"],
+        "single_thread": r["single_thread"],
     }
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size), or None."""
+def roofline(kernel, op_ms, tap_bytes):
+    """Rooflines of `kernel` (fwd2d or bp2d) at the live mean op time `op_ms`.
+
+    Per-launch counts come from the committed rocprofv3 counter summary of this workload
+    (profiles/counters_fan2d_1024.json): VALU wave instructions (SQ_INSTS_VALU, 64 lanes each),
+    LDS bytes (SQ_INSTS_LDS_{LOAD,STORE,ATOMIC}_BANDWIDTH, 64-byte units) and HBM bytes
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy). Each is divided by the op time measured
+    here with HIP events on the launch stream (the op includes its staging kernel, about 10 us)
+    and set against the MI355X peak; `bound` names the roof closest to binding. The tap count
+    (every interpolation tap priced as an HBM read) is kept as an equivalent figure only."""
+    t = op_ms * 1e-3
+    tap = {"GB_per_s": tap_bytes / t / 1e9, "bytes_per_launch": tap_bytes,
+           "note": "every interpolation tap counted as a 4-byte HBM read; LDS, L1 and L2 serve "
+                   "almost all of them, so this is an equivalent rate, not a bound"}
     try:
-        with open(TRAFFIC_FILE) as f:
-            return json.load(f)["kernels"][kernel]["hbm_bytes_per_launch"]
+        with open(COUNTERS_FILE) as f:
+            k = json.load(f)["kernels"][kernel]
     except (OSError, KeyError, ValueError):
-        return None
-
-
-def _cpu_model():
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
+        return {"bound": None, "kernel": kernel, "achieved": None, "peak": None, "unit": None,
+                "frac": None, "traffic": None, "equivalent": tap,
+                "note": "no committed counter summary for this kernel"}
+    d = k["derived"]
+    roofs = {
+        "valu": {"achieved": d["valu_wave_instr_per_launch"] * 64 / t / 1e12,
+                 "peak": VALU_PEAK_LANES / 1e12, "unit": "T FP32 lane-instr/s"},
+        "lds": {"achieved": d["lds_bytes_per_launch"] / t / 1e9, "peak": LDS_PEAK_BPS / 1e9,
+                "unit": "GB/s"},
+        "hbm": {"achieved": d["hbm_bytes_per_launch"] / t / 1e9, "peak": HBM_PEAK_BPS / 1e9,
+                "unit": "GB/s"},
+    }
+    for r in roofs.values():
+        r["frac"] = r["achieved"] / r["peak"]
+    bound = max(roofs, key=lambda n: roofs[n]["frac"])
+    b = roofs[bound]
+    return {"bound": bound, "kernel": k["kernel"], "achieved": b["achieved"], "peak": b["peak"],
+            "unit": b["unit"], "frac": b["frac"], "traffic": d["hbm_bytes_per_launch"],
+            "roofs": roofs, "lds_bank_conflict_frac": d["lds_bank_conflict_frac"],
+            "l2_hit": d["l2_hit"], "equivalent": tap,
+            "source": "per-launch counts: profiles/counters_fan2d_1024.json; time: HIP events "
+                      "around the op on its launch stream, mean over the timed steps"}
 
 
 def main():
@@ -316,12 +416,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary workloads (256^3 cone beam, batched 512^2)")
-    ap.add_argument("--serving-jobs", type=int, default=40,
+    ap.add_argument("--serving-jobs", type=int, default=400,
                     help="jobs in the FBP serving workload (0 skips it)")
     ap.add_argument("--serving-only", action="store_true",
                     help="run only the FBP serving workload and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run plain device copies of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--kernels-only", choices=("fan2d", "cone3d"), default=None,
+                    help="only run the forward, backprojection and adjoint of the 1024^2 fan "
+                         "beam or the 256^3 cone beam, --steps times each (for profilers)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -338,6 +441,9 @@ def main():
     if args.copy_calibration:
         copy_calibration(torch, args.copy_calibration, max(1, args.steps))
         return
+    if args.kernels_only:
+        kernels_only(torch, cm, phantom, np, args.kernels_only, max(1, args.steps))
+        return
     if args.serving_only:
         print(json.dumps(serving_case(torch, cm, phantom, np, max(1, args.serving_jobs))),
               flush=True)
@@ -346,7 +452,7 @@ def main():
     log("bench: building plan and inputs")
     geom, img, sino, vol = build_case(torch, cm, phantom, np)
     info = geom.plan.info()
-    fwd_bytes, bwd_bytes = algorithmic_bytes(info, WORKLOAD)
+    fwd_bytes, bwd_bytes = tap_bytes(info, WORKLOAD)
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):
@@ -374,8 +480,8 @@ def main():
     fwd_gbs = fwd_bytes / (fwd_ms.mean() * 1e-3) / 1e9
     bwd_gbs = bwd_bytes / (bwd_ms.mean() * 1e-3) / 1e9
     dominant = "fwd2d" if fwd_ms.mean() >= bwd_ms.mean() else "bp2d"
-    dom_gbs = fwd_gbs if dominant == "fwd2d" else bwd_gbs
-    dom_bytes = fwd_bytes if dominant == "fwd2d" else bwd_bytes
+    roof = (roofline("fwd2d", float(fwd_ms.mean()), fwd_bytes) if dominant == "fwd2d" else
+            roofline("bp2d", float(bwd_ms.mean()), bwd_bytes))
 
     # matched adjoint (autograd's backward of the forward), timed separately
     adj_vol = torch.zeros_like(vol)
@@ -398,6 +504,11 @@ def main():
                  "batch2d": batch_case(torch, cm, phantom, np, stream, 3)}
         log(f"bench: cone3d {extra['cone3d']['fwd3d']['ms']:.2f} / {extra['cone3d']['bp3d']['ms']:.2f} ms")
 
+    cold = None
+    if not args.no_extra:
+        log("bench: cold calls")
+        cold = cold_calls(torch, cm, phantom, np)
+
     serving = None
     if args.serving_jobs > 0:
         log("bench: FBP serving workload")
@@ -406,14 +517,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle FP32 port)")
-        import types
-        w = WORKLOAD
-        rec = types.SimpleNamespace(
-            vol_shape=np.array([w["n"], w["n"]]), vol_spacing=np.ones(2), vol_offset=np.zeros(2),
-            det_shape=w["D"], det_spacing=1.0, det_offset=0.0,
-            angles=np.linspace(0, 2 * np.pi, w["A"], endpoint=False),
-            DSO=np.full(w["A"], w["DSO"]), DSD=np.full(w["A"], w["DSD"]))
-        cpu = cpu_baseline(np, rec)
+        cpu = cpu_baseline()
 
     props = torch.cuda.get_device_properties(0)
     pair_ms = wall / args.steps * 1e3
@@ -438,24 +542,18 @@ def main():
         "ops": {
             "fwd2d": {"ms_mean": float(fwd_ms.mean()), "ms_p50": float(np.percentile(fwd_ms, 50)),
                       "ms_p99": float(np.percentile(fwd_ms, 99)), "op_per_s": 1e3 / fwd_ms.mean(),
-                      "alg_GB_per_s": fwd_gbs, "alg_bytes": fwd_bytes},
+                      "tap_equiv_GB_per_s": fwd_gbs, "tap_bytes": fwd_bytes},
             "bp2d": {"ms_mean": float(bwd_ms.mean()), "ms_p50": float(np.percentile(bwd_ms, 50)),
                      "ms_p99": float(np.percentile(bwd_ms, 99)), "op_per_s": 1e3 / bwd_ms.mean(),
-                     "alg_GB_per_s": bwd_gbs, "alg_bytes": bwd_bytes},
-            "adj2d": {"ms_mean": adj_ms, "op_per_s": 1e3 / adj_ms,
-                      "alg_GB_per_s": fwd_bytes / (adj_ms * 1e-3) / 1e9},
+                     "tap_equiv_GB_per_s": bwd_gbs, "tap_bytes": bwd_bytes},
+            "adj2d": {"ms_mean": adj_ms, "op_per_s": 1e3 / adj_ms},
             "pair": {"ms_p50": float(np.percentile(step_ms, 50)),
                      "ms_p99": float(np.percentile(step_ms, 99))},
             "total_samples": info["total_samples"],
         },
-        "roofline": {
-            "bound": "hbm", "kernel": dominant, "achieved": dom_gbs, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": dom_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
-            "alg_bytes_per_launch": dom_bytes,
-            "note": "achieved = algorithmic bytes (every interpolation tap counted as an HBM "
-                    "read) / mean op time from HIP events on the launch stream",
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "cold": cold,
         "extra": extra,
         "serving": serving,
         "device": {"name": props.name, "cus": props.multi_processor_count,

@@ -4,7 +4,8 @@ Jobs (raw FP32 projections + a FanGeometry or ConeGeometry) arrive over time. Th
 
 * admits a job once its HBM footprint fits the remaining budget and an active slot is free
   (first come, first admitted);
-* prepares it on a prep stream: cosine weights, ramp filter (row pairs through complex
+* prepares it on one of two prep streams (jobs with little filtering work on their own, so
+  they do not wait behind a large job's filtering): cosine weights, ramp filter (row pairs through complex
   torch.fft transforms, rocFFT) in FP32, scale, and staging of the filtered projections into
   the zero-bordered layout of the backprojector in the storage precision (FP16 by default);
   the volume is allocated in the storage precision and stays resident;
@@ -45,6 +46,17 @@ CONE_COST = 1.15
 PIXELS_PER_TILE_2D = 64 * 4
 # voxels per tile of the cone-beam kernel: y runs (16 x 16 x 16) / z columns (16 x 16 x 8)
 VOXELS_PER_TILE_3D = {True: 16 * 16 * 16, False: 16 * 16 * 8}
+
+
+# Projection elements (rows x cells) up to which a job's filtering counts as small and is
+# prepared on the small-job prep stream.
+PREP_SMALL_ELEMENTS = 1 << 22
+
+
+def _prep_elements(g) -> int:
+    if isinstance(g, FanGeometry):
+        return int(g.nangles) * int(g.det_shape)
+    return int(g.nangles) * int(np.prod([int(v) for v in g.det_shape]))
 
 
 @dataclass
@@ -127,7 +139,10 @@ class FBPServer:
         self.oldest_share = float(oldest_share)
         self.max_rounds_in_flight = int(max_rounds_in_flight)
         self._native = _lib.Server(device, max_jobs=max_active)
-        self._prep = torch.cuda.Stream(self.device)
+        # two prep streams (default priority): jobs with little filtering work are prepared on
+        # their own stream, so they do not queue behind a large job's filtering
+        self._prep_small = torch.cuda.Stream(self.device)
+        self._prep_large = torch.cuda.Stream(self.device)
         # launches alternate between streams: while one launch drains its last tiles, the
         # next one's workgroups take the freed compute units
         self._compute = [torch.cuda.Stream(self.device) for _ in range(max(1, compute_streams))]
@@ -170,7 +185,8 @@ class FBPServer:
         if not fan and not isinstance(g, ConeGeometry):
             raise ValueError("Unknown geometry type")
         dev = self.device
-        with torch.cuda.stream(self._prep):
+        prep = self._prep_large if _prep_elements(g) > PREP_SMALL_ELEMENTS else self._prep_small
+        with torch.cuda.stream(prep):
             cw, row_scale, scale, table = self._filter_parts(g, job.window,
                                                              job.frequency_scaling)
             ycols = False
@@ -202,10 +218,10 @@ class FBPServer:
                                                       nx=nx, ny=ny, nz=nz))
                 tile_cost = float(VOXELS_PER_TILE_3D[bool(ycols)] * A * CONE_COST)
             ev = torch.cuda.Event()
-            ev.record(self._prep)
+            ev.record(prep)
         job.volume = volume
-        # p and y are released here: the allocator reuses their blocks on the prep stream only,
-        # after the staging kernel in stream order.
+        # p and y are released here: the allocator reuses their blocks on the same prep stream
+        # only, after the staging kernel in stream order.
         return _Active(job=job, fan=fan, ntiles=ntiles, tile_cost=tile_cost,
                        footprint=footprint(job, self.storage)[0], staged=staged, table=table,
                        ycols=ycols, keep=keep, prep_done=ev)
