@@ -329,12 +329,14 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
         P.c_lo = __builtin_amdgcn_readfirstlane(min(min(ea, eb), min(ec, ed)) - 1);
         P.c_hi = __builtin_amdgcn_readfirstlane(max(max(ea, eb), max(ec, ed)) + 1);
         const int W = P.c_hi + 2 - P.c_lo; // columns [c_lo, c_hi + 1] hold the taps
-        P.lds = W <= kPitch - 4;
+        // staged when the band fits a row and at least two groups of Q threads exist (so that
+        // K <= 16); a 64-thread workgroup stages bands of up to 128 columns
+        P.lds = W <= kPitch - 4 && 2 * ((W + 3) >> 2) <= nthreads;
         if (P.lds) {
-            const int Q = (W + 3) >> 2; // 1 .. 47, below nthreads (a multiple of 64)
+            const int Q = (W + 3) >> 2; // 1 .. 47, at most nthreads / 2 (a multiple of 32)
             const float rq = __builtin_amdgcn_rcpf((float)Q);
             const int G = __builtin_amdgcn_readfirstlane((int)(((float)nthreads + 0.5f) * rq));
-            // G' = 2^lg <= min(G, kBandRows), K = kBandRows >> lg (G >= 2 as Q <= 47)
+            // G' = 2^lg <= min(G, kBandRows), K = kBandRows >> lg <= 16 (G >= 2 as 2 Q <= nthreads)
             const int lg = min(31 - __builtin_clz((unsigned)G), 5);
             P.K = kBandRows >> lg;
             const int g = (int)(((float)threadIdx.x + 0.5f) * rq);
@@ -781,18 +783,30 @@ __device__ __forceinline__ float adj2d_ymax(const float *__restrict__ parts) {
 // each thread takes one ray, keeps its parameters in registers and walks the kT2 + 1 dominant
 // rows whose samples reach the tile. A sample on row pd (lower tap) with column c adds y q.x
 // times its forward weights to the texels (pd, floor c), (pd, floor c + 1), (pd + 1, ...)
-// (fan2d.hip header), in 64-bit fixed point (units of max|y q.x| / 2^30, one term fits in 32
-// bits) with integer LDS atomics: gfx950 runs them about 45 times faster than FP32 LDS atomic
-// adds, and integer sums are exact, so the result does not depend on the order of the
-// additions. y- and x-dominant rays add to separate tiles whose rows run along their dominant
-// axis, padded by one row and column on each side: the four taps of a sample are then four
-// immediate offsets from one LDS address and need no bounds checks. Each tile then adds its
-// sums to a 64-bit image with integer global atomics (several angle groups share a tile), and
-// k_adj2d_finish adds that image to the volume.
+// (fan2d.hip header), in fixed point with integer LDS atomics: gfx950 runs them about 45 times
+// faster than FP32 LDS atomic adds, and integer sums are exact, so the result does not depend on
+// the order of the additions. A term is rounded to units of max|y q.x| / 2^19 (so |term| <=
+// 2^19) and added to a 32-bit LDS cell: half the LDS atomic traffic of 64-bit cells, which
+// bounds this kernel. A candidate ray adds at most two terms to a cell (one sample per dominant
+// row, and a cell takes the lower-row weights of one sample and the upper-row weights of the
+// sample before), so a run of passes whose candidates number c adds at most 2 c 2^19 to a cell:
+// each thread moves its share of the cells into 64-bit registers and clears them before that
+// bound could pass 2^31 - 1 (runs of at most 2047 candidates), which keeps every 32-bit sum
+// exact. y- and x-dominant rays add to separate tiles whose rows run along their dominant axis,
+// padded by one row and column on each side: the four taps of a sample are then four immediate
+// offsets from one LDS address and need no bounds checks. Each tile then adds its sums, in units
+// of max|y q.x| / 2^30, to a 64-bit image with integer global atomics (several angle groups
+// share a tile), and k_adj2d_finish adds that image to the volume.
 constexpr int kT2 = 32;       // texel tile edge
-constexpr int kAdjAngles = 4; // angles whose candidate rays are staged together
+constexpr int kAdjAngles = 8; // angles whose candidate rays are staged together
+static_assert(kAdjAngles <= 16, "lanes 4 s .. 4 s + 3 of a wavefront project angle s");
 constexpr int kT2Pad = kT2 + 2;   // padded rows and columns
-constexpr int kT2Pitch = kT2 + 3; // odd pitch in 64-bit words
+constexpr int kT2Pitch = kT2 + 3; // odd pitch in words
+constexpr int kT2Cells = 2 * kT2Pad * kT2Pitch;
+constexpr int kT2Own = (kT2Cells + kMaxBlock - 1) / kMaxBlock; // cells per thread
+constexpr int kAdjBits = 19;        // a term's bound: 2^kAdjBits units
+// candidates per 32-bit accumulation run: 2 kAdjBudget 2^kAdjBits <= 2^31 - 2^20
+constexpr int kAdjBudget = (1 << (31 - kAdjBits - 1)) - 1;
 
 template <class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__restrict__ accimg,
@@ -804,16 +818,38 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
                                                          int ny, int D, int A, int G) {
     // tile[0]: y-dominant rays, rows along y; tile[1]: x-dominant rays, rows along x. Padded
     // by one row and column before and after, so every tap of a kept sample lands in the array.
-    __shared__ unsigned long long tile[2][kT2Pad * kT2Pitch];
+    // 32-bit cells while the angles are added; 64-bit cells (the same memory) for the totals.
+    __shared__ union {
+        unsigned int c32[2][kT2Pad * kT2Pitch];
+        unsigned long long c64[2][kT2Pad * kT2Pitch];
+    } sh;
+    auto &tile = sh.c32;
     const int tid = threadIdx.x;
+    long long own[kT2Own]; // this thread's cells tid + k kMaxBlock, moved out of LDS
+#pragma unroll
+    for (int k = 0; k < kT2Own; ++k) own[k] = 0;
+    // cells -> registers, then clear (every thread calls it)
+    auto drain = [&]() {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kT2Own; ++k) {
+            const int i = tid + k * kMaxBlock;
+            if (i < kT2Cells) {
+                own[k] += (int)(&tile[0][0])[i];
+                (&tile[0][0])[i] = 0u;
+            }
+        }
+        __syncthreads();
+    };
+    int run = 0; // candidates added since the last drain
     const int g = blockIdx.z % G;
     const size_t b = blockIdx.z / G;
     const int tx0 = blockIdx.x * kT2, ty0 = blockIdx.y * kT2;
     const int a0 = (int)((int64_t)g * A / G), a1 = (int)((int64_t)(g + 1) * A / G);
-    for (int i = tid; i < 2 * kT2Pad * kT2Pitch; i += kMaxBlock) (&tile[0][0])[i] = 0ull;
+    for (int i = tid; i < kT2Cells; i += kMaxBlock) (&tile[0][0])[i] = 0u;
     __syncthreads(); // the tiles are clear before any wave adds to them
     const float ymax = adj2d_ymax(parts + b * kAdjParts);
-    const float yscale = ymax > 0.0f ? 1073741824.0f / ymax : 0.0f;
+    const float yscale = ymax > 0.0f ? (float)(1 << kAdjBits) / ymax : 0.0f;
     // the tile's first texel in the ray table's centred padded frame
     const int px0 = tx0 + kPad - (nx + 2 * kPad) / 2, py0 = ty0 + kPad - (ny + 2 * kPad) / 2;
     // footprint box corner this lane projects (lanes 0..3 of every wavefront)
@@ -854,8 +890,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
             cnts[q] = cnt;
             total += cnt;
         }
-        // one candidate ray per thread, its parameters in registers
-        for (int ri = tid; ri < total; ri += kMaxBlock) {
+        // one candidate ray per thread, its parameters in registers; `total` and the pass
+        // sizes are the same for the whole workgroup
+        for (int base = 0; base < total; base += kMaxBlock) {
+            const int pass = min(kMaxBlock, total - base);
+            if (run + pass > kAdjBudget) {
+                drain();
+                run = 0;
+            }
+            run += pass;
+            const int ri = base + tid;
+            if (ri >= total) continue;
             int q = 0;
 #pragma unroll
             for (int u = 0; u < kAdjAngles; ++u)
@@ -864,7 +909,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
             const float4 qn = ns[r];
             const int n = __float_as_int(qn.y);
             if (n <= 0) continue;
-            const float yv = to_f32<SS>(sb[r]) * qn.x * yscale; // |yv| <= 2^30
+            const float yv = to_f32<SS>(sb[r]) * qn.x * yscale; // |yv| <= 2^kAdjBits
             if (yv == 0.0f) continue;
             const float4 e = ev[r];
             const bool ydom = fabsf(e.w) == 1.0f;
@@ -883,7 +928,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
             // Step t adds wl [1 - fa, fa] to tile row t - 1 and wh [1 - fa, fa] to row t at the
             // columns lc, lc + 1: padded rows t, t + 1 and columns lc + 1, lc + 2, four
             // immediate offsets from one address. Steps with lc outside [-1, kT2) miss the tile.
-            unsigned long long *row = &tile[ydom ? 0 : 1][t_lo * kT2Pitch + 1];
+            unsigned int *row = &tile[ydom ? 0 : 1][t_lo * kT2Pitch + 1];
             float jf = (float)(s * (d0 - 1 + t_lo - r0));
             const float fs = (float)s;
             for (int t = t_lo; t <= t_hi; ++t) {
@@ -893,15 +938,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
                 if ((unsigned)(lc + 1) <= (unsigned)kT2) {
                     const float fa = __builtin_amdgcn_fractf(c);
                     const float v1 = wl * fa, v0 = wl - v1, u1 = wh * fa, u0 = wh - u1;
-                    unsigned long long *q = row + lc;
-                    atomicAdd(q, (unsigned long long)(long long)round_i32(v0));
-                    atomicAdd(q + 1, (unsigned long long)(long long)round_i32(v1));
-                    atomicAdd(q + kT2Pitch, (unsigned long long)(long long)round_i32(u0));
-                    atomicAdd(q + kT2Pitch + 1, (unsigned long long)(long long)round_i32(u1));
+                    unsigned int *q = row + lc;
+                    atomicAdd(q, (unsigned int)round_i32(v0));
+                    atomicAdd(q + 1, (unsigned int)round_i32(v1));
+                    atomicAdd(q + kT2Pitch, (unsigned int)round_i32(u0));
+                    atomicAdd(q + kT2Pitch + 1, (unsigned int)round_i32(u1));
                 }
                 row += kT2Pitch;
             }
         }
+    }
+    drain();
+    // totals in units of max|y q.x| / 2^30
+#pragma unroll
+    for (int k = 0; k < kT2Own; ++k) {
+        const int i = tid + k * kMaxBlock;
+        if (i < kT2Cells) (&sh.c64[0][0])[i] = (unsigned long long)(own[k] * (1ll << (30 - kAdjBits)));
     }
     __syncthreads();
     unsigned long long *out = accimg + b * (size_t)nx * ny;
@@ -909,7 +961,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
         const int lx = i % kT2, ly = i / kT2;
         const int X = tx0 + lx, Y = ty0 + ly;
         const unsigned long long v =
-            tile[0][(ly + 1) * kT2Pitch + lx + 1] + tile[1][(lx + 1) * kT2Pitch + ly + 1];
+            sh.c64[0][(ly + 1) * kT2Pitch + lx + 1] + sh.c64[1][(lx + 1) * kT2Pitch + ly + 1];
         if (X < nx && Y < ny && v != 0ull) atomicAdd(&out[(size_t)Y * nx + X], v);
     }
 }
@@ -1475,10 +1527,10 @@ CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_
         hipLaunchKernelGGL((k_adj2d_absmax<SS>), dim3(kAdjParts, (unsigned)batch),
                            dim3(kMaxBlock), 0, s, (const typename SS::raw *)sinogram, plan->d_ns,
                            rays, parts);
-        // angle groups: enough workgroups to fill the GPU several times over
+        // angle groups: enough workgroups to fill the GPU many times over (about 32 per CU)
         const unsigned tx = (unsigned)((nx + kT2 - 1) / kT2), ty = (unsigned)((ny + kT2 - 1) / kT2);
         const uint64_t tiles = (uint64_t)tx * ty * batch;
-        const int G = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)A, (4096 + tiles - 1) / tiles));
+        const int G = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)A, (8192 + tiles - 1) / tiles));
         hipLaunchKernelGGL((k_adj2d_tile<SS>), dim3(tx, ty, (unsigned)(G * batch)),
                            dim3(kMaxBlock), 0, s, accimg, (const typename SS::raw *)sinogram,
                            plan->d_ageo, plan->d_ev, plan->d_ns, parts, nx, ny, D, A, G);

@@ -195,6 +195,30 @@ def test_forward_deterministic_and_block_independent(g1, gpu):
     assert torch.equal(cm.adjoint(y, geom), cm.adjoint(y, geom))
 
 
+def test_forward_wide_bands_all_block_shapes(oracle, gpu):
+    """Detector cells 2.5 texels apart: a 64-ray workgroup's bands are about 70 to 160 columns
+    wide, so some are staged in LDS and some sampled from global memory, and a 128-ray
+    workgroup's wide bands all take the global path. Every launch shape gives the same sums bit
+    for bit, and they match the oracle."""
+    rec = fan(256, 160, 90, 512.0, 768.0, du=2.5)
+    geom = as_fan_geometry(rec)
+    img = phantom.ellipses_2d(256, seed=4)
+    x = dev(img, gpu)
+    ref = oracle.fwd2d(rec, img)
+    plan = geom.plan
+    old = (plan.forward_block_x, plan.forward_block_y)
+    outs = []
+    try:
+        for bx, by in ((64, 1), (128, 1), (64, 2)):
+            plan.set_block(bx, by)
+            outs.append(cm.forward(x, geom))
+    finally:
+        plan.set_block(*old)
+    for o in outs[1:]:
+        assert torch.equal(outs[0], o)
+    check_close(host(outs[0]), ref, "wide-band forward")
+
+
 def test_block_shape_limit(g1):
     """Workgroups above the kernels' 256-thread launch bound are refused on the host."""
     plan = g1[1].plan
