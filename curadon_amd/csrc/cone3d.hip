@@ -330,20 +330,34 @@ __global__ __launch_bounds__(kMaxBlock) void k_absmax_parts(const typename S::ra
 // whose trilinear footprint reaches the tile), through the geometry-matched matrices onto the
 // panel; the rays through the covered pixels (plus one pixel of margin) are the candidates. Each
 // candidate walks only its samples inside the grown box, at the forward's FP32 positions, and
-// adds y * w to the tile voxels its footprint covers. The accumulators are 64-bit fixed point
-// in units of max|y| / 2^30: a contribution y * w is rounded to that grid (|y w| <= max|y|, so
-// it fits in 32 bits) and added with an integer LDS atomic, which gfx950 executes about 45
-// times faster than an FP32 LDS atomic add. Integer sums are exact, so the result does not
-// depend on the order of the additions; the rounding of each term is at most max|y| 2^-31.
+// adds y * w to the tile voxels its footprint covers. The accumulators are fixed point in units
+// of max|y| / 2^19: a contribution y * w is rounded to that grid (|term| <= 2^19) and added to a
+// 32-bit LDS cell with an integer LDS atomic, which gfx950 executes about 45 times faster than an
+// FP32 LDS atomic add (and 32-bit cells halve the LDS atomic traffic of 64-bit ones). Integer
+// sums are exact, so the result does not depend on the order of the additions; the rounding of
+// each term is at most max|y| 2^-20.
+// Overflow: a ray adds at most two terms to a voxel (its samples are one apart along the
+// dominant axis, so at most two have their lower corner within one voxel of it), and only rays
+// through the projection of the voxel's 2 x 2 x 2 neighbourhood reach it. Per angle that
+// projection is at most Du x Dv pixels, with Du <= 2 sum_i |du/dx_i| bounded over the grown box
+// by (sum |m0..2| + max|u| sum |m8..10|) / min w from its corners (and Dv alike), so one angle
+// adds at most 2 (Du + 2)(Dv + 2) terms to a voxel (2 per candidate where that is smaller).
+// The workgroup keeps a running bound of these and, before it could pass 2^31 / 2^19, moves
+// the 32-bit sums of its 16^3 voxels into 64-bit registers (16 per thread) and clears the tile.
 // The tile is added to the volume once, in its storage type.
 constexpr int kAT = 16;
 constexpr int kPh = 4; // candidate rays of one wavefront are kPh pixels apart in u and v
 // LDS tile: padded by one voxel before and after on each axis, so that every tap of a sample
 // whose lower corner lies in [-1, kAT) on each axis lands in the array (taps outside the tile
 // are added to the border and dropped): the eight taps are then immediate offsets from one LDS
-// address. Odd row and slice pitches (in 64-bit words) spread voxels that differ only in y or
-// z over different banks, as the lanes of one atomic instruction hit voxels at various depths.
+// address. Odd row and slice pitches (in words) spread voxels that differ only in y or z over
+// different banks, as the lanes of one atomic instruction hit voxels at various depths.
 constexpr int kATP = kAT + 2, kATRow = kATP + 1, kATSlice = kATP * kATRow + 1;
+constexpr int kATCells = kATP * kATSlice;
+constexpr int kATOwn = kAT * kAT * kAT / kMaxBlock; // tile voxels per thread
+static_assert(kAT * kAT * kAT % kMaxBlock == 0, "whole voxels per thread");
+constexpr int kAdj3Bits = 19;                             // a term's bound: 2^kAdj3Bits units
+constexpr int kAdj3Budget = (1 << (31 - kAdj3Bits)) - 16; // terms per voxel between drains
 
 template <class SV, class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restrict__ vol,
@@ -353,8 +367,25 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
                                                      const float *__restrict__ M,
                                                      const float *__restrict__ parts, int nx,
                                                      int ny, int nz, int U, int V, int A) {
-    __shared__ unsigned long long tile[kATP * kATSlice];
+    __shared__ unsigned int tile[kATCells];
     const int tid = threadIdx.x;
+    // voxel tid + k kMaxBlock of the tile: its 64-bit sum so far
+    long long own[kATOwn];
+#pragma unroll
+    for (int k = 0; k < kATOwn; ++k) own[k] = 0;
+    auto cell_of = [](int j) {
+        return (j / (kAT * kAT) + 1) * kATSlice + ((j / kAT) % kAT + 1) * kATRow + j % kAT + 1;
+    };
+    // the tile's voxel sums -> registers, then the tile (border included) cleared
+    auto drain = [&]() {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kATOwn; ++k) own[k] += (int)tile[cell_of(tid + k * kMaxBlock)];
+        __syncthreads();
+        for (int i = tid; i < kATCells; i += kMaxBlock) tile[i] = 0u;
+        __syncthreads();
+    };
+    int bound = 0; // terms a voxel may have received since the last drain
     const int ntz = (nz + kAT - 1) / kAT;
     const int x0 = blockIdx.x * kAT, y0 = blockIdx.y * kAT, z0 = (blockIdx.z % ntz) * kAT;
     const size_t b = blockIdx.z / ntz;
@@ -362,8 +393,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
     float ymax = 0.0f;
     for (int i = tid & 63; i < kMaxParts; i += 64) ymax = fmaxf(ymax, parts[b * kMaxParts + i]);
     for (int off = 32; off > 0; off >>= 1) ymax = fmaxf(ymax, __shfl_xor(ymax, off, 64));
-    const float yscale = ymax > 0.0f ? 1073741824.0f / ymax : 0.0f;
-    for (int i = tid; i < kATP * kATSlice; i += kMaxBlock) tile[i] = 0ull;
+    const float yscale = ymax > 0.0f ? (float)(1 << kAdj3Bits) / ymax : 0.0f;
+    for (int i = tid; i < kATCells; i += kMaxBlock) tile[i] = 0u;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
     // centred padded coordinate p -> voxel index floor(p) + c
     const int cx = Wp / 2 - kPad, cy = Hp / 2 - kPad, cz = Dp / 2 - kPad;
@@ -387,6 +418,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
         const float Wh = fmaf(m[8], qx, fmaf(m[9], qy, fmaf(m[10], qz, m[11])));
         const float rw = __builtin_amdgcn_rcpf(Wh);
         float umin = Uh * rw, umax = umin, vmin = Vh * rw, vmax = vmin;
+        float uabs = fabsf(umin), vabs = fabsf(vmin), wmin = Wh;
         int ok = Wh > 0.0f ? 1 : 0;
 #pragma unroll
         for (int off = 1; off < 8; off <<= 1) {
@@ -395,10 +427,27 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
             vmin = fminf(vmin, __shfl_xor(vmin, off, 64));
             vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
             ok = min(ok, __shfl_xor(ok, off, 64));
+            uabs = fmaxf(uabs, __shfl_xor(uabs, off, 64));
+            vabs = fmaxf(vabs, __shfl_xor(vabs, off, 64));
+            wmin = fminf(wmin, __shfl_xor(wmin, off, 64));
         }
         // every lane group holds the same values; take lane 0's
         int iu0 = 0, iu1 = U - 1, iv0 = 0, iv1 = V - 1;
+        // this angle's terms per voxel, at most (no bound where a corner is not in front of
+        // the source); 2^30 stands for unbounded
+        int per_angle = 1 << 30;
         if (__builtin_amdgcn_readfirstlane(ok)) {
+            const float ua = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(uabs)));
+            const float va = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vabs)));
+            const float wm = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wmin)));
+            const float su = fabsf(m[0]) + fabsf(m[1]) + fabsf(m[2]);
+            const float sv = fabsf(m[4]) + fabsf(m[5]) + fabsf(m[6]);
+            const float sw = fabsf(m[8]) + fabsf(m[9]) + fabsf(m[10]);
+            // 1 % margin for the approximate reciprocals; the +2 covers the pixel count of an
+            // interval (floor + 1) and rays that FP32 positions put one pixel further
+            const float du = 2.02f * (su + ua * sw) / wm, dv = 2.02f * (sv + va * sw) / wm;
+            if (du < 1024.0f && dv < 1024.0f)
+                per_angle = 2 * ((int)du + 2) * ((int)dv + 2);
             const float u0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(umin)));
             const float u1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(umax)));
             const float v0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vmin)));
@@ -418,7 +467,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
         // size leaves holes past the rectangle's edge, which are skipped.
         const int nqu = (niu + kPh - 1) / kPh, nqv = (niv + kPh - 1) / kPh;
         const int nq = nqu * nqv, nc = kPh * kPh * nq;
-        for (int c = tid; c < nc; c += kMaxBlock) {
+        // passes of kMaxBlock candidates, the same for the whole workgroup; a pass adds at most
+        // two terms per candidate to a voxel, and the angle at most per_angle in all
+        int allowance = per_angle;
+        for (int cb = 0; cb < nc; cb += kMaxBlock) {
+            const int inc = min(2 * min(kMaxBlock, nc - cb), allowance);
+            allowance -= inc;
+            if (bound + inc > kAdj3Budget) {
+                drain();
+                bound = 0;
+            }
+            bound += inc;
+            const int c = cb + tid;
+            if (c >= nc) continue;
             const int ph = c / nq, rq = c - ph * nq;
             const int qv = rq / nqu, qu = rq - qv * nqu;
             const int lu = (ph % kPh) + kPh * qu, lv = (ph / kPh) + kPh * qv;
@@ -466,23 +527,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
                 const float y01 = x0w * fy, y00 = x0w - y01, y11 = x1 * fy, y10 = x1 - y11;
                 const float w[8] = {y00 - y00 * fz, y10 - y10 * fz, y01 - y01 * fz,
                                     y11 - y11 * fz, y00 * fz, y10 * fz, y01 * fz, y11 * fz};
-                unsigned long long *t = &tile[__mul24(lz, kATSlice) + __mul24(ly, kATRow) + lx];
+                unsigned int *t = &tile[__mul24(lz, kATSlice) + __mul24(ly, kATRow) + lx];
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
                     atomicAdd(t + (k >> 2) * kATSlice + ((k >> 1) & 1) * kATRow + (k & 1),
-                              (unsigned long long)(long long)round_i32(w[k]));
+                              (unsigned int)round_i32(w[k]));
             }
         }
     }
-    __syncthreads();
-    for (int i = tid; i < kAT * kAT * kAT; i += kMaxBlock) {
+    drain();
+#pragma unroll
+    for (int k = 0; k < kATOwn; ++k) {
+        const int i = tid + k * kMaxBlock;
         const int lx = i % kAT, ly = (i / kAT) % kAT, lz = i / (kAT * kAT);
         const int X = x0 + lx, Y = y0 + ly, Z = z0 + lz;
         if (X < nx && Y < ny && Z < nz) {
             const size_t o = ((b * nz + Z) * ny + Y) * (size_t)nx + X;
-            const long long q =
-                (long long)tile[(lz + 1) * kATSlice + (ly + 1) * kATRow + lx + 1];
-            const float add = (float)((double)q * ((double)ymax * 0x1p-30));
+            const float add = (float)((double)own[k] * ((double)ymax * 0x1p-19));
             vol[o] = from_f32<SV>(to_f32<SV>(vol[o]) + add);
         }
     }

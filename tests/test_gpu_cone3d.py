@@ -224,6 +224,30 @@ def test_c3_adjoint_blocks(oracle, big, gpu, a0, a1):
     check_close(got, ref, f"c3 adj3d (projections {a0}..{a1 - 1})")
 
 
+def test_c3_full_adjoint_dot_and_linearity(big, gpu):
+    """The adjoint over all 360 projections, where each tile's 32-bit sums are drained into
+    64-bit registers several times: the dot test against the forward, and linearity over four
+    blocks of 90 projections (each a call of its own, with its own fixed-point unit)."""
+    rec, geom, _ = big
+    g = torch.Generator(device="cpu").manual_seed(5)
+    x = torch.rand(256, 256, 256, generator=g).to(gpu)
+    y = (torch.rand(360, 384, 384, generator=g) - 0.3).to(gpu)
+    lhs = torch.vdot(cm.forward(x, geom).double().ravel(), y.double().ravel())
+    aty = cm.adjoint(y, geom)
+    rhs = torch.vdot(x.double().ravel(), aty.double().ravel())
+    r = abs(float(lhs - rhs)) / abs(float(lhs))
+    print(f"c3 dot test: {r:.3e}")
+    assert r <= 1e-5
+    parts = torch.zeros_like(aty)
+    for a0 in range(0, 360, 90):
+        yb = torch.zeros_like(y)
+        yb[a0:a0 + 90] = y[a0:a0 + 90]
+        cm.adjoint(yb, parts, geom)
+    r = float((parts.double() - aty.double()).norm() / aty.double().norm())
+    print(f"c3 adjoint linearity: {r:.3e}")
+    assert r <= 1e-6
+
+
 @pytest.mark.parametrize("prec,tol", [(torch.float16, 1e-3), (torch.bfloat16, 6e-3)])
 def test_c3_half_storage(oracle, big, gpu, prec, tol):
     """16-bit storage at the 256^3 workload: forward on four projections and the reference

@@ -172,6 +172,24 @@ def test_c2_dot_product(g2, gpu):
     assert r <= 1e-5
 
 
+def test_c2_adjoint_linearity_over_blocks(g2, gpu):
+    """The full adjoint (each workgroup's 32-bit tile sums drained into 64-bit registers several
+    times) equals the sum of the adjoints of four blocks of 180 angles, each a call of its own
+    with its own fixed-point unit, to the rounding of the terms."""
+    rec, geom, img = g2
+    g = torch.Generator(device="cpu").manual_seed(8)
+    y = (torch.rand(720, 1449, generator=g) - 0.3).to(gpu)
+    full = cm.adjoint(y, geom).double()
+    parts = torch.zeros(1024, 1024, device=gpu)
+    for a0 in range(0, 720, 180):
+        yb = torch.zeros_like(y)
+        yb[a0:a0 + 180] = y[a0:a0 + 180]
+        cm.adjoint(yb, parts, geom)
+    r = float((parts.double() - full).norm() / full.norm())
+    print(f"c2 adjoint linearity: {r:.3e}")
+    assert r <= 1e-6
+
+
 def test_total_samples_match_oracle(oracle, g2):
     rec, geom, img = g2
     _, n = oracle.fwd2d(rec, np.zeros((1024, 1024)), angles=(0, 720), count=True)
