@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "bp_tiles.hpp"
@@ -211,74 +212,227 @@ __device__ __forceinline__ float sample3d(rsrc_t R, float f, float m, float l, i
     return fmaf(fl, b1 - b0, b0);
 }
 
-// One thread per ray (iu, iv) of projection a. A wavefront covers 4 x 16 detector pixels, the
-// 16 along v, and reads the volume copy whose fastest axis is the one v follows (fast[a]), so
-// the wavefront's samples of one step lie in a few runs of consecutive voxels whatever the
-// projection angle. A workgroup holds four wavefronts side by side along u (16 x 16 pixels).
+// LDS-staged march. The rays of a workgroup (16 x 16 detector pixels of one projection) lie
+// inside the pyramid spanned by its four corner rays. The workgroup marches along the axis K
+// that its rays follow most closely, in slabs of kSlab slices of lower taps: per slab it stages
+// into LDS the box of voxels that the pyramid's samples there can tap (the corner rays'
+// crossings of the slab's two bounding planes, one voxel of margin for FP32 positions, and the
+// upper taps), read in 16-byte row pieces from the volume copy whose fastest axis P is not K,
+// and every ray takes its samples whose lower tap lies in the slab from there, in its own order
+// j = 0 .. n - 1, slab after slab: positions and trilinear weights are those of a plain per-ray
+// march, and the ray's sum adds one partial sum per slab (at most about 2 kSlab samples each).
+// A sample outside the box (not expected; a safety net), and workgroups whose rays do not all
+// advance the same way along K or whose K is the copy's fast axis, read global memory instead,
+// with the same arithmetic (the latter in blocks of kSumBlock samples).
+// A wavefront covers 4 x 16 detector pixels, the 16 along v, and the copy read is the one
+// whose fastest axis v follows (fast[a]), so that a wavefront's rows of a staged box are runs
+// of consecutive voxels.
 // sino[b][a][iv][iu] = sum_{i=0..nst} trilinear(vol, s + (tmin + i dt) d)  (forward_3d.hpp:87-93)
+constexpr int kSlab = 8;          // slices of lower taps per slab
+constexpr int kBoxCells = 6144;   // floats of one staged box (24 KiB)
+constexpr int kBoxQuads = 8;      // at most 8 x 4 cells per staged row
+
 template <class SI, class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__restrict__ pad,
-                                                     const FwdGeom *__restrict__ geo,
-                                                     const int *__restrict__ fast, int slot0,
-                                                     int slot1, int slot2, int ncopies, int nx,
-                                                     int ny, int nz, int U, int V, int A,
-                                                     typename SS::raw *__restrict__ sino) {
-    const int lane = threadIdx.x; // blockDim (64, 4); consecutive lanes step along v
-    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so hardware id
-    // i runs on XCD i % 8; logical tile (i % 8) * (n / 8) + i / 8 gives each XCD a contiguous
-    // run of tiles (neighbouring tiles read the same voxels, from that XCD's L2). The last
-    // n % 8 ids keep their own position.
+                                                          const FwdGeom *__restrict__ geo,
+                                                          const int *__restrict__ fast, int slot0,
+                                                          int slot1, int slot2, int ncopies,
+                                                          int nx, int ny, int nz, int U, int V,
+                                                          int A,
+                                                          typename SS::raw *__restrict__ sino) {
+    __shared__ float4 box4[kBoxCells / 4];
+    float *box = reinterpret_cast<float *>(box4);
+    __shared__ float cl[4][6]; // corner rays in padded index coordinates: point, direction
+    __shared__ int red[2][kMaxBlock / 64];
+    const int tid = threadIdx.x;
+    // XCD-aware order, as in k_fwd3d
     const unsigned gx = gridDim.x, gy = gridDim.y;
-    const unsigned n = gx * gy * gridDim.z;
+    const unsigned nwg = gx * gy * gridDim.z;
     const unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-    const unsigned n8 = n & ~7u;
+    const unsigned n8 = nwg & ~7u;
     const unsigned li = hw < n8 ? (hw & 7u) * (n8 >> 3) + (hw >> 3) : hw;
     const unsigned tx = li % gx, ty = (li / gx) % gy, tz = li / (gx * gy);
-    const int iu = (int)tx * 16 + threadIdx.y * 4 + (lane >> 4);
-    const int iv = (int)ty * 16 + (lane & 15);
+    const int iu0 = (int)tx * 16, iv0 = (int)ty * 16;
+    // a wavefront covers 4 x 16 pixels, consecutive lanes along v
+    const int iu = iu0 + (tid >> 4), iv = iv0 + (tid & 15);
+    const bool live = iu < U && iv < V;
     const int a = (int)tz % A;
     const size_t b = tz / A;
-    if (iu >= U || iv >= V) return;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
-    const March3 p = march_params(geo[a], nx, ny, nz, iu, iv, Wp, Hp, Dp);
-    // axes (fast, middle, slow): x fastest (x, y, z), y fastest (y, x, z), z fastest (z, x, y)
-    const int F = fast[a];
+    if (tid < 4) {
+        const FwdGeom &g = geo[a];
+        const int cu = (tid & 1) ? min(iu0 + 15, U - 1) : iu0;
+        const int cv = (tid & 2) ? min(iv0 + 15, V - 1) : iv0;
+        for (int d = 0; d < 3; ++d) {
+            const double q = g.org[d] + (double)cu * g.du[d] + (double)cv * g.dv[d];
+            cl[tid][d] = (float)(g.src[d] + (double)kPad);
+            cl[tid][3 + d] = (float)(q - g.src[d]);
+        }
+    }
+    March3 p{};
+    if (live) p = march_params(geo[a], nx, ny, nz, iu, iv, Wp, Hp, Dp);
+    // copy axes (fast, middle, slow): x fastest (x, y, z), y fastest (y, x, z), z fastest (z, x, y)
+    // wave-uniform (so that the buffer resource below is a scalar one)
+    const int F = __builtin_amdgcn_readfirstlane(fast[a]);
+    const int M = F == 0 ? 1 : 0, L = F == 2 ? 1 : 2;
     const int slot = F == 0 ? slot0 : F == 1 ? slot1 : slot2;
-    const float mf = F == 0 ? p.m[0] : F == 1 ? p.m[1] : p.m[2];
-    const float mm = F == 0 ? p.m[1] : p.m[0];
-    const float ml = F == 2 ? p.m[1] : p.m[2];
-    const float tf = F == 0 ? p.st[0] : F == 1 ? p.st[1] : p.st[2];
-    const float tm = F == 0 ? p.st[1] : p.st[0];
-    const float tl = F == 2 ? p.st[1] : p.st[2];
-    const int Ef = F == 0 ? Wp : F == 1 ? Hp : Dp;
-    const int Em = F == 0 ? Hp : Wp;
-    const int El = F == 2 ? Hp : Dp;
+    // component d of a three-vector without indexing a local array (which would live in
+    // scratch memory)
+    auto pick = [](auto v0, auto v1, auto v2, int d) { return d == 0 ? v0 : d == 1 ? v1 : v2; };
+    const int Ef = pick(Wp, Hp, Dp, F), Em = pick(Wp, Hp, Dp, M), El = pick(Wp, Hp, Dp, L);
     const uint32_t one = (uint32_t)Wp * Hp * Dp;
     const rsrc_t R = make_rsrc(pad + (b * ncopies + slot) * one, bytes_of<SI>(one));
-    const int es = (int)sizeof(typename SI::raw);
+    constexpr int es = (int)sizeof(typename SI::raw);
     const int sM = Ef * es, sL = Ef * Em * es, sML = sL + sM;
     const int cofs = ((El / 2) * Em + Em / 2) * Ef + Ef / 2;
-    float acc = 0.0f, jf = p.j0;
+    const float mf = pick(p.m[0], p.m[1], p.m[2], F), mm = pick(p.m[0], p.m[1], p.m[2], M),
+                ml = pick(p.m[0], p.m[1], p.m[2], L);
+    const float tf = pick(p.st[0], p.st[1], p.st[2], F), tm = pick(p.st[0], p.st[1], p.st[2], M),
+                tl = pick(p.st[0], p.st[1], p.st[2], L);
+    __syncthreads(); // corner rays
+    // march axis K: the largest component of the corner rays' summed direction
+    float ds[3];
+    for (int d = 0; d < 3; ++d) ds[d] = cl[0][3 + d] + cl[1][3 + d] + cl[2][3 + d] + cl[3][3 + d];
+    const int K = fabsf(ds[0]) >= fabsf(ds[1]) ? (fabsf(ds[0]) >= fabsf(ds[2]) ? 0 : 2)
+                                               : (fabsf(ds[1]) >= fabsf(ds[2]) ? 1 : 2);
+    const float sgn = ds[K] > 0.0f ? 1.0f : -1.0f;
+    bool slab_ok = K != F;
+    for (int c = 0; c < 4; ++c) slab_ok = slab_ok && cl[c][3 + K] * sgn > 0.0f;
+    const int Q = 3 - K - F; // the third axis (when K != F)
+    // padded index = centred coordinate + E / 2 on each axis
+    const int cK = pick(Wp, Hp, Dp, K) / 2, cP = Ef / 2, cQ = pick(Wp, Hp, Dp, Q) / 2;
+    const float mK = pick(p.m[0], p.m[1], p.m[2], K), tK = pick(p.st[0], p.st[1], p.st[2], K);
+    const float mP = mf, tP = tf;
+    const float mQ = pick(p.m[0], p.m[1], p.m[2], Q), tQ = pick(p.st[0], p.st[1], p.st[2], Q);
+    // element strides of K and Q in the copy; and whether K is the copy's middle axis (then a
+    // step in K is the trilinear m step, else the l step)
+    const bool kmid = K == M;
+    const int sK = kmid ? Ef : Ef * Em, sQ = kmid ? Ef * Em : Ef;
+
+    float acc = 0.0f;
     int j = 0;
-    while (j < p.n) {
-        const int jend = min(j + kSumBlock, p.n);
-        float s0 = 0.0f, s1 = 0.0f;
-        for (; j + 1 < jend; j += 2, jf += 2.0f) {
-            const float jg = jf + 1.0f;
-            s0 += sample3d<SI>(R, fmaf(jf, tf, mf), fmaf(jf, tm, mm), fmaf(jf, tl, ml), Ef, Em,
-                               cofs, sM, sL, sML);
-            s1 += sample3d<SI>(R, fmaf(jg, tf, mf), fmaf(jg, tm, mm), fmaf(jg, tl, ml), Ef, Em,
-                               cofs, sM, sL, sML);
+    const int n = p.n;
+    float jf = p.j0;
+    // sample j at the forward's position, from global memory
+    auto global_sample = [&]() {
+        return sample3d<SI>(R, fmaf(jf, tf, mf), fmaf(jf, tm, mm), fmaf(jf, tl, ml), Ef, Em,
+                            cofs, sM, sL, sML);
+    };
+    if (!__builtin_amdgcn_readfirstlane((int)slab_ok)) {
+        while (j < n) {
+            const int jend = min(j + kSumBlock, n);
+            float s0 = 0.0f;
+            for (; j < jend; ++j, jf += 1.0f) s0 += global_sample();
+            acc += s0;
         }
-        if (j < jend) {
-            s0 += sample3d<SI>(R, fmaf(jf, tf, mf), fmaf(jf, tm, mm), fmaf(jf, tl, ml), Ef, Em,
-                               cofs, sM, sL, sML);
-            ++j;
-            jf += 1.0f;
+    } else {
+        // lower-tap K range of the workgroup's samples
+        int klo = 0x7fffffff, khi = -0x7fffffff - 1;
+        if (n > 0) {
+            const int k0 = floor_i32(fmaf(p.j0, tK, mK)) + cK;
+            const int k1 = floor_i32(fmaf(p.j0 + (float)(n - 1), tK, mK)) + cK;
+            klo = min(k0, k1);
+            khi = max(k0, k1);
         }
-        acc += s0 + s1;
+        for (int off = 32; off > 0; off >>= 1) {
+            klo = min(klo, __shfl_xor(klo, off, 64));
+            khi = max(khi, __shfl_xor(khi, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = klo;
+            red[1][tid >> 6] = khi;
+        }
+        __syncthreads();
+        klo = min(min(red[0][0], red[0][1]), min(red[0][2], red[0][3]));
+        khi = max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3]));
+        const int nslab = khi >= klo ? (khi - klo) / kSlab + 1 : 0;
+        const float rtK = n > 0 ? 1.0f / tK : 0.0f;
+        // lanes 8 g + c + 4 e of every wavefront project corner ray c onto plane e of a slab
+        const int cc = tid & 3, ce = (tid >> 2) & 1;
+        const float cKo = cl[cc][K], cKr = __builtin_amdgcn_rcpf(cl[cc][3 + K]);
+        const float cPo = cl[cc][F], cPd = cl[cc][3 + F], cQo = cl[cc][Q], cQd = cl[cc][3 + Q];
+        for (int si = 0; si < nslab; ++si) {
+            // lower taps K in [D0, D0 + kSlab - 1]; slabs follow the rays' direction along K
+            const int D0 = sgn > 0.0f ? klo + si * kSlab : khi - kSlab + 1 - si * kSlab;
+            // the box: the corner rays where they cross K = D0 and K = D0 + kSlab
+            const float t = ((float)(D0 + ce * kSlab) - cKo) * cKr;
+            float pmin = fmaf(t, cPd, cPo), qmin = fmaf(t, cQd, cQo);
+            float pmax = pmin, qmax = qmin;
+#pragma unroll
+            for (int off = 1; off < 8; off <<= 1) {
+                pmin = fminf(pmin, __shfl_xor(pmin, off, 64));
+                pmax = fmaxf(pmax, __shfl_xor(pmax, off, 64));
+                qmin = fminf(qmin, __shfl_xor(qmin, off, 64));
+                qmax = fmaxf(qmax, __shfl_xor(qmax, off, 64));
+            }
+            const int P0 = __builtin_amdgcn_readfirstlane((int)floorf(pmin) - 1);
+            const int Q0 = __builtin_amdgcn_readfirstlane((int)floorf(qmin) - 1);
+            const int WP = __builtin_amdgcn_readfirstlane((int)floorf(pmax) + 2 - P0 + 1);
+            const int WQ = __builtin_amdgcn_readfirstlane((int)floorf(qmax) + 2 - Q0 + 1);
+            const int WPp = (WP + 3) & ~3;
+            const int S = WQ * WPp; // floats per staged slice
+            const bool staged = WPp <= 4 * kBoxQuads && WQ <= kBoxCells / ((kSlab + 1) * WPp);
+            if (staged) {
+                // rows (slice kk, row qq) of WP cells, 16 bytes per thread: 32 rows a pass
+                const int rows = (kSlab + 1) * WQ, quad = tid & (kBoxQuads - 1);
+                int r = tid >> 3, kk = r / WQ, qq = r - kk * WQ;
+                for (; r < rows; r += kMaxBlock / kBoxQuads) {
+                    if (4 * quad < WPp) {
+                        const int e = (D0 + kk) * sK + (Q0 + qq) * sQ + P0 + 4 * quad;
+                        const Quad v = buf_quad_at<SI>(R, e * es);
+                        *reinterpret_cast<float4 *>(&box[(kk * WQ + qq) * WPp + 4 * quad]) =
+                            make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+                    }
+                    qq += kMaxBlock / kBoxQuads;
+                    while (qq >= WQ) {
+                        qq -= WQ;
+                        ++kk;
+                    }
+                }
+            }
+            __syncthreads(); // the box is staged
+            // this ray's samples in the slab: j .. je - 1, where its position along K crosses
+            // the slab's far plane. A sample that the rounding of this estimate puts in the
+            // wrong slab lies outside the box and takes the global path.
+            const float bK = (float)((sgn > 0.0f ? D0 + kSlab : D0) - cK);
+            const int je = j < n ? min(max((int)ceilf(fmaf(bK - mK, rtK, -p.j0)), j), n) : j;
+            const int kofs = cK - D0, pofs = cP - P0, qofs = cQ - Q0;
+            // KM: K is the copy's middle axis (a step in K is the trilinear m step)
+            auto sample_slab = [&](auto km) {
+                constexpr bool KM = decltype(km)::value;
+                const int mstep = KM ? S : WPp, lstep = KM ? WPp : S;
+                float part = 0.0f;
+                for (; j < je; ++j, jf += 1.0f) {
+                    const float pK = fmaf(jf, tK, mK);
+                    const float pf = fmaf(jf, tP, mP), pq = fmaf(jf, tQ, mQ);
+                    const int kk = floor_i32(pK) + kofs;
+                    const int ip = floor_i32(pf) + pofs, iq = floor_i32(pq) + qofs;
+                    if (staged && (unsigned)kk < (unsigned)kSlab &&
+                        (unsigned)ip <= (unsigned)(WP - 2) && (unsigned)iq <= (unsigned)(WQ - 2)) {
+                        const float *c = &box[__mul24(kk, S) + __mul24(iq, WPp) + ip];
+                        const float ff = __builtin_amdgcn_fractf(pf);
+                        const float fm = __builtin_amdgcn_fractf(KM ? pK : pq);
+                        const float fl = __builtin_amdgcn_fractf(KM ? pq : pK);
+                        const float a00 = fmaf(ff, c[1] - c[0], c[0]);
+                        const float a10 = fmaf(ff, c[mstep + 1] - c[mstep], c[mstep]);
+                        const float a01 = fmaf(ff, c[lstep + 1] - c[lstep], c[lstep]);
+                        const float a11 = fmaf(ff, c[mstep + lstep + 1] - c[mstep + lstep],
+                                               c[mstep + lstep]);
+                        const float b0 = fmaf(fm, a10 - a00, a00);
+                        const float b1 = fmaf(fm, a11 - a01, a01);
+                        part += fmaf(fl, b1 - b0, b0);
+                    } else {
+                        part += global_sample();
+                    }
+                }
+                acc += part;
+            };
+            if (kmid) sample_slab(std::true_type{});
+            else sample_slab(std::false_type{});
+            __syncthreads(); // the box is read before the next slab is staged
+        }
     }
-    sino[((b * A + a) * V + iv) * (size_t)U + iu] = from_f32<SS>(acc);
+    if (live) sino[((b * A + a) * V + iv) * (size_t)U + iu] = from_f32<SS>(acc);
 }
 
 // ---------------------------------------------------------------- adjoint -----------------
@@ -1096,8 +1250,8 @@ CMI_API cmi_status cmi_forward_3d(const cmi_plan3d *plan, const void *volume, in
                                    copies + plan->copy_slot[2] * one, nx, ny, nz, Wp, Hp, Dp);
             }
         }
-        dim3 fb(64, 4), fg((unsigned)((U + 15) / 16), (unsigned)((V + 15) / 16),
-                           (unsigned)(A * batch));
+        dim3 fb(kMaxBlock), fg((unsigned)((U + 15) / 16), (unsigned)((V + 15) / 16),
+                               (unsigned)(A * batch));
         hipLaunchKernelGGL((k_fwd3d<SI, SS>), fg, fb, 0, s, (const RI *)scratch, plan->d_fgeo,
                            plan->d_fast, plan->copy_slot[0], plan->copy_slot[1],
                            plan->copy_slot[2], plan->ncopies, nx, ny, nz, U, V, A,

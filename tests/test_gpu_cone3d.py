@@ -66,6 +66,19 @@ def test_small_forward(oracle, small, gpu):
     check_close(host(cm.forward(dev(vol, gpu), geom)), ref, "small fwd3d")
 
 
+@pytest.mark.parametrize("spacing", [0.5, 0.12])
+def test_forward_wide_beams(oracle, gpu, spacing):
+    """Voxels smaller than the pixels' footprint: a 16 x 16-pixel workgroup's beam spans more
+    voxels than one staged box holds at oblique angles (voxel 0.5) or everywhere (0.12), and
+    those slabs take the global path. The forward still matches the oracle."""
+    rec = cone(48, 40, 36, 12, 120.0, 180.0, vol_spacing=(spacing,) * 3)
+    geom = as_cone_geometry(rec)
+    vol = phantom.ellipsoids_3d(48, seed=3, dtype=np.float64)
+    ref = oracle.fwd3d(rec, vol)
+    out = host(cm.forward(dev(vol, gpu), geom))
+    check_close(out, ref, f"wide-beam forward (voxel {spacing})")
+
+
 def test_small_backward_reference(oracle, small, gpu):
     rec, geom = small
     y = np.random.default_rng(2).random((24, 40, 48))
