@@ -363,7 +363,22 @@ def cpu_baseline():
         "cpu_model": r["cpu_model"],
         "fan1024": f, "fan64": r["fan64"], "cone256": r["cone256"],
         "single_thread": r["single_thread"],
+        "full_runs": full_runs(),
     }
+
+
+def full_runs():
+    """The committed full-operator runs of the same port (`oracle/cpu_baseline.py --full-cone`,
+    1 + 5 full 256^3 cone-beam runs and one full 1024^2 run on one thread; too long for every
+    bench run), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "cpu_baseline_full.json")) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return {"source": "profiles/cpu_baseline_full.json (committed measurement on the GPU host)",
+            "threads": d.get("threads"), "cone256": d.get("cone256"),
+            "single_thread_fan1024": d.get("single_thread", {}).get("fan1024")}
 
 
 def roofline(kernel, op_ms, tap_bytes):
@@ -550,6 +565,8 @@ def main():
             "pair": {"ms_p50": float(np.percentile(step_ms, 50)),
                      "ms_p99": float(np.percentile(step_ms, 99))},
             "total_samples": info["total_samples"],
+            # plan on a fresh geometry plus its first forward and backprojection
+            "cold_ms": (cold or {}).get("fan2d_1024", {}).get("plan_and_first_pair_ms"),
         },
         "roofline": roof,
         "cpu_baseline": cpu,

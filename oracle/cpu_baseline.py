@@ -56,14 +56,20 @@ def _stats(ts):
             "runs": int(ts.size)}
 
 
-def _time(fn, warm, runs):
+def _progress(msg):
+    print(f"cpu_baseline: {msg}", file=sys.stderr, flush=True)
+
+
+def _time(fn, warm, runs, what=None):
     for _ in range(warm):
         fn()
     out = []
-    for _ in range(runs):
+    for k in range(runs):
         t0 = time.perf_counter()
         fn()
         out.append(time.perf_counter() - t0)
+        if what and out[-1] > 5.0:
+            _progress(f"{what} run {k + 1}/{runs}: {out[-1]:.1f} s")
     return out
 
 
@@ -72,8 +78,9 @@ def fan(n, D, A, dso, dsd, warm, runs, seed=0):
     rec = _rec2d(n, D, A, dso, dsd)
     img = phantom.ellipses_2d(n, seed=seed).astype(np.float32)
     sino = oracle.fwd2d_f32(rec, img)
-    f = _stats(_time(lambda: oracle.fwd2d_f32(rec, img), warm, runs))
-    b = _stats(_time(lambda: oracle.bp2d_f32(rec, sino), warm, runs))
+    f = _stats(_time(lambda: oracle.fwd2d_f32(rec, img), warm, runs, f"fan{n} forward"))
+    b = _stats(_time(lambda: oracle.bp2d_f32(rec, sino), warm, runs, f"fan{n} backprojection"))
+    _progress(f"fan{n} done ({oracle.num_threads()} threads)")
     return {"fwd": f, "bwd": b, "pairs_per_s": 1e3 / (f["ms_mean"] + b["ms_mean"])}
 
 
@@ -86,8 +93,10 @@ def cone(warm, runs, fwd_angles=None, bp_slices=None, seed=0):
     fa = 360 if fwd_angles is None else int(fwd_angles)
     bz = 256 if bp_slices is None else int(bp_slices)
     sino = oracle.fwd3d_f32(rec, vol, angles=(0, fa))
-    f = _time(lambda: oracle.fwd3d_f32(rec, vol, angles=(0, fa)), warm, runs)
-    b = _time(lambda: oracle.bp3d_f32(rec, sino, slabs=(0, bz)), warm, runs)
+    f = _time(lambda: oracle.fwd3d_f32(rec, vol, angles=(0, fa)), warm, runs, "cone256 forward")
+    b = _time(lambda: oracle.bp3d_f32(rec, sino, slabs=(0, bz)), warm, runs,
+              "cone256 backprojection")
+    _progress("cone256 done")
     f = _stats(np.asarray(f) * 360 / fa)
     b = _stats(np.asarray(b) * 256 / bz)
     return {"fwd": f, "bwd": b, "pairs_per_s": 1e3 / (f["ms_mean"] + b["ms_mean"]),
