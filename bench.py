@@ -315,7 +315,8 @@ def kernels_only(torch, cm, phantom, np, which, reps):
         geom, x, sino, vol = build_case(torch, cm, phantom, np)
         ops = [("forward", lambda: cm.forward(x, sino, geom)),
                ("backprojection", lambda: cm.backward(sino, vol, geom)),
-               ("adjoint", lambda: cm.adjoint(sino, vol, geom))]
+               ("adjoint", lambda: cm.adjoint(sino, vol, geom)),
+               ("backprojection transpose", lambda: cm.backward_transpose(x, geom, sino))]
     else:
         geom = cm.ConeGeometry(DSD=768.0, DSO=512.0,
                                angles=np.linspace(0, 2 * np.pi, 360, endpoint=False),
@@ -511,6 +512,17 @@ def main():
     torch.cuda.synchronize()
     adj_ms = ea.elapsed_time(eb) / n_adj
     log(f"bench: adjoint {adj_ms:.3f} ms")
+    # transpose of the backprojection (TorchBackprojection's gradient), timed separately
+    bt_sino = torch.empty_like(sino)
+    cm.backward_transpose(vol, geom, bt_sino)
+    torch.cuda.synchronize()
+    ea.record(stream)
+    for _ in range(n_adj):
+        cm.backward_transpose(vol, geom, bt_sino)
+    eb.record(stream)
+    torch.cuda.synchronize()
+    bt_ms = ea.elapsed_time(eb) / n_adj
+    log(f"bench: backprojection transpose {bt_ms:.3f} ms")
 
     extra = None
     if not args.no_extra:
@@ -562,6 +574,7 @@ def main():
                      "ms_p99": float(np.percentile(bwd_ms, 99)), "op_per_s": 1e3 / bwd_ms.mean(),
                      "tap_equiv_GB_per_s": bwd_gbs, "tap_bytes": bwd_bytes},
             "adj2d": {"ms_mean": adj_ms, "op_per_s": 1e3 / adj_ms},
+            "bp2d_transpose": {"ms_mean": bt_ms, "op_per_s": 1e3 / bt_ms},
             "pair": {"ms_p50": float(np.percentile(step_ms, 50)),
                      "ms_p99": float(np.percentile(step_ms, 99))},
             "total_samples": info["total_samples"],

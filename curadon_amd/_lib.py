@@ -129,6 +129,7 @@ SIGNATURES = {
     "cmi_backward_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_backward_2d_weighted": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_adjoint_2d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_backward_2d_transpose": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_plan3d_create": (_i32, [_i32, _i32, _i32, C.POINTER(Cone3DDesc), C.POINTER(_vp)]),
     "cmi_plan3d_create_vec": (_i32, [_i32, _i32, _i32, _u64p, _dp, _dp, _u64p, _dp, _u64,
                                      C.POINTER(_vp)]),
@@ -415,6 +416,19 @@ def backward_2d(volume, sinogram, plan, weighting="reference"):
         return
     check(_lib.cmi_backward_2d_weighted(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(),
                                         sc, b, w, stream_of(volume)))
+
+
+def backward_2d_transpose(volume, sinogram, plan, weighting="reference"):
+    """sinogram := B^T volume (overwrites): the transpose of backward_2d with the same
+    weighting, the gradient of the voxel-driven backprojection."""
+    w = {"reference": BP2D_REFERENCE, "fbp": BP2D_FBP}.get(weighting)
+    if w is None:
+        raise ValueError(f"unknown weighting {weighting!r}")
+    vc, sc, b = _check_pair_2d(volume, sinogram, plan, "backward_2d_transpose")
+    if b == 0:
+        return
+    check(_lib.cmi_backward_2d_transpose(plan.handle, sinogram.data_ptr(), sc,
+                                         volume.data_ptr(), vc, b, w, stream_of(volume)))
 
 
 def adjoint_2d(volume, sinogram, plan):

@@ -62,6 +62,10 @@ struct cmi_plan2d {
     float4 *d_ev = nullptr;
     float4 *d_ns = nullptr;
     bool quarter = false; // backprojection by quarter turns applies (k_bp2d_qt)
+    // bounds on the backprojection's per-projection weight over the image (w / du, w^2 / du
+    // for circular plans, with the view's 1 / |u| for plans from views); infinite when a
+    // source lies inside the image's bounding box
+    double bt_bound[2] = {INFINITY, INFINITY};
 };
 
 namespace {
@@ -988,6 +992,178 @@ __global__ __launch_bounds__(kMaxBlock) void k_fill_u64(unsigned long long *__re
     if (i < n) p[i] = 0ull;
 }
 
+// ------------------------------------------------ transpose of the backprojection -------
+
+// Bᵀ: the transpose of the voxel-driven backprojection (k_bp2d / k_bp2d_vec), the gradient of
+// TorchBackprojection. The backprojection gives pixel p, per projection a, wt_a(p) times the
+// linear interpolation of row a at u_a(p) (cells floor(u) and floor(u) + 1, weights 1 - al and
+// al; cells outside the row read zero), times 1/du (circular plans; the view's 1/|u| is part of
+// wt for plans from views). So sino[a][c] = sum_p (those weights) v[p], with k_bp2d's FP32
+// positions and weights. A workgroup owns a 16 x 16 pixel tile and a group of projections; per
+// chunk of kTC projections it projects the tile's corners (lanes 4j .. 4j + 3 for projection
+// j), adds each pixel's two terms into that projection's kTS-cell segment of LDS, and adds the
+// segments to a 64-bit image of the sinogram with integer global atomics. Terms are fixed
+// point in units of max|v| * bound / 2^30, where `bound` is the plan's bound on wt (times
+// 1/du) over the image and all projections, so a term fits in 31 bits and the 64-bit sums are
+// exact: the result does not depend on the order of the additions. A projection whose tile
+// footprint is wider than a segment adds its terms to the image directly.
+constexpr int kTC = 16; // projections per chunk
+constexpr int kTS = 64; // cells per projection's segment
+static_assert(4 * kTC == 64, "lanes 4 j .. 4 j + 3 project the corners of projection j");
+
+// Largest |v| of each image of a batch, in kAdjParts partial maxima per image.
+template <class SV>
+__global__ __launch_bounds__(kMaxBlock) void k_absmax_img2d(const typename SV::raw *__restrict__ v,
+                                                            size_t n, float *__restrict__ parts) {
+    __shared__ float red[kMaxBlock / 64];
+    const size_t b = blockIdx.y;
+    v += b * n;
+    parts += b * kAdjParts;
+    float m = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * kMaxBlock + threadIdx.x; i < n;
+         i += (size_t)kAdjParts * kMaxBlock)
+        m = fmaxf(m, fabsf(to_f32<SV>(v[i])));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kMaxBlock / 64; ++w) m = fmaxf(m, red[w]);
+        parts[blockIdx.x] = m;
+    }
+}
+
+// One projection's detector coordinate u (lower-tap cell floor(u), clamped to [-2, D] as the
+// backprojector clamps it) and weight wt of the pixel at world (x, y): k_bp2d's arithmetic for
+// circular plans (tab = meta, wt without the 1/du), k_bp2d_vec's for plans from views
+// (tab = three float4 per projection, wt including 1/|u|).
+template <bool VIEWS, int WPOW>
+__device__ __forceinline__ void bpT_geom(const float4 *__restrict__ tab, int a, float x, float y,
+                                         float px, float py, float c0, float hi, float &u,
+                                         float &wt) {
+    if (VIEWS) {
+        const float4 m0 = tab[3 * a], m1 = tab[3 * a + 1];
+        const float idu = tab[3 * a + 2].x;
+        const float w = m0.w * __builtin_amdgcn_rcpf(fmaf(m0.x, x, fmaf(m0.y, y, m0.z)));
+        const float eta = fmaf(m1.x, x, fmaf(m1.y, y, m1.z));
+        u = __builtin_amdgcn_fmed3f(fmaf(w, eta, m1.w), -2.0f, hi);
+        wt = (WPOW == 2 ? w * w : w) * idu;
+    } else {
+        const float4 m = tab[a];
+        const float depth = fmaf(m.x, x, fmaf(-m.y, y, m.w)); // xi + DSO
+        const float eta = fmaf(m.y, px, m.x * py);
+        const float w = m.z * __builtin_amdgcn_rcpf(depth);
+        u = __builtin_amdgcn_fmed3f(fmaf(w, eta, c0), -2.0f, hi);
+        wt = WPOW == 2 ? w * w : w;
+    }
+}
+
+template <class SV, int WPOW, bool VIEWS>
+__global__ __launch_bounds__(kMaxBlock) void k_bpT2d(unsigned long long *__restrict__ accs,
+                                                     const typename SV::raw *__restrict__ vol,
+                                                     const float4 *__restrict__ tab,
+                                                     const float *__restrict__ parts, float bound,
+                                                     int nx, int ny, int D, int A, int G, float sx,
+                                                     float sy, float ox, float oy, float inv_du,
+                                                     float half_d) {
+    __shared__ long long seg[kTC * kTS];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int g = blockIdx.z % G;
+    const size_t b = blockIdx.z / G;
+    const int a0 = (int)((int64_t)g * A / G), a1 = (int)((int64_t)(g + 1) * A / G);
+    const int ix0 = blockIdx.x * 16, iy0 = blockIdx.y * 16;
+    const int ix = ix0 + (tid & 15), iy = iy0 + (tid >> 4);
+    const bool live = ix < nx && iy < ny;
+    const float hi = (float)D, c0 = half_d - 0.5f;
+    // fixed-point scale: a term v wt (1/du) of magnitude up to max|v| bound maps to 2^30
+    // (in FP64: max|v| bound may lie far from 1)
+    const float vmax = adj2d_ymax(parts + b * kAdjParts);
+    const double unit_inv = vmax > 0.0f ? 0x1p30 / ((double)vmax * (double)bound) : 0.0;
+    auto world = [&](int i, int n, float s, float o) {
+        return ((float)i - (float)n / 2.0f) * s + o + 0.5f;
+    };
+    const float x = world(ix, nx, sx, ox), y = world(iy, ny, sy, oy);
+    const float px = x * inv_du, py = y * inv_du;
+    const float v =
+        live ? (float)((double)to_f32<SV>(vol[(b * ny + iy) * (size_t)nx + ix]) * unit_inv) : 0.0f;
+    // the tile corner (pixel centre) this lane projects
+    const int cxi = (lane & 1) ? min(ix0 + 15, nx - 1) : ix0;
+    const int cyi = (lane & 2) ? min(iy0 + 15, ny - 1) : iy0;
+    const float qx = world(cxi, nx, sx, ox), qy = world(cyi, ny, sy, oy);
+    const float qpx = qx * inv_du, qpy = qy * inv_du;
+    unsigned long long *out = accs + b * (size_t)A * D;
+    for (int c = a0; c < a1; c += kTC) {
+        // footprint of projection c + (lane >> 2): its first segment cell, or -2^30 when the
+        // footprint is wider than a segment
+        int s0v;
+        {
+            const int a = min(c + (lane >> 2), A - 1);
+            float u, wt;
+            bpT_geom<VIEWS, WPOW>(tab, a, qx, qy, qpx, qpy, c0, hi, u, wt);
+            float lo = u, up = u;
+            lo = fminf(lo, __shfl_xor(lo, 1, 64));
+            lo = fminf(lo, __shfl_xor(lo, 2, 64));
+            up = fmaxf(up, __shfl_xor(up, 1, 64));
+            up = fmaxf(up, __shfl_xor(up, 2, 64));
+            const int first = floor_i32(lo) - 1, last = floor_i32(up) + 2;
+            s0v = last - first + 1 <= kTS ? first : -(1 << 30);
+        }
+        for (int i = tid; i < kTC * kTS; i += kMaxBlock) seg[i] = 0;
+        __syncthreads();
+        if (live && v != 0.0f) {
+#pragma unroll 4
+            for (int j = 0; j < kTC; ++j) {
+                const int a = c + j;
+                if (a >= a1) break;
+                float u, wt;
+                bpT_geom<VIEWS, WPOW>(tab, a, x, y, px, py, c0, hi, u, wt);
+                const int i = floor_i32(u);
+                const float al = __builtin_amdgcn_fractf(u);
+                const float t = v * wt * (VIEWS ? 1.0f : inv_du);
+                const float t1 = t * al, t0 = t - t1;
+                const int s0 = __builtin_amdgcn_readlane(s0v, 4 * j);
+                const int k = i - s0;
+                if ((unsigned)k <= (unsigned)(kTS - 2)) {
+                    atomicAdd((unsigned long long *)&seg[j * kTS + k],
+                              (unsigned long long)(long long)round_i32(t0));
+                    atomicAdd((unsigned long long *)&seg[j * kTS + k + 1],
+                              (unsigned long long)(long long)round_i32(t1));
+                } else { // a wide footprint: straight into the image
+                    unsigned long long *row = out + (size_t)a * D;
+                    if (i >= 0 && i < D)
+                        atomicAdd(row + i, (unsigned long long)(long long)round_i32(t0));
+                    if (i + 1 >= 0 && i + 1 < D)
+                        atomicAdd(row + i + 1, (unsigned long long)(long long)round_i32(t1));
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < kTC * kTS; e += kMaxBlock) {
+            const int j = e / kTS, k = e % kTS;
+            const int s0 = __shfl(s0v, 4 * j, 64);
+            const int cell = s0 + k;
+            const long long q = seg[e];
+            if (c + j < a1 && q != 0 && cell >= 0 && cell < D)
+                atomicAdd(out + (size_t)(c + j) * D + cell, (unsigned long long)q);
+        }
+        __syncthreads(); // the segments are read before the next chunk clears them
+    }
+}
+
+// sino := accs * max|v| bound / 2^30.
+template <class SS>
+__global__ __launch_bounds__(kMaxBlock) void k_bpT2d_finish(typename SS::raw *__restrict__ sino,
+                                                            const unsigned long long *__restrict__ accs,
+                                                            const float *__restrict__ parts,
+                                                            float bound, size_t rays) {
+    const size_t b = blockIdx.y;
+    const float vmax = adj2d_ymax(parts + b * kAdjParts);
+    const double unit = (double)vmax * (double)bound * 0x1p-30;
+    const size_t j = (size_t)blockIdx.x * kMaxBlock + threadIdx.x;
+    if (j >= rays) return;
+    const size_t i = b * rays + j;
+    sino[i] = from_f32<SS>((float)((double)(long long)accs[i] * unit));
+}
+
 // ---------------------------------------------------------------- dispatch -----------------
 
 template <class F>
@@ -1108,6 +1284,45 @@ cmi_status finish_plan2d(std::unique_ptr<cmi_plan2d> p, const std::vector<AngleG
                          const std::vector<float4> &meta, const std::vector<float4> &bpv,
                          cmi_plan2d **plan) {
     const int64_t A = p->A, D = p->g.D;
+    // the backprojection's weight bounds: depth is affine over the image, so its minimum over
+    // the pixel centres is at a corner; 1 % of margin for the kernels' FP32 arithmetic
+    {
+        const double nx = (double)p->g.nx, ny = (double)p->g.ny;
+        const double xs[2] = {(0.0 - nx / 2.0) * p->g.sx + p->g.ox + 0.5,
+                              (nx - 1.0 - nx / 2.0) * p->g.sx + p->g.ox + 0.5};
+        const double ys[2] = {(0.0 - ny / 2.0) * p->g.sy + p->g.oy + 0.5,
+                              (ny - 1.0 - ny / 2.0) * p->g.sy + p->g.oy + 0.5};
+        double b1 = 0.0, b2 = 0.0;
+        bool ok = true;
+        for (int64_t a = 0; a < A && ok; ++a) {
+            for (int c = 0; c < 4; ++c) {
+                const double x = xs[c & 1], y = ys[c >> 1];
+                double depth, L, scale;
+                if (!meta.empty()) {
+                    const float4 m = meta[a];
+                    depth = (double)m.x * x - (double)m.y * y + (double)m.w;
+                    L = m.z;
+                    scale = 1.0 / p->g.du;
+                } else {
+                    const float4 m0 = bpv[3 * a];
+                    depth = (double)m0.x * x + (double)m0.y * y + (double)m0.z;
+                    L = m0.w;
+                    scale = bpv[3 * a + 2].x;
+                }
+                if (!(depth > 0.0)) {
+                    ok = false;
+                    break;
+                }
+                const double w = std::fabs(L) / depth;
+                b1 = std::max(b1, w * scale);
+                b2 = std::max(b2, w * w * scale);
+            }
+        }
+        if (ok && b1 > 0.0 && std::isfinite(b2)) {
+            p->bt_bound[0] = 1.01 * b1;
+            p->bt_bound[1] = 1.01 * b2;
+        }
+    }
     std::vector<float> ageo(6 * A);
     for (int64_t a = 0; a < A; ++a)
         for (int d = 0; d < 2; ++d) {
@@ -1502,6 +1717,65 @@ CMI_API cmi_status cmi_backward_2d(const cmi_plan2d *plan, void *volume, int vol
                                    void *stream) {
     return cmi_backward_2d_weighted(plan, volume, vol_dtype, sinogram, sino_dtype, batch,
                                     CMI_BP2D_REFERENCE, stream);
+}
+
+CMI_API cmi_status cmi_backward_2d_transpose(const cmi_plan2d *plan, void *sinogram,
+                                             int sino_dtype, const void *volume, int vol_dtype,
+                                             uint64_t batch, int weighting, void *stream) {
+    clear_error();
+    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch,
+                       "backward_2d_transpose"));
+    if (weighting != CMI_BP2D_REFERENCE && weighting != CMI_BP2D_FBP)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "backward_2d_transpose: unknown weighting %d",
+                    weighting);
+    const double bound = plan->bt_bound[weighting == CMI_BP2D_FBP ? 1 : 0];
+    if (!std::isfinite(bound))
+        return fail(CMI_ERR_NOT_SUPPORTED,
+                    "backward_2d_transpose: a source lies inside the image's bounding box, so "
+                    "the backprojection's weight is unbounded");
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(plan->device));
+    const hipStream_t s = (hipStream_t)stream;
+    const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
+    const size_t img = (size_t)nx * ny, rays = (size_t)A * D;
+    // scratch: the 64-bit sinogram sums, then kAdjParts partial maxima per image
+    void *scratch = nullptr;
+    CMI_HIP_TRY(hipMallocAsync(&scratch, rays * batch * 8 + kAdjParts * batch * sizeof(float), s));
+    unsigned long long *accs = (unsigned long long *)scratch;
+    float *parts = (float *)(accs + rays * batch);
+    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
+        using SV = decltype(sv);
+        using SS = decltype(ss);
+        const size_t n = rays * batch;
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + kMaxBlock - 1) / kMaxBlock)),
+                           dim3(kMaxBlock), 0, s, accs, n);
+        hipLaunchKernelGGL((k_absmax_img2d<SV>), dim3(kAdjParts, (unsigned)batch),
+                           dim3(kMaxBlock), 0, s, (const typename SV::raw *)volume, img, parts);
+        const unsigned tx = (unsigned)((nx + 15) / 16), ty = (unsigned)((ny + 15) / 16);
+        const uint64_t tiles = (uint64_t)tx * ty * batch;
+        // projection groups: enough workgroups to fill the GPU many times over
+        const int G = (int)std::max<uint64_t>(
+            1, std::min<uint64_t>((uint64_t)A, (8192 + tiles - 1) / tiles));
+        const bool views = plan->d_bpv != nullptr;
+        const bool fbp = weighting == CMI_BP2D_FBP;
+        auto kern = views ? (fbp ? k_bpT2d<SV, 2, true> : k_bpT2d<SV, 1, true>)
+                          : (fbp ? k_bpT2d<SV, 2, false> : k_bpT2d<SV, 1, false>);
+        hipLaunchKernelGGL(kern, dim3(tx, ty, (unsigned)(G * batch)), dim3(kMaxBlock), 0, s,
+                           accs, (const typename SV::raw *)volume,
+                           views ? plan->d_bpv : plan->d_meta, parts, (float)bound, nx, ny, D,
+                           A, G, (float)plan->g.sx, (float)plan->g.sy, (float)plan->g.ox,
+                           (float)plan->g.oy, (float)(1.0 / plan->g.du), (float)D / 2.0f);
+        hipLaunchKernelGGL((k_bpT2d_finish<SS>),
+                           dim3((unsigned)((rays + kMaxBlock - 1) / kMaxBlock), (unsigned)batch),
+                           dim3(kMaxBlock), 0, s, (typename SS::raw *)sinogram, accs, parts,
+                           (float)bound, rays);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+    const hipError_t fe = hipFreeAsync(scratch, s);
+    if (st != CMI_OK) return st;
+    CMI_HIP_TRY(fe);
+    return CMI_OK;
 }
 
 CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_dtype,

@@ -5,9 +5,10 @@ adjoint ``A^T grad`` (exact transpose of the ray march), not the voxel-driven ba
 the reference pairs it with (diffable.py:22-24).
 
 ``TorchBackprojection.apply(sinogram, geom)`` returns the voxel-driven backprojection
-``B sinogram``. Its gradient is ``A grad``, as the reference intends (diffable.py:35-38).
-``B`` is not the transpose of ``A`` (different weighting and interpolation), so this gradient
-is an approximation; use ``TorchAdjoint`` for an exact pair.
+``B sinogram``. For fan beams its gradient is the exact transpose ``B^T grad`` (the
+backprojector's own detector coordinates and weights); for cone beams it is ``A grad``, as the
+reference intends (diffable.py:35-38), an approximation since ``B`` is not the transpose of
+``A``.
 
 Each function returns exactly one gradient per input.
 """
@@ -16,8 +17,9 @@ from __future__ import annotations
 import torch
 from torch.autograd import Function
 
-from .backward import adjoint, backward
+from .backward import adjoint, backward, backward_transpose
 from .forward import forward
+from .geometry import FanGeometry
 
 
 class TorchForward(Function):
@@ -52,4 +54,6 @@ class TorchBackprojection(Function):
 
     @staticmethod
     def backward(ctx, grad_vol):
+        if isinstance(ctx.geom, FanGeometry):
+            return backward_transpose(grad_vol.contiguous(), ctx.geom), None
         return forward(grad_vol.contiguous(), ctx.geom), None

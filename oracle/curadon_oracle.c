@@ -354,6 +354,57 @@ void orc_bp2d(const orc_fan2d *g, const double *sino, double *img, int64_t y0, i
     orc_bp2d_w(g, sino, img, y0, y1, 1);
 }
 
+/* Transpose of orc_bp2d_w: sino[a][c] += sum over pixels of the weight with which pixel p takes
+ * cell c of projection a, times img[p]. The backprojection gives pixel p, per projection, wt
+ * times the linear interpolation of the row at u (orc_lerp1: cells floor(u - 1/2) and the next,
+ * weights 1 - al and al; cells outside the row contribute nothing), times 1/du (circular
+ * plans; the view's 1/|u| for plans from views); the transpose adds img[p] times those weights
+ * to the two cells. Projections a0 .. a1 - 1. */
+void orc_bp2d_T_w(const orc_fan2d *g, const double *img, double *sino, int64_t a0, int64_t a1,
+                  int wpow) {
+    const int64_t D = g->det_count;
+    const double half_d = (double)D / 2.0, inv_du = 1.0 / g->du;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t a = a0; a < a1; ++a) {
+        double *row = sino + a * D;
+        for (int64_t iy = 0; iy < g->ny; ++iy) {
+            for (int64_t ix = 0; ix < g->nx; ++ix) {
+                const double x = ((double)ix - (double)g->nx / 2.0) * g->sx + g->ox + 0.5;
+                const double y = ((double)iy - (double)g->ny / 2.0) * g->sy + g->oy + 0.5;
+                double u, wt;
+                if (g->vecs) { /* bp2d_vec's view formula */
+                    const double *v = g->vecs + 6 * a;
+                    const double ul = sqrt(v[4] * v[4] + v[5] * v[5]);
+                    const double e0 = v[4] / ul, e1 = v[5] / ul;
+                    double n0 = -e1, n1 = e0;
+                    const double cs0 = v[2] - v[0], cs1 = v[3] - v[1];
+                    if (n0 * cs0 + n1 * cs1 < 0.0) {
+                        n0 = -n0;
+                        n1 = -n1;
+                    }
+                    const double L = n0 * cs0 + n1 * cs1;
+                    const double px = x - v[0], py = y - v[1];
+                    const double w = L / (n0 * px + n1 * py);
+                    u = w * (e0 * px + e1 * py) / ul - (e0 * cs0 + e1 * cs1) / ul + half_d;
+                    wt = (wpow == 2 ? w * w : w) / ul;
+                } else { /* orc_bp2d_w's circular formula */
+                    const double sn = -sin(g->angles[a]), cs = cos(g->angles[a]);
+                    const double xi = sn * x - cs * y;
+                    const double yi = cs * (x * inv_du) + sn * (y * inv_du);
+                    const double w = g->dsd[a] / (xi + g->dso[a]);
+                    u = w * yi + half_d;
+                    wt = (wpow == 2 ? w * w : w) * inv_du;
+                }
+                const double ub = u - 0.5, fl = floor(ub), al = ub - fl;
+                const int64_t i = (int64_t)fl;
+                const double vw = img[iy * g->nx + ix] * wt;
+                if (i >= 0 && i < D) row[i] += (1.0 - al) * vw;
+                if (i + 1 >= 0 && i + 1 < D) row[i + 1] += al * vw;
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------ FP32 ports for the CPU baseline ----- */
 
 static inline float fetch2f(const float *img, int64_t nx, int64_t ny, int64_t i, int64_t j) {

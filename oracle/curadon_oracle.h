@@ -75,6 +75,10 @@ void orc_bp2d(const orc_fan2d *g, const double *sino, double *img, int64_t y0, i
  * (DSD / (DSO + xi))^2. */
 void orc_bp2d_w(const orc_fan2d *g, const double *sino, double *img, int64_t y0, int64_t y1,
                 int wpow);
+/* Transpose of orc_bp2d_w (sino[a][c] += the backprojection's weights times img), projections
+ * a0 .. a1 - 1; the gradient of the voxel-driven backprojection. */
+void orc_bp2d_T_w(const orc_fan2d *g, const double *img, double *sino, int64_t a0, int64_t a1,
+                  int wpow);
 
 /* FP32 ports of the two hot loops, same arithmetic in single precision, OpenMP over
  * angles / image rows. Used only to time the reference math on host cores. */

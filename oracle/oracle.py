@@ -64,6 +64,7 @@ def lib():
             "orc_adj2d": (None, [F2, _dp, _dp, _i64, _i64]),
             "orc_bp2d": (None, [F2, _dp, _dp, _i64, _i64]),
             "orc_bp2d_w": (None, [F2, _dp, _dp, _i64, _i64, C.c_int]),
+            "orc_bp2d_T_w": (None, [F2, _dp, _dp, _i64, _i64, C.c_int]),
             "orc_bp3d_fdk": (None, [F3, _dp, _dp, _i64, _i64]),
             "orc_fwd3d_f32": (None, [F3, _fp, _fp, _i64, _i64]),
             "orc_bp3d_f32": (None, [F3, _fp, _fp, _i64, _i64]),
@@ -219,6 +220,18 @@ def bp2d(geom, sino, rows=None, img=None, wpow=1):
     out = np.zeros((f.ny, f.nx)) if img is None else _c64(img).copy()
     y0, y1 = (0, f.ny) if rows is None else rows
     lib().orc_bp2d_w(C.byref(f.s), _d(sino), _d(out), y0, y1, int(wpow))
+    return out
+
+
+def bp2d_T(geom, img, angles=None, wpow=1):
+    """FP64 transpose of ``bp2d`` (the backprojection's weights applied to an image, into a
+    sinogram); ``angles`` = (a0, a1) restricts it to those projections (the others stay 0)."""
+    f = Fan(geom)
+    img = _c64(img)
+    assert img.shape == (f.ny, f.nx)
+    out = np.zeros((f.A, f.D))
+    a0, a1 = (0, f.A) if angles is None else angles
+    lib().orc_bp2d_T_w(C.byref(f.s), _d(img), _d(out), a0, a1, int(wpow))
     return out
 
 

@@ -484,3 +484,42 @@ def test_exact_geometry_adjoint_is_transpose(oracle):
     y = np.random.default_rng(7).random((5, 9))
     np.testing.assert_allclose(oracle.adj2d(rec, y).ravel(), M.T @ y.ravel(), rtol=1e-12,
                                atol=1e-12)
+
+
+# ------------------------------------------------- transpose of the backprojection --------
+
+def _bp_matrix(oracle, g, wpow):
+    """The backprojection as a matrix (image pixels x sinogram cells), one cell at a time."""
+    A, D = g.nangles, int(g.det_shape)
+    ny, nx = (int(v) for v in g.vol_shape)
+    M = np.zeros((ny * nx, A * D))
+    for j in range(A * D):
+        e = np.zeros(A * D)
+        e[j] = 1.0
+        M[:, j] = oracle.bp2d(g, e.reshape(A, D), wpow=wpow).ravel()
+    return M
+
+
+@pytest.mark.parametrize("wpow", [1, 2])
+@pytest.mark.parametrize("views", [False, True])
+def test_backprojection_transpose_is_assembled_matrix_transpose(oracle, wpow, views):
+    g = fan(8, 9, 6, 16.0, 24.0, du=1.2, spacing=(1.1, 0.9), offset=(0.3, -0.2))
+    if views:
+        g = _with_views(g, _general_views(21, 6, 9, dso=16.0, dsd=24.0))
+    M = _bp_matrix(oracle, g, wpow)
+    img = rng(22).random((8, 8))
+    bt = oracle.bp2d_T(g, img, wpow=wpow)
+    np.testing.assert_allclose(bt.ravel(), M.T @ img.ravel(), rtol=1e-12, atol=1e-12)
+    # a projection subset is the same rows of the full transpose
+    part = oracle.bp2d_T(g, img, angles=(2, 4), wpow=wpow)
+    np.testing.assert_allclose(part[2:4], bt[2:4], rtol=0, atol=0)
+    assert not part[:2].any() and not part[4:].any()
+
+
+def test_backprojection_transpose_dot_product_c1(oracle):
+    g = c1()
+    s = rng(23).random((360, 91))
+    v = rng(24).random((64, 64))
+    lhs = np.vdot(oracle.bp2d(g, s), v)
+    rhs = np.vdot(s, oracle.bp2d_T(g, v))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
