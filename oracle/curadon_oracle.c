@@ -832,6 +832,90 @@ void orc_bp3d_fdk(const orc_cone3d *g, const double *sino, double *vol, int64_t 
     bp3d_matched(g, sino, vol, z0, z1, 1);
 }
 
+/* Transpose of the 3D backprojections: sino[a][v][u] += the weight with which each voxel
+ * takes panel cell (u, v) of projection a, times vol[voxel]. The backprojection gives a voxel,
+ * per projection, wt times the bilinear interpolation of the panel at (u, v) (panel_bilerp:
+ * cells floor(u - 1/2), floor(v - 1/2) and the next ones, cells outside the panel read zero);
+ * (u, v) and wt are orc_bp3d's (mode 0; plans from views project as matched, as there),
+ * bp3d_matched's (mode 1) or its FDK form (mode 2), restated below per voxel. Projections
+ * a0 .. a1 - 1. */
+void orc_bp3d_T(const orc_cone3d *g, int mode, const double *vol, double *sino, int64_t a0,
+                int64_t a1) {
+    const size_t n3 = (size_t)g->nangles * 3;
+    const int ref = mode == 0 && !g->vecs;
+    double *O = NULL, *dx = NULL, *dy = NULL, *dz = NULL, s[3] = {0, 0, 0};
+    double *org = NULL, *du = NULL, *dv = NULL, *src = NULL;
+    if (ref) {
+        O = malloc(n3 * sizeof(double));
+        dx = malloc(n3 * sizeof(double));
+        dy = malloc(n3 * sizeof(double));
+        dz = malloc(n3 * sizeof(double));
+        orc_plan3d_bp(g, O, dx, dy, dz, s);
+    } else {
+        org = malloc(n3 * sizeof(double));
+        du = malloc(n3 * sizeof(double));
+        dv = malloc(n3 * sizeof(double));
+        src = malloc(n3 * sizeof(double));
+        orc_plan3d_fwd(g, org, du, dv, src);
+    }
+    const double plane = g->dso - g->dsd;
+    const double hu = (double)(g->nu / 2), hv = (double)(g->nv / 2);
+    const int64_t UV = g->nu * g->nv;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t a = a0; a < a1; ++a) {
+        double *panel = sino + a * UV;
+        for (int64_t iz = 0; iz < g->nz; ++iz)
+            for (int64_t iy = 0; iy < g->ny; ++iy)
+                for (int64_t ix = 0; ix < g->nx; ++ix) {
+                    double u, v, wt = 1.0;
+                    if (ref) { /* orc_bp3d */
+                        double dir[3];
+                        for (int d = 0; d < 3; ++d)
+                            dir[d] = O[3 * a + d] + (double)ix * dx[3 * a + d] +
+                                     (double)iy * dy[3 * a + d] + (double)iz * dz[3 * a + d] - s[d];
+                        const double t = (plane - s[2]) / dir[2];
+                        u = dir[0] * t + s[0] + hu;
+                        v = dir[1] * t + s[1] + hv;
+                    } else { /* bp3d_matched */
+                        double c0[3], w[3], x[3];
+                        const double P[3] = {(double)ix, (double)iy, (double)iz};
+                        for (int d = 0; d < 3; ++d) {
+                            c0[d] = org[3 * a + d] - src[3 * a + d];
+                            w[d] = P[d] - src[3 * a + d];
+                        }
+                        solve3(c0, du + 3 * a, dv + 3 * a, w, x);
+                        if (!(x[0] > 0.0)) continue;
+                        u = x[1] / x[0] + 0.5;
+                        v = x[2] / x[0] + 0.5;
+                        if (mode == 2) {
+                            const double r = g->dso / (x[0] * g->dsd);
+                            wt = r * r;
+                        }
+                    }
+                    const double val = wt * vol[(iz * g->ny + iy) * g->nx + ix];
+                    const double ub = u - 0.5, vb = v - 0.5;
+                    const double fu = floor(ub), fv = floor(vb);
+                    const double al = ub - fu, be = vb - fv;
+                    const int64_t i = (int64_t)fu, j = (int64_t)fv;
+                    const double w4[4] = {(1.0 - be) * (1.0 - al), (1.0 - be) * al,
+                                          be * (1.0 - al), be * al};
+                    for (int k = 0; k < 4; ++k) {
+                        const int64_t ci = i + (k & 1), cj = j + (k >> 1);
+                        if (ci >= 0 && ci < g->nu && cj >= 0 && cj < g->nv)
+                            panel[cj * g->nu + ci] += w4[k] * val;
+                    }
+                }
+    }
+    free(O);
+    free(dx);
+    free(dy);
+    free(dz);
+    free(org);
+    free(du);
+    free(dv);
+    free(src);
+}
+
 /* ------------------------------------------------ 3D FP32 ports (CPU baseline only) --- */
 
 static inline float fetch3f(const float *v, int64_t nx, int64_t ny, int64_t nz, int64_t i,

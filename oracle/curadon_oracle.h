@@ -163,6 +163,11 @@ void orc_bp3d_matched(const orc_cone3d *g, const double *sino, double *vol, int6
  * parameter on its ray (t = 1 at the panel), i.e. (DSO / depth)^2 for an untilted panel. */
 void orc_bp3d_fdk(const orc_cone3d *g, const double *sino, double *vol, int64_t z0, int64_t z1);
 
+/* Transpose of the 3D backprojection of `mode` (0 reference, 1 matched, 2 FDK):
+ * sino[a] += B_a^T vol for projections a0 .. a1 - 1. */
+void orc_bp3d_T(const orc_cone3d *g, int mode, const double *vol, double *sino, int64_t a0,
+                int64_t a1);
+
 /* FP32 ports of orc_fwd3d and orc_bp3d (reference mode), used only to time a CPU port of the
  * reference arithmetic. */
 void orc_fwd3d_f32(const orc_cone3d *g, const float *vol, float *sino, int64_t a0, int64_t a1);

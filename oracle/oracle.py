@@ -66,6 +66,7 @@ def lib():
             "orc_bp2d_w": (None, [F2, _dp, _dp, _i64, _i64, C.c_int]),
             "orc_bp2d_T_w": (None, [F2, _dp, _dp, _i64, _i64, C.c_int]),
             "orc_bp3d_fdk": (None, [F3, _dp, _dp, _i64, _i64]),
+            "orc_bp3d_T": (None, [F3, C.c_int, _dp, _dp, _i64, _i64]),
             "orc_fwd3d_f32": (None, [F3, _fp, _fp, _i64, _i64]),
             "orc_bp3d_f32": (None, [F3, _fp, _fp, _i64, _i64]),
             "orc_fwd2d_f32": (None, [F2, _fp, _fp, _i64, _i64]),
@@ -389,6 +390,20 @@ def bp3d(geom, sino, slabs=None, vol=None, matched=False, fdk=False):
     z0, z1 = (0, c.shape_vol[0]) if slabs is None else slabs
     fn = lib().orc_bp3d_fdk if fdk else (lib().orc_bp3d_matched if matched else lib().orc_bp3d)
     fn(C.byref(c.s), _d(sino), _d(out), z0, z1)
+    return out
+
+
+def bp3d_T(geom, vol, angles=None, matched=False, fdk=False):
+    """FP64 transpose of ``bp3d`` (same modes) applied to a volume, into a projection stack;
+    ``angles`` = (a0, a1) restricts it to those projections (the others stay 0)."""
+    c = Cone(geom)
+    if fdk and c.vecs is not None:
+        raise ValueError("FDK weighting needs a circular orbit, not views")
+    vol = _c64(vol)
+    assert vol.shape == c.shape_vol
+    out = np.zeros(c.shape_sino)
+    a0, a1 = (0, c.A) if angles is None else angles
+    lib().orc_bp3d_T(C.byref(c.s), 2 if fdk else (1 if matched else 0), _d(vol), _d(out), a0, a1)
     return out
 
 

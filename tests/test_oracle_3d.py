@@ -350,3 +350,51 @@ def test_fp32_ports_follow_the_fp64_oracle(oracle):
     ref = oracle.bp3d(g, y)
     got = oracle.bp3d_f32(g, y)
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+
+
+# ------------------------------------------------- transpose of the backprojection --------
+
+def _bp_matrix3(oracle, g, **mode):
+    """A 3D backprojection as a matrix (voxels x panel cells), one cell at a time."""
+    A = g.angles.shape[1]
+    V, U = (int(v) for v in g.det_shape)
+    cols = []
+    for j in range(A * V * U):
+        e = np.zeros(A * V * U)
+        e[j] = 1.0
+        cols.append(oracle.bp3d(g, e.reshape(A, V, U), **mode).ravel())
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("mode", [{}, {"matched": True}, {"fdk": True}])
+def test_backprojection_transpose_is_assembled_matrix_transpose(oracle, mode):
+    g = cone(5, 6, 4, 3, 12.0, 20.0, du=(1.1, 0.9), nz=4, ny=3, vol_spacing=(1.0, 1.2, 0.9),
+             vol_offset=(0.3, -0.2, 0.1), det_offset=(0.4, -0.3))
+    M = _bp_matrix3(oracle, g, **mode)
+    x = rng(31).random((4, 3, 5))
+    np.testing.assert_allclose(oracle.bp3d_T(g, x, **mode).ravel(), M.T @ x.ravel(),
+                               rtol=1e-12, atol=1e-12)
+    # a projection subset is the same projections of the full transpose
+    full = oracle.bp3d_T(g, x, **mode)
+    part = oracle.bp3d_T(g, x, angles=(1, 2), **mode)
+    np.testing.assert_array_equal(part[1], full[1])
+    assert not part[0].any() and not part[2].any()
+
+
+def test_backprojection_transpose_with_views(oracle):
+    g = cone(4, 5, 4, 3, 12.0, 20.0, nz=3, ny=4)
+    gv = _with_views(g, _helix(3, 4, 12.0, 3.0, 8.0, 5, 4, du=1.1, dv=0.9, tilt=0.1))
+    M = _bp_matrix3(oracle, gv)
+    x = rng(32).random((3, 4, 4))
+    np.testing.assert_allclose(oracle.bp3d_T(gv, x).ravel(), M.T @ x.ravel(), rtol=1e-12,
+                               atol=1e-12)
+
+
+def test_backprojection_transpose_dot_product_3d(oracle):
+    g = cone(16, 20, 18, 12, 40.0, 60.0)
+    s = rng(33).random((12, 18, 20))
+    x = rng(34).random((16, 16, 16))
+    for mode in ({}, {"matched": True}, {"fdk": True}):
+        lhs = np.vdot(oracle.bp3d(g, s, **mode), x)
+        rhs = np.vdot(s, oracle.bp3d_T(g, x, **mode))
+        assert abs(lhs - rhs) / abs(lhs) < 1e-12
