@@ -138,6 +138,7 @@ SIGNATURES = {
     "cmi_forward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
     "cmi_backward_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_adjoint_3d": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _vp]),
+    "cmi_backward_3d_transpose": (_i32, [_vp, _vp, _i32, _vp, _i32, _u64, _i32, _vp]),
     "cmi_fan2d_bp_table": (_i32, [_u64, _dp, _dp, _dp, C.POINTER(C.c_float)]),
     "cmi_cone3d_bp_matrices": (_i32, [C.POINTER(Cone3DDesc), _i32, C.POINTER(C.c_float),
                                       C.POINTER(C.c_int)]),
@@ -617,6 +618,18 @@ def backward_3d_plan(volume, sinogram, plan, mode="reference"):
     if b:
         check(_lib.cmi_backward_3d(plan.handle, volume.data_ptr(), vc, sinogram.data_ptr(), sc,
                                    b, m, stream_of(volume)))
+
+
+def backward_3d_transpose(volume, sinogram, plan, mode="reference"):
+    """sinogram := B^T volume (overwrites): the transpose of backward_3d_plan in the same mode,
+    the gradient of the voxel-driven backprojection."""
+    m = _BP3D_MODES.get(mode)
+    if m is None:
+        raise ValueError(f"unknown backprojection mode {mode!r}")
+    vc, sc, b = _check_pair_3d(volume, sinogram, plan, "backward_3d_transpose")
+    if b:
+        check(_lib.cmi_backward_3d_transpose(plan.handle, sinogram.data_ptr(), sc,
+                                             volume.data_ptr(), vc, b, m, stream_of(volume)))
 
 
 def adjoint_3d_plan(volume, sinogram, plan):

@@ -59,15 +59,24 @@ def adjoint(sinogram, volume=None, geom=None, *, volume_out=None):
 
 
 def backward_transpose(volume, geom, sinogram=None, *, mode="reference"):
-    """Transpose of ``backward`` for fan beams: ``sinogram := B^T volume`` (overwritten, or
-    allocated when None) with the same weighting ``mode`` ('reference' or 'fbp'); the gradient
-    of the voxel-driven backprojection. Returns the sinogram."""
-    if not isinstance(geom, FanGeometry):
-        raise NotImplementedError("backward_transpose: fan-beam geometries only")
-    if mode not in ("reference", "fbp"):
-        raise ValueError(f"fan-beam backward_transpose: unknown mode {mode!r}")
-    if sinogram is None:
-        sinogram = torch.empty(tuple(volume.shape[:-2]) + tuple(geom.sinogram_shape()),
-                               dtype=_C.dtype_of_code(geom.plan.det_prec), device=volume.device)
-    _C.backward_2d_transpose(volume, sinogram, geom.plan, weighting=mode)
-    return sinogram
+    """Transpose of ``backward``: ``sinogram := B^T volume`` (overwritten, or allocated when
+    None) with the same ``mode`` (fan beams 'reference' or 'fbp'; cone beams 'reference',
+    'matched' or 'fdk'); the gradient of the voxel-driven backprojection. Returns the
+    sinogram."""
+    if isinstance(geom, FanGeometry):
+        if mode not in ("reference", "fbp"):
+            raise ValueError(f"fan-beam backward_transpose: unknown mode {mode!r}")
+        if sinogram is None:
+            sinogram = torch.empty(tuple(volume.shape[:-2]) + tuple(geom.sinogram_shape()),
+                                   dtype=_C.dtype_of_code(geom.plan.det_prec),
+                                   device=volume.device)
+        _C.backward_2d_transpose(volume, sinogram, geom.plan, weighting=mode)
+        return sinogram
+    if isinstance(geom, ConeGeometry):
+        if sinogram is None:
+            sinogram = torch.empty(tuple(volume.shape[:-3]) + tuple(geom.sinogram_shape()),
+                                   dtype=_C.dtype_of_code(geom.plan.det_prec),
+                                   device=volume.device)
+        _C.backward_3d_transpose(volume, sinogram, geom.plan, mode=mode)
+        return sinogram
+    raise ValueError("Unknown geometry type")

@@ -48,6 +48,10 @@ struct cmi_plan3d {
     float *d_mfdk = nullptr;
     bool ycols[3] = {false, false, false}; // per cmi_bp3d_mode: U and W rows free of y
     bool views = false; // built from ASTRA-like views: reference mode = matched, no FDK mode
+    // per cmi_bp3d_mode: smallest W of the mode's matrices over the volume's corner voxels and
+    // all angles (W is affine in the voxel, so this bounds it over the volume); the transpose
+    // of the backprojection needs it positive
+    double wmin[3] = {0.0, 0.0, 0.0};
 };
 
 namespace {
@@ -769,6 +773,248 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp3d_y(typename SV::raw *__restri
     }
 }
 
+// ------------------------------------------------ transpose of the backprojection -------
+
+// Bᵀ: the transpose of the voxel-driven backprojection (k_bp3d, k_bp3d_y), the gradient of
+// TorchBackprojection. The backprojection adds to voxel h, per projection a, the bilinear
+// interpolation of panel a at (u, v) = (U/W, V/W) from the mode's matrix rows (cells floor(u)
+// + {0, 1} and floor(v) + {0, 1}, weights 1 - fu, fu and 1 - fv, fv; cells outside the panel
+// read zero), times 1/W^2 for FDK. So sino[a][j][i] = sum_h (those weights) vol[h].
+// A workgroup owns a 16 x 16 pixel tile of one projection and gathers it. The voxels that reach
+// the tile are those with (u, v) in [iu0 - 1, iu0 + 16) x [iv0 - 1, iv0 + 16): a pyramid from
+// the source, bounded by the four lines through the source and that rectangle's corners. The
+// workgroup marches through the volume along the axis K that the tile's central ray follows
+// most closely, in chunks of kBTSlab slices. In each slice the pyramid's section is the
+// quadrilateral where the corner lines cross it; its bounding box (one voxel of margin on each
+// side, clipped to the volume) is walked by the workgroup's threads, x fastest (y when K is x,
+// from a y-fastest copy of the volume), so that a wavefront reads runs of consecutive voxels.
+// Every voxel computes its own (u, v) and adds its taps inside the tile to the tile's cells in
+// LDS. The corner lines need the rays of the pyramid to cross the slices (each corner line's
+// direction has a K component of a quarter of its length, all of one sign); otherwise the
+// whole slice is walked.
+// Fixed point: terms are rounded to units of max|vol| wbound / 2^kBTBits, where wbound >= the
+// weight (1, or 1 / min W^2 for FDK, W > 0 over the volume), so |term| <= 2^kBTBits and a
+// 32-bit LDS cell holds the sum of up to 2^(31 - kBTBits) - 1 terms. A voxel adds at most one
+// term to a cell, so the workgroup moves the cells into 64-bit registers (one cell per thread)
+// and clears them before its voxel visits since the last such drain could pass that count.
+// Integer sums are exact: the result does not depend on the order of the additions. The tile is
+// written once, in the sinogram's storage type.
+constexpr int kBT = 16;                                        // panel tile edge, pixels
+constexpr int kBTP = kBT + 2;                                  // LDS cells per row and rows
+constexpr int kBTSlab = 32;                                    // slices per chunk
+constexpr int kBTBits = 18;                                    // a term's bound: 2^kBTBits
+constexpr int kBTBudget = (1 << (31 - kBTBits)) - 1;           // visits between drains
+constexpr int kBTPass = (kBTBudget / kMaxBlock) * kMaxBlock;   // visits per pass
+static_assert(kBT * kBT == kMaxBlock, "one tile cell per thread");
+static_assert(kBTSlab <= 64, "one lane per slice of a chunk");
+
+template <class SV, class SS, bool FDK>
+__global__ __launch_bounds__(kMaxBlock) void k_bpT3d(typename SS::raw *__restrict__ sino,
+                                                     const typename SV::raw *__restrict__ vol,
+                                                     const typename SV::raw *__restrict__ ycopy,
+                                                     const float *__restrict__ M,
+                                                     const float *__restrict__ parts,
+                                                     float wbound, int nx, int ny, int nz, int U,
+                                                     int V, int A) {
+    __shared__ unsigned int cells[kBTP * kBTP]; // pixel iu0 - 1 + i, iv0 - 1 + j at j kBTP + i
+    __shared__ int4 slab[2][kBTSlab];           // per slice: F range, G range (inclusive)
+    __shared__ int dims[2][2];                  // per chunk: box extents along F and G
+    __shared__ float coef[4][4];                // per corner line: F = a + b hK, G = c + d hK
+    __shared__ int whole;                       // 1: walk whole slices
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int a = (int)(blockIdx.x % (unsigned)A), ti = (int)(blockIdx.x / (unsigned)A);
+    const int tv = (V + kBT - 1) / kBT;
+    const int tj = (int)(blockIdx.y % (unsigned)tv);
+    const size_t b = blockIdx.y / (unsigned)tv;
+    const int iu0 = ti * kBT, iv0 = tj * kBT;
+    const float *m = M + 12 * a;
+    const float cx = (float)((nx - 1) / 2.0), cy = (float)((ny - 1) / 2.0),
+                cz = (float)((nz - 1) / 2.0);
+    // fixed-point scale (FP64: max|vol| wbound may lie far from 1)
+    float vmax = 0.0f;
+    for (int i = lane; i < kMaxParts; i += 64) vmax = fmaxf(vmax, parts[b * kMaxParts + i]);
+    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+    const double unit_inv = vmax > 0.0f ? 0x1p18 / ((double)vmax * (double)wbound) : 0.0;
+    static_assert(kBTBits == 18, "unit_inv uses 2^18");
+    // march axis: the largest component of the central ray's direction, the cross product of
+    // the (U - uc W) and (V - vc W) rows' voxel parts
+    int K;
+    {
+        const float uc = (float)iu0 + 0.5f * (kBT - 1), vc = (float)iv0 + 0.5f * (kBT - 1);
+        float e1[3], e2[3];
+        for (int k = 0; k < 3; ++k) {
+            e1[k] = fmaf(-uc, m[8 + k], m[k]);
+            e2[k] = fmaf(-vc, m[8 + k], m[4 + k]);
+        }
+        const float d0 = fabsf(e1[1] * e2[2] - e1[2] * e2[1]);
+        const float d1 = fabsf(e1[2] * e2[0] - e1[0] * e2[2]);
+        const float d2 = fabsf(e1[0] * e2[1] - e1[1] * e2[0]);
+        K = d0 >= d1 && d0 >= d2 ? 0 : (d1 >= d2 ? 1 : 2);
+        K = __builtin_amdgcn_readfirstlane(K);
+    }
+    const int F = K == 0 ? 1 : 0, G = K == 2 ? 1 : 2;
+    // corner lines (FP64), lanes 0..3 of the first wavefront
+    if (tid < 64) {
+        const int c = lane & 3;
+        const double uc = (double)(iu0 - 1 + ((c & 1) ? kBT + 1 : 0));
+        const double vc = (double)(iv0 - 1 + ((c & 2) ? kBT + 1 : 0));
+        const double a0 = (double)m[0] - uc * (double)m[8], a1 = (double)m[1] - uc * (double)m[9],
+                     a2 = (double)m[2] - uc * (double)m[10], a3 = (double)m[3] - uc * (double)m[11];
+        const double q0 = (double)m[4] - vc * (double)m[8], q1 = (double)m[5] - vc * (double)m[9],
+                     q2 = (double)m[6] - vc * (double)m[10], q3 = (double)m[7] - vc * (double)m[11];
+        const double cr0 = a1 * q2 - a2 * q1, cr1 = a2 * q0 - a0 * q2, cr2 = a0 * q1 - a1 * q0;
+        // components along K, F and G
+        auto pick = [](double x, double y, double z, int i) { return i == 0 ? x : (i == 1 ? y : z); };
+        const double pF = pick(a0, a1, a2, F), pG = pick(a0, a1, a2, G), pK = pick(a0, a1, a2, K);
+        const double qF = pick(q0, q1, q2, F), qG = pick(q0, q1, q2, G), qK = pick(q0, q1, q2, K);
+        const double det = pF * qG - pG * qF; // +-cr[K]
+        const double len = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+        const bool ok = fabs(det) >= 0.25 * len && len > 0.0;
+        const unsigned long long okm = __ballot(ok), posm = __ballot(det > 0.0);
+        if (lane < 4) {
+            // (a . (h, 1) = 0, q . (h, 1) = 0) solved for (hF, hG) given hK
+            const double id = ok ? 1.0 / det : 0.0;
+            coef[c][0] = (float)((pG * q3 - a3 * qG) * id);
+            coef[c][1] = (float)((pG * qK - pK * qG) * id);
+            coef[c][2] = (float)((a3 * qF - pF * q3) * id);
+            coef[c][3] = (float)((pK * qF - pF * qK) * id);
+        }
+        if (lane == 0)
+            whole = !(okm == ~0ull && (posm == 0ull || posm == ~0ull));
+    }
+    for (int i = tid; i < kBTP * kBTP; i += kMaxBlock) cells[i] = 0u;
+    __syncthreads();
+    const int nK = K == 0 ? nx : (K == 1 ? ny : nz);
+    const int nF = F == 0 ? nx : ny, nG = G == 1 ? ny : nz;
+    const float cK = K == 0 ? cx : (K == 1 ? cy : cz);
+    const float cF = F == 0 ? cx : cy, cG = G == 1 ? cy : cz;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad;
+    const typename SV::raw *vb = vol + b * (size_t)nx * ny * nz;
+    const typename SV::raw *yb = ycopy + b * (size_t)Wp * Hp * (nz + 2 * kPad);
+    const float m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6],
+                m7 = m[7], m8 = m[8], m9 = m[9], m10 = m[10], m11 = m[11];
+    unsigned int *mine = &cells[(tid / kBT + 1) * kBTP + tid % kBT + 1];
+    long long own = 0;
+    // cells -> registers (border cells, which belong to other tiles, are never read)
+    auto drain = [&]() {
+        __syncthreads();
+        own += (long long)(int)*mine;
+        *mine = 0u;
+        __syncthreads();
+    };
+    int run = 0; // voxel visits since the last drain
+    int buf = 0;
+    for (int k0 = 0; k0 < nK; k0 += kBTSlab, buf ^= 1) {
+        const int S = min(kBTSlab, nK - k0);
+        if (tid < 64) {
+            int flo = 0, fhi = -1, glo = 0, ghi = -1;
+            if (lane < S) {
+                if (whole) {
+                    fhi = nF - 1;
+                    ghi = nG - 1;
+                } else {
+                    const float hK = (float)(k0 + lane) - cK;
+                    float f0 = INFINITY, f1 = -INFINITY, g0 = INFINITY, g1 = -INFINITY;
+                    for (int c = 0; c < 4; ++c) {
+                        const float f = fmaf(coef[c][1], hK, coef[c][0]);
+                        const float g = fmaf(coef[c][3], hK, coef[c][2]);
+                        f0 = fminf(f0, f);
+                        f1 = fmaxf(f1, f);
+                        g0 = fminf(g0, g);
+                        g1 = fmaxf(g1, g);
+                    }
+                    constexpr float big = 0x1p24f;
+                    flo = max(0, floor_i32(__builtin_amdgcn_fmed3f(f0 + cF, -big, big)) - 1);
+                    fhi = min(nF - 1, floor_i32(__builtin_amdgcn_fmed3f(f1 + cF, -big, big)) + 1);
+                    glo = max(0, floor_i32(__builtin_amdgcn_fmed3f(g0 + cG, -big, big)) - 1);
+                    ghi = min(nG - 1, floor_i32(__builtin_amdgcn_fmed3f(g1 + cG, -big, big)) + 1);
+                }
+                slab[buf][lane] = make_int4(flo, fhi, glo, ghi);
+            }
+            int ef = max(fhi - flo + 1, 0), eg = max(ghi - glo + 1, 0);
+            for (int off = 32; off > 0; off >>= 1) {
+                ef = max(ef, __shfl_xor(ef, off, 64));
+                eg = max(eg, __shfl_xor(eg, off, 64));
+            }
+            if (lane == 0) {
+                dims[buf][0] = ef;
+                dims[buf][1] = eg;
+            }
+        }
+        __syncthreads();
+        const int NF = dims[buf][0], NG = dims[buf][1];
+        const int total = NF * NG * S;
+        if (total == 0) continue;
+        // this thread's element e = tid + n kMaxBlock of the chunk's box, as (dF, dG, dk) with
+        // dF fastest; a step of kMaxBlock elements is (sF, sG, sK)
+        int dF = tid % NF, dG = (tid / NF) % NG, dk = tid / (NF * NG);
+        const int sF = kMaxBlock % NF, sG = (kMaxBlock / NF) % NG, sK = kMaxBlock / (NF * NG);
+        int e = tid;
+        for (int base = 0; base < total; base += kBTPass) {
+            const int n = min(kBTPass, total - base);
+            if (run + n > kBTBudget) {
+                drain();
+                run = 0;
+            }
+            run += n;
+            for (; e < base + n; e += kMaxBlock) {
+                const int4 sl = slab[buf][dk];
+                const int f = sl.x + dF, g = sl.z + dG;
+                if (f <= sl.y && g <= sl.w) {
+                    const int k = k0 + dk;
+                    const int ix = K == 0 ? k : f;
+                    const int iy = K == 1 ? k : (K == 0 ? f : g);
+                    const int iz = K == 2 ? k : g;
+                    const float val =
+                        K == 0 ? to_f32<SV>(yb[((iz + kPad) * Wp + ix + kPad) * Hp + iy + kPad])
+                               : to_f32<SV>(vb[(iz * ny + iy) * nx + ix]);
+                    const float vs = (float)((double)val * unit_inv);
+                    if (vs != 0.0f) {
+                        const float hx = (float)ix - cx, hy = (float)iy - cy, hz = (float)iz - cz;
+                        const float Uh = fmaf(m0, hx, fmaf(m1, hy, fmaf(m2, hz, m3)));
+                        const float Vh = fmaf(m4, hx, fmaf(m5, hy, fmaf(m6, hz, m7)));
+                        const float Wh = fmaf(m8, hx, fmaf(m9, hy, fmaf(m10, hz, m11)));
+                        const float rw = __builtin_amdgcn_rcpf(Wh);
+                        const float u = Uh * rw, v = Vh * rw;
+                        const int i = floor_i32(u) - (iu0 - 1), j = floor_i32(v) - (iv0 - 1);
+                        if ((unsigned)i <= (unsigned)kBT && (unsigned)j <= (unsigned)kBT) {
+                            const float fu = __builtin_amdgcn_fractf(u),
+                                        fv = __builtin_amdgcn_fractf(v);
+                            const float t = FDK ? vs * (rw * rw) : vs;
+                            const float t1 = t * fu, t0 = t - t1;
+                            const float t01 = t0 * fv, t00 = t0 - t01;
+                            const float t11 = t1 * fv, t10 = t1 - t11;
+                            unsigned int *c = &cells[j * kBTP + i];
+                            atomicAdd(c, (unsigned)round_i32(t00));
+                            atomicAdd(c + 1, (unsigned)round_i32(t10));
+                            atomicAdd(c + kBTP, (unsigned)round_i32(t01));
+                            atomicAdd(c + kBTP + 1, (unsigned)round_i32(t11));
+                        }
+                    }
+                }
+                dF += sF;
+                if (dF >= NF) {
+                    dF -= NF;
+                    ++dG;
+                }
+                dG += sG;
+                if (dG >= NG) {
+                    dG -= NG;
+                    ++dk;
+                }
+                dk += sK;
+            }
+        }
+    }
+    __syncthreads();
+    own += (long long)(int)*mine;
+    const int iu = iu0 + tid % kBT, iv = iv0 + tid / kBT;
+    if (iu < U && iv < V) {
+        const double unit = (double)vmax * (double)wbound * 0x1p-18;
+        sino[((b * A + a) * V + iv) * (size_t)U + iu] = from_f32<SS>((float)((double)own * unit));
+    }
+}
+
 // ---------------------------------------------------------------- host helpers -----------
 
 // True when no angle's U or W row depends on y, so k_bp3d_y applies.
@@ -776,6 +1022,22 @@ bool y_invariant(const std::vector<float> &m, int64_t A) {
     for (int64_t a = 0; a < A; ++a)
         if (m[12 * a + 1] != 0.0f || m[12 * a + 9] != 0.0f) return false;
     return true;
+}
+
+// Smallest W = M[a][8..11] . (h, 1) over the volume's eight corner voxels (h = index - centre)
+// and all angles, in FP64 from the FP32 matrices.
+double corner_wmin(const cone3d::Desc &g, const std::vector<float> &m, int64_t A) {
+    const double c[3] = {((double)g.nx - 1.0) / 2.0, ((double)g.ny - 1.0) / 2.0,
+                         ((double)g.nz - 1.0) / 2.0};
+    double w = INFINITY;
+    for (int64_t a = 0; a < A; ++a) {
+        const float *r = m.data() + 12 * a + 8;
+        for (int k = 0; k < 8; ++k)
+            w = std::min(w, (double)r[3] + (k & 1 ? c[0] : -c[0]) * (double)r[0] +
+                                (k & 2 ? c[1] : -c[1]) * (double)r[1] +
+                                (k & 4 ? c[2] : -c[2]) * (double)r[2]);
+    }
+    return w;
 }
 
 // Projection matrices for the backprojector, rows (U, V, W) over h = (ix-cx, iy-cy, iz-cz, 1),
@@ -986,6 +1248,9 @@ cmi_status finish_plan3d(std::unique_ptr<cmi_plan3d> p, const std::vector<FwdGeo
     p->ycols[CMI_BP3D_REFERENCE] = y_invariant(mref, A);
     p->ycols[CMI_BP3D_MATCHED] = y_invariant(mmat, A);
     p->ycols[CMI_BP3D_FDK] = !mfdk.empty() && y_invariant(mfdk, A);
+    p->wmin[CMI_BP3D_REFERENCE] = corner_wmin(g, mref, A);
+    p->wmin[CMI_BP3D_MATCHED] = corner_wmin(g, mmat, A);
+    p->wmin[CMI_BP3D_FDK] = mfdk.empty() ? 0.0 : corner_wmin(g, mfdk, A);
 
     DeviceGuard guard;
     CMI_TRY(guard.enter(p->device));
@@ -1361,6 +1626,77 @@ CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_
         return CMI_OK;
     });
     const hipError_t fe = hipFreeAsync(table, s);
+    if (st != CMI_OK) return st;
+    CMI_HIP_TRY(fe);
+    return CMI_OK;
+}
+
+CMI_API cmi_status cmi_backward_3d_transpose(const cmi_plan3d *plan, void *sinogram,
+                                             int sino_dtype, const void *volume, int vol_dtype,
+                                             uint64_t batch, int mode, void *stream) {
+    clear_error();
+    CMI_TRY(check_call(plan, volume, vol_dtype, sinogram, sino_dtype, batch,
+                       "backward_3d_transpose"));
+    if (mode != CMI_BP3D_REFERENCE && mode != CMI_BP3D_MATCHED && mode != CMI_BP3D_FDK)
+        return fail(CMI_ERR_INVALID_ARGUMENT, "backward_3d_transpose: unknown mode %d", mode);
+    if (mode == CMI_BP3D_FDK && plan->views)
+        return fail(CMI_ERR_NOT_SUPPORTED,
+                    "backward_3d_transpose: FDK weighting needs a circular orbit (DSO, DSD); "
+                    "this plan was built from views");
+    const double wmin = plan->wmin[mode];
+    if (!(wmin > 0.0) || !std::isfinite(wmin))
+        return fail(CMI_ERR_NOT_SUPPORTED,
+                    "backward_3d_transpose: a voxel lies at or behind a source (W <= 0), so "
+                    "the backprojection's projection is unbounded");
+    const cone3d::Desc &g = plan->g;
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz, U = (int)g.nu, V = (int)g.nv,
+              A = (int)plan->A;
+    const uint64_t tu = (uint64_t)(U + kBT - 1) / kBT, tv = (uint64_t)(V + kBT - 1) / kBT;
+    if ((uint64_t)A * tu * kMaxBlock >= (1ull << 32) || tv * batch > 65535)
+        return fail(CMI_ERR_INVALID_ARGUMENT,
+                    "backward_3d_transpose: too many projections, tiles or volumes for one "
+                    "launch");
+    // the largest weight: 1 (bilinear), times 1 / W^2 for FDK; 1 % of margin for FP32
+    const double wbound = 1.01 * (mode == CMI_BP3D_FDK ? 1.0 / (wmin * wmin) : 1.0);
+    if (!(wbound < 1e30))
+        return fail(CMI_ERR_NOT_SUPPORTED, "backward_3d_transpose: FDK weight out of range");
+    DeviceGuard guard;
+    CMI_TRY(guard.enter(plan->device));
+    const hipStream_t s = (hipStream_t)stream;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad, Dp = nz + 2 * kPad;
+    const size_t one = (size_t)Wp * Hp * Dp;
+    // scratch: per volume a y-fastest copy (read where the march axis is x), then kMaxParts
+    // partial maxima of |vol| per volume
+    const size_t copy_bytes = (one * dtype_size(vol_dtype) * batch + 255) / 256 * 256;
+    void *scratch = nullptr;
+    CMI_HIP_TRY(hipMallocAsync(&scratch, copy_bytes + kMaxParts * batch * sizeof(float), s));
+    float *parts = (float *)((char *)scratch + copy_bytes);
+    const float *M = mode == CMI_BP3D_MATCHED ? plan->d_mmat
+                     : mode == CMI_BP3D_FDK   ? plan->d_mfdk
+                                              : plan->d_mref;
+    const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
+        using SV = decltype(sv);
+        using SS = decltype(ss);
+        using RV = typename SV::raw;
+        for (uint64_t b = 0; b < batch; ++b) {
+            dim3 tb(32, 8), tg((unsigned)((Wp + 31) / 32), (unsigned)((Hp + 31) / 32),
+                               (unsigned)Dp);
+            hipLaunchKernelGGL((k_stage3d_t<SV, 1>), tg, tb, 0, s,
+                               (const RV *)volume + b * (size_t)nx * ny * nz,
+                               (RV *)scratch + b * one, nx, ny, nz, Wp, Hp, Dp);
+        }
+        hipLaunchKernelGGL((k_absmax_parts<SV>), dim3(kMaxParts, (unsigned)batch),
+                           dim3(kMaxBlock), 0, s, (const RV *)volume, (size_t)nx * ny * nz,
+                           parts);
+        auto kern = mode == CMI_BP3D_FDK ? k_bpT3d<SV, SS, true> : k_bpT3d<SV, SS, false>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)(A * tu), (unsigned)(tv * batch)),
+                           dim3(kMaxBlock), 0, s, (typename SS::raw *)sinogram,
+                           (const RV *)volume, (const RV *)scratch, M, parts, (float)wbound, nx,
+                           ny, nz, U, V, A);
+        CMI_HIP_TRY(hipGetLastError());
+        return CMI_OK;
+    });
+    const hipError_t fe = hipFreeAsync(scratch, s);
     if (st != CMI_OK) return st;
     CMI_HIP_TRY(fe);
     return CMI_OK;

@@ -5,10 +5,11 @@ adjoint ``A^T grad`` (exact transpose of the ray march), not the voxel-driven ba
 the reference pairs it with (diffable.py:22-24).
 
 ``TorchBackprojection.apply(sinogram, geom)`` returns the voxel-driven backprojection
-``B sinogram``. For fan beams its gradient is the exact transpose ``B^T grad`` (the
-backprojector's own detector coordinates and weights); for cone beams it is ``A grad``, as the
-reference intends (diffable.py:35-38), an approximation since ``B`` is not the transpose of
-``A``.
+``B sinogram``; its gradient is the exact transpose ``B^T grad`` (the backprojector's own
+projection and weights), where the reference uses the forward projection ``A grad``
+(diffable.py:35-38), an approximation since ``B`` is not the transpose of ``A``. For cone beams
+with a voxel at or behind a source, where ``B^T`` is refused, the gradient falls back to
+``A grad``.
 
 Each function returns exactly one gradient per input.
 """
@@ -56,4 +57,7 @@ class TorchBackprojection(Function):
     def backward(ctx, grad_vol):
         if isinstance(ctx.geom, FanGeometry):
             return backward_transpose(grad_vol.contiguous(), ctx.geom), None
-        return forward(grad_vol.contiguous(), ctx.geom), None
+        try:
+            return backward_transpose(grad_vol.contiguous(), ctx.geom), None
+        except NotImplementedError:
+            return forward(grad_vol.contiguous(), ctx.geom), None

@@ -322,6 +322,22 @@ CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_
                                   const void *sinogram, int sino_dtype, uint64_t batch,
                                   void *stream);
 
+/*
+ * Transpose of cmi_backward_3d, sinogram := B^T volume (overwrites), in the same mode: the
+ * gradient of the voxel-driven backprojection, with the backprojector's FP32 projection
+ * matrices and bilinear weights (times (DSO / (t DSD))^2 for CMI_BP3D_FDK). Replaces the
+ * reference's approximate gradient of TorchBackprojection, the forward projection
+ * (python/standalone/src/curadon/diffable.py:35-38). Sums are fixed point in units of
+ * max|volume| times the weight bound over 2^18, kept exactly (32-bit LDS cells moved into 64-bit
+ * registers before they could overflow), so the result is deterministic. CMI_ERR_NOT_SUPPORTED
+ * when W <= 0 at a corner voxel of the volume for some projection (a voxel at or behind the
+ * source), and for CMI_BP3D_FDK on plans from views. Scratch (stream-ordered): a y-fastest
+ * copy of each volume with a 2-voxel border, plus 1 KB per volume.
+ */
+CMI_API cmi_status cmi_backward_3d_transpose(const cmi_plan3d *plan, void *sinogram,
+                                             int sino_dtype, const void *volume, int vol_dtype,
+                                             uint64_t batch, int mode, void *stream);
+
 /* ============================================================ serving: multi-job launch == */
 
 /*
