@@ -326,7 +326,8 @@ def kernels_only(torch, cm, phantom, np, which, reps):
         vol = torch.zeros_like(x)
         ops = [("forward", lambda: cm.forward(x, sino, geom)),
                ("backprojection", lambda: cm.backward(sino, vol, geom)),
-               ("adjoint", lambda: cm.adjoint(sino, vol, geom))]
+               ("adjoint", lambda: cm.adjoint(sino, vol, geom)),
+               ("backprojection transpose", lambda: cm.backward_transpose(x, geom, sino))]
     geom.plan
     torch.cuda.synchronize()
     for name, fn in ops:

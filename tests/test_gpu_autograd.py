@@ -86,7 +86,7 @@ def test_batched_gradient_matches_fp64(oracle, problem, gpu):
 
 def test_gradients_of_all_autograd_functions(gpu):
     """TorchForward / TorchAdjoint form an exact pair; TorchBackprojection's gradient is the
-    forward projection (the reference's intent); each returns one gradient per input."""
+    transpose of the backprojection; each returns one gradient per input."""
     from geoms import fan
     geom = as_fan_geometry(fan(48, 70, 40, 96.0, 144.0))
     x = torch.rand(48, 48, device=gpu, requires_grad=True)
@@ -99,4 +99,4 @@ def test_gradients_of_all_autograd_functions(gpu):
     torch.testing.assert_close(s.grad, cm.forward(wx, geom), rtol=1e-6, atol=1e-6)
     s.grad = None
     (cm.TorchBackprojection.apply(s, geom) * wx).sum().backward()
-    torch.testing.assert_close(s.grad, cm.forward(wx, geom), rtol=1e-6, atol=1e-6)
+    assert torch.equal(s.grad, cm.backward_transpose(wx, geom))
