@@ -235,6 +235,7 @@ __device__ __forceinline__ float sample3d(rsrc_t R, float f, float m, float l, i
 constexpr int kSlab = 8;          // slices of lower taps per slab
 constexpr int kBoxCells = 6144;   // floats of one staged box (24 KiB)
 constexpr int kBoxQuads = 8;      // at most 8 x 4 cells per staged row
+constexpr unsigned kFwdGroup = 32; // projections per group in the workgroup order
 
 template <class SI, class SS>
 __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__restrict__ pad,
@@ -249,13 +250,37 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
     __shared__ float cl[4][6]; // corner rays in padded index coordinates: point, direction
     __shared__ int red[2][kMaxBlock / 64];
     const int tid = threadIdx.x;
-    // XCD-aware order, as in k_fwd3d
+    // XCD-aware order. Workgroups are dealt to the 8 XCDs in turn, so dispatch slot hw runs
+    // on XCD hw % 8; logical index li gives each XCD a contiguous share of the order below.
+    // In that order kFwdGroup consecutive projections of one tile come first, then the tiles
+    // (u fastest), then the next group of projections: the 256 or so workgroups resident on an
+    // XCD cover about 8 tiles over 32 neighbouring projections, whose pyramids overlap, so
+    // their staged boxes share L2 (21.2 -> 18.8 ms at 256^3 against tiles first; projections
+    // first across all 360 was 19.8 ms).
     const unsigned gx = gridDim.x, gy = gridDim.y;
     const unsigned nwg = gx * gy * gridDim.z;
     const unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     const unsigned n8 = nwg & ~7u;
     const unsigned li = hw < n8 ? (hw & 7u) * (n8 >> 3) + (hw >> 3) : hw;
-    const unsigned tx = li % gx, ty = (li / gx) % gy, tz = li / (gx * gy);
+    unsigned tx, ty, tz;
+    {
+        const unsigned T = gx * gy, AT = (unsigned)A * T, G = kFwdGroup;
+        const unsigned bb = li / AT, l = li % AT;
+        const unsigned full = ((unsigned)A / G) * G * T; // workgroups in whole groups
+        unsigned t, ai;
+        if (l < full) {
+            const unsigned r = l % (G * T);
+            ai = l / (G * T) * G + r % G;
+            t = r / G;
+        } else { // the last, partial group
+            const unsigned r = l - full, Gl = (unsigned)A - (unsigned)A / G * G;
+            ai = (unsigned)A / G * G + r % Gl;
+            t = r / Gl;
+        }
+        tx = t % gx;
+        ty = t / gx;
+        tz = ai + (unsigned)A * bb;
+    }
     const int iu0 = (int)tx * 16, iv0 = (int)ty * 16;
     // a wavefront covers 4 x 16 pixels, consecutive lanes along v
     const int iu = iu0 + (tid >> 4), iv = iv0 + (tid & 15);
@@ -804,7 +829,8 @@ constexpr int kBTP = kBT + 2;                                  // LDS cells per 
 constexpr int kBTSlab = 32;                                    // slices per chunk
 constexpr int kBTBits = 18;                                    // a term's bound: 2^kBTBits
 constexpr int kBTBudget = (1 << (31 - kBTBits)) - 1;           // visits between drains
-constexpr int kBTPass = (kBTBudget / kMaxBlock) * kMaxBlock;   // visits per pass
+constexpr int kBTBatch = 4;                                    // voxels per thread at a time
+constexpr int kBTPass = kBTBudget / (kBTBatch * kMaxBlock) * (kBTBatch * kMaxBlock); // per pass
 static_assert(kBT * kBT == kMaxBlock, "one tile cell per thread");
 static_assert(kBTSlab <= 64, "one lane per slice of a chunk");
 
@@ -834,8 +860,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_bpT3d(typename SS::raw *__restric
     float vmax = 0.0f;
     for (int i = lane; i < kMaxParts; i += 64) vmax = fmaxf(vmax, parts[b * kMaxParts + i]);
     for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+    // a voxel value maps to vol ldexp(smant, sexp) units, the FP64 scale 2^18 / (max|vol| wbound)
+    // split into an FP32 mantissa and an exponent (the scale may lie outside the FP32 range)
+    static_assert(kBTBits == 18, "the scale uses 2^18");
     const double unit_inv = vmax > 0.0f ? 0x1p18 / ((double)vmax * (double)wbound) : 0.0;
-    static_assert(kBTBits == 18, "unit_inv uses 2^18");
+    const int sexp = unit_inv > 0.0 ? ilogb(unit_inv) : 0;
+    const float smant = (float)ldexp(unit_inv, -sexp);
     // march axis: the largest component of the central ray's direction, the cross product of
     // the (U - uc W) and (V - vc W) rows' voxel parts
     int K;
@@ -888,11 +918,26 @@ __global__ __launch_bounds__(kMaxBlock) void k_bpT3d(typename SS::raw *__restric
     const int nF = F == 0 ? nx : ny, nG = G == 1 ? ny : nz;
     const float cK = K == 0 ? cx : (K == 1 ? cy : cz);
     const float cF = F == 0 ? cx : cy, cG = G == 1 ? cy : cz;
+    // voxel (f, g, k) along (F, G, K) lies at src[f + g strG + k strK]: the volume, or for K = x
+    // the y-fastest copy [z + 2][x + 2][y + 2]
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad;
-    const typename SV::raw *vb = vol + b * (size_t)nx * ny * nz;
-    const typename SV::raw *yb = ycopy + b * (size_t)Wp * Hp * (nz + 2 * kPad);
-    const float m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6],
-                m7 = m[7], m8 = m[8], m9 = m[9], m10 = m[10], m11 = m[11];
+    const typename SV::raw *src =
+        K == 0 ? ycopy + b * (size_t)Wp * Hp * (nz + 2 * kPad) + (kPad * Wp + kPad) * Hp + kPad
+               : vol + b * (size_t)nx * ny * nz;
+    const int strG = K == 0 ? Wp * Hp : (K == 1 ? nx * ny : nx);
+    const int strK = K == 0 ? Hp : (K == 1 ? nx : nx * ny);
+    // matrix rows over (hF, hG, hK, 1)
+    auto row = [&](int r, float &rF, float &rG, float &rK, float &r3) {
+        const float x = m[4 * r], y = m[4 * r + 1], z = m[4 * r + 2];
+        rF = F == 0 ? x : y;
+        rG = G == 1 ? y : z;
+        rK = K == 0 ? x : (K == 1 ? y : z);
+        r3 = m[4 * r + 3];
+    };
+    float uF, uG, uK, u3, vF, vG, vK, v3, wF, wG, wK, w3;
+    row(0, uF, uG, uK, u3);
+    row(1, vF, vG, vK, v3);
+    row(2, wF, wG, wK, w3);
     unsigned int *mine = &cells[(tid / kBT + 1) * kBTP + tid % kBT + 1];
     long long own = 0;
     // cells -> registers (border cells, which belong to other tiles, are never read)
@@ -950,6 +995,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_bpT3d(typename SS::raw *__restric
         int dF = tid % NF, dG = (tid / NF) % NG, dk = tid / (NF * NG);
         const int sF = kMaxBlock % NF, sG = (kMaxBlock / NF) % NG, sK = kMaxBlock / (NF * NG);
         int e = tid;
+        auto advance = [&]() {
+            dF += sF;
+            if (dF >= NF) {
+                dF -= NF;
+                ++dG;
+            }
+            dG += sG;
+            if (dG >= NG) {
+                dG -= NG;
+                ++dk;
+            }
+            dk += sK;
+        };
         for (int base = 0; base < total; base += kBTPass) {
             const int n = min(kBTPass, total - base);
             if (run + n > kBTBudget) {
@@ -957,52 +1015,48 @@ __global__ __launch_bounds__(kMaxBlock) void k_bpT3d(typename SS::raw *__restric
                 run = 0;
             }
             run += n;
-            for (; e < base + n; e += kMaxBlock) {
-                const int4 sl = slab[buf][dk];
-                const int f = sl.x + dF, g = sl.z + dG;
-                if (f <= sl.y && g <= sl.w) {
-                    const int k = k0 + dk;
-                    const int ix = K == 0 ? k : f;
-                    const int iy = K == 1 ? k : (K == 0 ? f : g);
-                    const int iz = K == 2 ? k : g;
-                    const float val =
-                        K == 0 ? to_f32<SV>(yb[((iz + kPad) * Wp + ix + kPad) * Hp + iy + kPad])
-                               : to_f32<SV>(vb[(iz * ny + iy) * nx + ix]);
-                    const float vs = (float)((double)val * unit_inv);
-                    if (vs != 0.0f) {
-                        const float hx = (float)ix - cx, hy = (float)iy - cy, hz = (float)iz - cz;
-                        const float Uh = fmaf(m0, hx, fmaf(m1, hy, fmaf(m2, hz, m3)));
-                        const float Vh = fmaf(m4, hx, fmaf(m5, hy, fmaf(m6, hz, m7)));
-                        const float Wh = fmaf(m8, hx, fmaf(m9, hy, fmaf(m10, hz, m11)));
-                        const float rw = __builtin_amdgcn_rcpf(Wh);
-                        const float u = Uh * rw, v = Vh * rw;
-                        const int i = floor_i32(u) - (iu0 - 1), j = floor_i32(v) - (iv0 - 1);
-                        if ((unsigned)i <= (unsigned)kBT && (unsigned)j <= (unsigned)kBT) {
-                            const float fu = __builtin_amdgcn_fractf(u),
-                                        fv = __builtin_amdgcn_fractf(v);
-                            const float t = FDK ? vs * (rw * rw) : vs;
-                            const float t1 = t * fu, t0 = t - t1;
-                            const float t01 = t0 * fv, t00 = t0 - t01;
-                            const float t11 = t1 * fv, t10 = t1 - t11;
-                            unsigned int *c = &cells[j * kBTP + i];
-                            atomicAdd(c, (unsigned)round_i32(t00));
-                            atomicAdd(c + 1, (unsigned)round_i32(t10));
-                            atomicAdd(c + kBTP, (unsigned)round_i32(t01));
-                            atomicAdd(c + kBTP + 1, (unsigned)round_i32(t11));
-                        }
-                    }
+            // kBTBatch elements per thread at a time: their loads are issued together. Passes
+            // start at multiples of kBTBatch kMaxBlock elements, so a batch never straddles two.
+            for (; e < base + n; e += kBTBatch * kMaxBlock) {
+                int fq[kBTBatch], gq[kBTBatch], kq[kBTBatch], addr[kBTBatch];
+                bool live[kBTBatch];
+#pragma unroll
+                for (int q = 0; q < kBTBatch; ++q) {
+                    const int4 sl = slab[buf][min(dk, S - 1)];
+                    fq[q] = sl.x + dF;
+                    gq[q] = sl.z + dG;
+                    kq[q] = k0 + dk;
+                    live[q] = e + q * kMaxBlock < base + n && fq[q] <= sl.y && gq[q] <= sl.w;
+                    addr[q] = live[q] ? fq[q] + gq[q] * strG + kq[q] * strK : 0;
+                    advance();
                 }
-                dF += sF;
-                if (dF >= NF) {
-                    dF -= NF;
-                    ++dG;
+                float val[kBTBatch];
+#pragma unroll
+                for (int q = 0; q < kBTBatch; ++q) val[q] = to_f32<SV>(src[addr[q]]);
+#pragma unroll
+                for (int q = 0; q < kBTBatch; ++q) {
+                    const float vs = live[q] ? ldexpf(val[q] * smant, sexp) : 0.0f;
+                    if (vs == 0.0f) continue;
+                    const float hF = (float)fq[q] - cF, hG = (float)gq[q] - cG,
+                                hK = (float)kq[q] - cK;
+                    const float Uh = fmaf(uF, hF, fmaf(uG, hG, fmaf(uK, hK, u3)));
+                    const float Vh = fmaf(vF, hF, fmaf(vG, hG, fmaf(vK, hK, v3)));
+                    const float Wh = fmaf(wF, hF, fmaf(wG, hG, fmaf(wK, hK, w3)));
+                    const float rw = __builtin_amdgcn_rcpf(Wh);
+                    const float u = Uh * rw, v = Vh * rw;
+                    const int i = floor_i32(u) - (iu0 - 1), j = floor_i32(v) - (iv0 - 1);
+                    if ((unsigned)i > (unsigned)kBT || (unsigned)j > (unsigned)kBT) continue;
+                    const float fu = __builtin_amdgcn_fractf(u), fv = __builtin_amdgcn_fractf(v);
+                    const float t = FDK ? vs * (rw * rw) : vs;
+                    const float t1 = t * fu, t0 = t - t1;
+                    const float t01 = t0 * fv, t00 = t0 - t01;
+                    const float t11 = t1 * fv, t10 = t1 - t11;
+                    unsigned int *c = &cells[j * kBTP + i];
+                    atomicAdd(c, (unsigned)round_i32(t00));
+                    atomicAdd(c + 1, (unsigned)round_i32(t10));
+                    atomicAdd(c + kBTP, (unsigned)round_i32(t01));
+                    atomicAdd(c + kBTP + 1, (unsigned)round_i32(t11));
                 }
-                dG += sG;
-                if (dG >= NG) {
-                    dG -= NG;
-                    ++dk;
-                }
-                dk += sK;
             }
         }
     }
