@@ -999,17 +999,34 @@ __global__ __launch_bounds__(kMaxBlock) void k_fill_u64(unsigned long long *__re
 // linear interpolation of row a at u_a(p) (cells floor(u) and floor(u) + 1, weights 1 - al and
 // al; cells outside the row read zero), times 1/du (circular plans; the view's 1/|u| is part of
 // wt for plans from views). So sino[a][c] = sum_p (those weights) v[p], with k_bp2d's FP32
-// positions and weights. A workgroup owns a 16 x 16 pixel tile and a group of projections; per
-// chunk of kTC projections it projects the tile's corners (lanes 4j .. 4j + 3 for projection
-// j), adds each pixel's two terms into that projection's kTS-cell segment of LDS, and adds the
-// segments to a 64-bit image of the sinogram with integer global atomics. Terms are fixed
-// point in units of max|v| * bound / 2^30, where `bound` is the plan's bound on wt (times
-// 1/du) over the image and all projections, so a term fits in 31 bits and the 64-bit sums are
-// exact: the result does not depend on the order of the additions. A projection whose tile
-// footprint is wider than a segment adds its terms to the image directly.
-constexpr int kTC = 16; // projections per chunk
-constexpr int kTS = 64; // cells per projection's segment
-static_assert(4 * kTC == 64, "lanes 4 j .. 4 j + 3 project the corners of projection j");
+// positions and weights.
+// A workgroup owns a segment of kGS detector cells of one projection and gathers it. The pixels
+// whose terms reach the segment's cells are those with u in [c0 - 1, c0 + kGS): a wedge from
+// the source between the lines u = c0 - 1 and u = c0 + kGS (u is a ratio of two functions
+// affine in the pixel). The workgroup marches through the image along the axis K (rows or
+// columns) that the segment's central line follows most closely, in chunks of kGSlab lines;
+// per line the wedge is the interval between the two boundary lines' crossings (one pixel of
+// margin each side, clipped to the image), walked by the workgroup's threads along the line,
+// from the zero-bordered copy whose rows run along it (row-major for rows, transposed for
+// columns), so that a wavefront reads runs of consecutive pixels. Every pixel computes its own
+// u and weight with the backprojector's arithmetic and adds its terms that fall inside the
+// segment to kGS + 2 LDS cells with integer atomics. A segment whose boundary lines meet the
+// lines at grazing angles walks whole lines.
+// Fixed point: terms are rounded to units of max|v| * bound / 2^kGBits, where `bound` is the
+// plan's bound on wt (times 1/du) over the image and all projections, so |term| <= 2^kGBits
+// and a 32-bit LDS cell holds the sum of up to 2^(31 - kGBits) - 1 terms. A pixel adds at most
+// one term to a cell, so the cells are moved into 64-bit registers (one per thread of the first
+// kGS) and cleared before the pixel visits since the last such drain could pass that count.
+// Integer sums are exact: the result does not depend on the order of the additions. Each
+// segment is written once, in the sinogram's storage type.
+constexpr int kGS = 64;                                      // detector cells per segment
+constexpr int kGC = kGS + 2;                                 // LDS cells: c0 - 1 .. c0 + kGS
+constexpr int kGSlab = 64;                                   // image lines per chunk
+constexpr int kGBits = 18;                                   // a term's bound: 2^kGBits units
+constexpr int kGBudget = (1 << (31 - kGBits)) - 1;           // pixel visits between drains
+constexpr int kGBatch = 4;                                   // pixels per thread at a time
+constexpr int kGPass = kGBudget / (kGBatch * kMaxBlock) * (kGBatch * kMaxBlock); // per pass
+static_assert(kGS <= kMaxBlock && kGSlab <= 64, "one owned cell per thread, one lane per line");
 
 // Largest |v| of each image of a batch, in kAdjParts partial maxima per image.
 template <class SV>
@@ -1057,111 +1074,189 @@ __device__ __forceinline__ void bpT_geom(const float4 *__restrict__ tab, int a, 
     }
 }
 
-template <class SV, int WPOW, bool VIEWS>
-__global__ __launch_bounds__(kMaxBlock) void k_bpT2d(unsigned long long *__restrict__ accs,
-                                                     const typename SV::raw *__restrict__ vol,
-                                                     const float4 *__restrict__ tab,
-                                                     const float *__restrict__ parts, float bound,
-                                                     int nx, int ny, int D, int A, int G, float sx,
-                                                     float sy, float ox, float oy, float inv_du,
-                                                     float half_d) {
-    __shared__ long long seg[kTC * kTS];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int g = blockIdx.z % G;
-    const size_t b = blockIdx.z / G;
-    const int a0 = (int)((int64_t)g * A / G), a1 = (int)((int64_t)(g + 1) * A / G);
-    const int ix0 = blockIdx.x * 16, iy0 = blockIdx.y * 16;
-    const int ix = ix0 + (tid & 15), iy = iy0 + (tid >> 4);
-    const bool live = ix < nx && iy < ny;
-    const float hi = (float)D, c0 = half_d - 0.5f;
-    // fixed-point scale: a term v wt (1/du) of magnitude up to max|v| bound maps to 2^30
-    // (in FP64: max|v| bound may lie far from 1)
-    const float vmax = adj2d_ymax(parts + b * kAdjParts);
-    const double unit_inv = vmax > 0.0f ? 0x1p30 / ((double)vmax * (double)bound) : 0.0;
-    auto world = [&](int i, int n, float s, float o) {
-        return ((float)i - (float)n / 2.0f) * s + o + 0.5f;
-    };
-    const float x = world(ix, nx, sx, ox), y = world(iy, ny, sy, oy);
-    const float px = x * inv_du, py = y * inv_du;
-    const float v =
-        live ? (float)((double)to_f32<SV>(vol[(b * ny + iy) * (size_t)nx + ix]) * unit_inv) : 0.0f;
-    // the tile corner (pixel centre) this lane projects
-    const int cxi = (lane & 1) ? min(ix0 + 15, nx - 1) : ix0;
-    const int cyi = (lane & 2) ? min(iy0 + 15, ny - 1) : iy0;
-    const float qx = world(cxi, nx, sx, ox), qy = world(cyi, ny, sy, oy);
-    const float qpx = qx * inv_du, qpy = qy * inv_du;
-    unsigned long long *out = accs + b * (size_t)A * D;
-    for (int c = a0; c < a1; c += kTC) {
-        // footprint of projection c + (lane >> 2): its first segment cell, or -2^30 when the
-        // footprint is wider than a segment
-        int s0v;
-        {
-            const int a = min(c + (lane >> 2), A - 1);
-            float u, wt;
-            bpT_geom<VIEWS, WPOW>(tab, a, qx, qy, qpx, qpy, c0, hi, u, wt);
-            float lo = u, up = u;
-            lo = fminf(lo, __shfl_xor(lo, 1, 64));
-            lo = fminf(lo, __shfl_xor(lo, 2, 64));
-            up = fmaxf(up, __shfl_xor(up, 1, 64));
-            up = fmaxf(up, __shfl_xor(up, 2, 64));
-            const int first = floor_i32(lo) - 1, last = floor_i32(up) + 2;
-            s0v = last - first + 1 <= kTS ? first : -(1 << 30);
-        }
-        for (int i = tid; i < kTC * kTS; i += kMaxBlock) seg[i] = 0;
-        __syncthreads();
-        if (live && v != 0.0f) {
-#pragma unroll 4
-            for (int j = 0; j < kTC; ++j) {
-                const int a = c + j;
-                if (a >= a1) break;
-                float u, wt;
-                bpT_geom<VIEWS, WPOW>(tab, a, x, y, px, py, c0, hi, u, wt);
-                const int i = floor_i32(u);
-                const float al = __builtin_amdgcn_fractf(u);
-                const float t = v * wt * (VIEWS ? 1.0f : inv_du);
-                const float t1 = t * al, t0 = t - t1;
-                const int s0 = __builtin_amdgcn_readlane(s0v, 4 * j);
-                const int k = i - s0;
-                if ((unsigned)k <= (unsigned)(kTS - 2)) {
-                    atomicAdd((unsigned long long *)&seg[j * kTS + k],
-                              (unsigned long long)(long long)round_i32(t0));
-                    atomicAdd((unsigned long long *)&seg[j * kTS + k + 1],
-                              (unsigned long long)(long long)round_i32(t1));
-                } else { // a wide footprint: straight into the image
-                    unsigned long long *row = out + (size_t)a * D;
-                    if (i >= 0 && i < D)
-                        atomicAdd(row + i, (unsigned long long)(long long)round_i32(t0));
-                    if (i + 1 >= 0 && i + 1 < D)
-                        atomicAdd(row + i + 1, (unsigned long long)(long long)round_i32(t1));
-                }
-            }
-        }
-        __syncthreads();
-        for (int e = tid; e < kTC * kTS; e += kMaxBlock) {
-            const int j = e / kTS, k = e % kTS;
-            const int s0 = __shfl(s0v, 4 * j, 64);
-            const int cell = s0 + k;
-            const long long q = seg[e];
-            if (c + j < a1 && q != 0 && cell >= 0 && cell < D)
-                atomicAdd(out + (size_t)(c + j) * D + cell, (unsigned long long)q);
-        }
-        __syncthreads(); // the segments are read before the next chunk clears them
+// The line u(x, y) = uc of projection a as l0 x + l1 y + l2 = 0 in world coordinates (FP64),
+// from the same table as bpT_geom: u = w eta + c with w and eta ratios / sums of affine terms.
+template <bool VIEWS>
+__device__ __forceinline__ void bpT_line(const float4 *__restrict__ tab, int a, double uc,
+                                         double inv_du, double c0, double l[3]) {
+    if (VIEWS) {
+        const float4 m0 = tab[3 * a], m1 = tab[3 * a + 1];
+        const double k = uc - (double)m1.w; // k (m0 . (x, y, 1)) = m0.w (m1 . (x, y, 1))
+        l[0] = k * m0.x - (double)m0.w * m1.x;
+        l[1] = k * m0.y - (double)m0.w * m1.y;
+        l[2] = k * m0.z - (double)m0.w * m1.z;
+    } else {
+        const float4 m = tab[a];
+        const double k = uc - c0; // k (m.x x - m.y y + m.w) = m.z (m.y x + m.x y) / du
+        l[0] = k * m.x - (double)m.z * inv_du * m.y;
+        l[1] = -k * m.y - (double)m.z * inv_du * m.x;
+        l[2] = k * m.w;
     }
 }
 
-// sino := accs * max|v| bound / 2^30.
-template <class SS>
-__global__ __launch_bounds__(kMaxBlock) void k_bpT2d_finish(typename SS::raw *__restrict__ sino,
-                                                            const unsigned long long *__restrict__ accs,
-                                                            const float *__restrict__ parts,
-                                                            float bound, size_t rays) {
+template <class SV, class SS, int WPOW, bool VIEWS>
+__global__ __launch_bounds__(kMaxBlock) void k_bpT2d(typename SS::raw *__restrict__ sino,
+                                                     const typename SV::raw *__restrict__ staged,
+                                                     const float4 *__restrict__ tab,
+                                                     const float *__restrict__ parts, float bound,
+                                                     int nx, int ny, int D, int A, float sx,
+                                                     float sy, float ox, float oy, float inv_du,
+                                                     float half_d) {
+    __shared__ unsigned int cells[kGC]; // cell c0 - 1 + i at i
+    __shared__ int2 span[2][kGSlab];    // per line of a chunk: pixel range along it (inclusive)
+    __shared__ int width[2];            // per chunk: the widest range
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int a = (int)(blockIdx.x % (unsigned)A);
+    const int c0 = (int)(blockIdx.x / (unsigned)A) * kGS;
     const size_t b = blockIdx.y;
+    const float hi = (float)D, uoff = half_d - 0.5f;
+    // fixed-point scale 2^18 / (max|v| bound) as an FP32 mantissa and an exponent
+    static_assert(kGBits == 18, "the scale uses 2^18");
     const float vmax = adj2d_ymax(parts + b * kAdjParts);
-    const double unit = (double)vmax * (double)bound * 0x1p-30;
-    const size_t j = (size_t)blockIdx.x * kMaxBlock + threadIdx.x;
-    if (j >= rays) return;
-    const size_t i = b * rays + j;
-    sino[i] = from_f32<SS>((float)((double)(long long)accs[i] * unit));
+    const double unit_inv = vmax > 0.0f ? 0x1p18 / ((double)vmax * (double)bound) : 0.0;
+    const int sexp = unit_inv > 0.0 ? ilogb(unit_inv) : 0;
+    const float smant = (float)ldexp(unit_inv, -sexp);
+    // pixel index -> world: x = ix sx + bx (the world() map of the backprojector)
+    const double bx = (double)ox + 0.5 - (double)((float)nx / 2.0f) * sx;
+    const double by = (double)oy + 0.5 - (double)((float)ny / 2.0f) * sy;
+    // the segment's central and boundary lines in index coordinates: n0 ix + n1 iy + n2 = 0
+    auto index_line = [&](double uc, double (&n)[3]) {
+        double l[3];
+        bpT_line<VIEWS>(tab, a, uc, (double)inv_du, (double)uoff, l);
+        n[0] = l[0] * sx;
+        n[1] = l[1] * sy;
+        n[2] = l[0] * bx + l[1] * by + l[2];
+    };
+    // march along rows (K = y, ranges along x) when the central line runs closer to y
+    int rows;
+    {
+        double n[3];
+        index_line((double)c0 + 0.5 * kGS, n);
+        rows = fabs(n[0]) >= fabs(n[1]) ? 1 : 0;
+        rows = __builtin_amdgcn_readfirstlane(rows);
+    }
+    const int nK = rows ? ny : nx, nF = rows ? nx : ny;
+    // boundary lines as F = f0 + f1 K, valid where the line crosses the lines of K steeply
+    float f0[2], f1[2];
+    bool whole = false;
+    for (int e = 0; e < 2; ++e) {
+        double n[3];
+        index_line(e ? (double)(c0 + kGS) : (double)(c0 - 1), n);
+        const double nF_ = rows ? n[0] : n[1], nK_ = rows ? n[1] : n[0];
+        if (!(fabs(nF_) >= 0.25 * sqrt(nF_ * nF_ + nK_ * nK_)) || nF_ == 0.0) {
+            whole = true;
+            f0[e] = f1[e] = 0.0f;
+        } else {
+            f0[e] = (float)(-n[2] / nF_);
+            f1[e] = (float)(-nK_ / nF_);
+        }
+    }
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad;
+    // pixel (f, k) along (F, K) lies at src[f + k strK] of the copy whose rows run along F
+    const typename SV::raw *src = staged + b * 2 * (size_t)Wp * Hp +
+                                  (rows ? (size_t)0 : (size_t)Wp * Hp) +
+                                  (size_t)kPad * (rows ? Wp : Hp) + kPad;
+    const int strK = rows ? Wp : Hp;
+    for (int i = tid; i < kGC; i += kMaxBlock) cells[i] = 0u;
+    long long own = 0;
+    auto drain = [&]() {
+        __syncthreads();
+        if (tid < kGS) {
+            own += (long long)(int)cells[tid + 1];
+            cells[tid + 1] = 0u;
+        }
+        __syncthreads();
+    };
+    int run = 0; // pixel visits since the last drain
+    int buf = 0;
+    for (int k0 = 0; k0 < nK; k0 += kGSlab, buf ^= 1) {
+        const int S = min(kGSlab, nK - k0);
+        if (tid < 64) {
+            int lo = 0, up = -1;
+            if (lane < S) {
+                if (whole) {
+                    up = nF - 1;
+                } else {
+                    const float k = (float)(k0 + lane);
+                    const float g0 = fmaf(f1[0], k, f0[0]), g1 = fmaf(f1[1], k, f0[1]);
+                    constexpr float big = 0x1p24f;
+                    lo = max(0, floor_i32(__builtin_amdgcn_fmed3f(fminf(g0, g1), -big, big)) - 1);
+                    up = min(nF - 1,
+                             floor_i32(__builtin_amdgcn_fmed3f(fmaxf(g0, g1), -big, big)) + 1);
+                }
+                span[buf][lane] = make_int2(lo, up);
+            }
+            int w = max(up - lo + 1, 0);
+            for (int off = 32; off > 0; off >>= 1) w = max(w, __shfl_xor(w, off, 64));
+            if (lane == 0) width[buf] = w;
+        }
+        __syncthreads();
+        const int NW = width[buf];
+        const int total = NW * S;
+        if (total == 0) continue;
+        // element e = tid + n kMaxBlock of the chunk as (dF, dk), dF fastest; a step of
+        // kMaxBlock elements is (sF, sK). Quotients by NW through an FP32 reciprocal: (m + 0.5)
+        // / NW lies at least 0.5 / NW from an integer, far more than the rounding for m < 2^20.
+        const float rNW = 1.0f / (float)NW;
+        int dk = (int)(((float)tid + 0.5f) * rNW), dF = tid - dk * NW;
+        const int sK = (int)(((float)kMaxBlock + 0.5f) * rNW), sF = kMaxBlock - sK * NW;
+        int e = tid;
+        for (int base = 0; base < total; base += kGPass) {
+            const int n = min(kGPass, total - base);
+            if (run + n > kGBudget) {
+                drain();
+                run = 0;
+            }
+            run += n;
+            // kGBatch elements per thread at a time, their loads issued together; passes start
+            // at multiples of kGBatch kMaxBlock elements, so a batch never straddles two
+            for (; e < base + n; e += kGBatch * kMaxBlock) {
+                int fq[kGBatch], kq[kGBatch];
+                bool live[kGBatch];
+                float val[kGBatch];
+#pragma unroll
+                for (int q = 0; q < kGBatch; ++q) {
+                    const int2 sp = span[buf][min(dk, S - 1)];
+                    fq[q] = sp.x + dF;
+                    kq[q] = k0 + dk;
+                    live[q] = e + q * kMaxBlock < base + n && fq[q] <= sp.y;
+                    val[q] = to_f32<SV>(src[live[q] ? fq[q] + kq[q] * strK : 0]);
+                    dF += sF;
+                    if (dF >= NW) {
+                        dF -= NW;
+                        ++dk;
+                    }
+                    dk += sK;
+                }
+#pragma unroll
+                for (int q = 0; q < kGBatch; ++q) {
+                    const float vs = live[q] ? ldexpf(val[q] * smant, sexp) : 0.0f;
+                    if (vs == 0.0f) continue;
+                    const int ix = rows ? fq[q] : kq[q], iy = rows ? kq[q] : fq[q];
+                    const float x = ((float)ix - (float)nx / 2.0f) * sx + ox + 0.5f;
+                    const float y = ((float)iy - (float)ny / 2.0f) * sy + oy + 0.5f;
+                    float u, wt;
+                    bpT_geom<VIEWS, WPOW>(tab, a, x, y, x * inv_du, y * inv_du, uoff, hi, u, wt);
+                    const int i = floor_i32(u) - (c0 - 1);
+                    if ((unsigned)i > (unsigned)kGS) continue;
+                    const float al = __builtin_amdgcn_fractf(u);
+                    const float t = vs * wt * (VIEWS ? 1.0f : inv_du);
+                    const float t1 = t * al, t0 = t - t1;
+                    atomicAdd(&cells[i], (unsigned)round_i32(t0));
+                    atomicAdd(&cells[i + 1], (unsigned)round_i32(t1));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < kGS) {
+        own += (long long)(int)cells[tid + 1];
+        const int c = c0 + tid;
+        if (c < D) {
+            const double unit = (double)vmax * (double)bound * 0x1p-18;
+            sino[(b * A + a) * (size_t)D + c] = from_f32<SS>((float)((double)own * unit));
+        }
+    }
 }
 
 // ---------------------------------------------------------------- dispatch -----------------
@@ -1737,38 +1832,40 @@ CMI_API cmi_status cmi_backward_2d_transpose(const cmi_plan2d *plan, void *sinog
     CMI_TRY(guard.enter(plan->device));
     const hipStream_t s = (hipStream_t)stream;
     const int nx = (int)plan->g.nx, ny = (int)plan->g.ny, D = (int)plan->g.D, A = (int)plan->A;
-    const size_t img = (size_t)nx * ny, rays = (size_t)A * D;
-    // scratch: the 64-bit sinogram sums, then kAdjParts partial maxima per image
+    const size_t img = (size_t)nx * ny;
+    const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad;
+    const uint64_t nseg = (uint64_t)(D + kGS - 1) / kGS;
+    if ((uint64_t)A * nseg * kMaxBlock >= (1ull << 32) || batch > 65535)
+        return fail(CMI_ERR_INVALID_ARGUMENT,
+                    "backward_2d_transpose: too many projections, cells or images for one "
+                    "launch");
+    // scratch: per image its zero-bordered row-major and transposed copies, then kAdjParts
+    // partial maxima per image
+    const size_t copy_bytes =
+        (2 * (size_t)Wp * Hp * dtype_size(vol_dtype) * batch + 255) / 256 * 256;
     void *scratch = nullptr;
-    CMI_HIP_TRY(hipMallocAsync(&scratch, rays * batch * 8 + kAdjParts * batch * sizeof(float), s));
-    unsigned long long *accs = (unsigned long long *)scratch;
-    float *parts = (float *)(accs + rays * batch);
+    CMI_HIP_TRY(hipMallocAsync(&scratch, copy_bytes + kAdjParts * batch * sizeof(float), s));
+    float *parts = (float *)((char *)scratch + copy_bytes);
     const cmi_status st = dispatch2(vol_dtype, sino_dtype, [&](auto sv, auto ss) -> cmi_status {
         using SV = decltype(sv);
         using SS = decltype(ss);
-        const size_t n = rays * batch;
-        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + kMaxBlock - 1) / kMaxBlock)),
-                           dim3(kMaxBlock), 0, s, accs, n);
+        using RV = typename SV::raw;
+        dim3 sb(32, 8), sg((unsigned)((Wp + 31) / 32), (unsigned)((Hp + 31) / 32),
+                           (unsigned)batch);
+        hipLaunchKernelGGL((k_stage2d<SV>), sg, sb, 0, s, (const RV *)volume, (RV *)scratch, nx,
+                           ny, Wp, Hp);
         hipLaunchKernelGGL((k_absmax_img2d<SV>), dim3(kAdjParts, (unsigned)batch),
-                           dim3(kMaxBlock), 0, s, (const typename SV::raw *)volume, img, parts);
-        const unsigned tx = (unsigned)((nx + 15) / 16), ty = (unsigned)((ny + 15) / 16);
-        const uint64_t tiles = (uint64_t)tx * ty * batch;
-        // projection groups: enough workgroups to fill the GPU many times over
-        const int G = (int)std::max<uint64_t>(
-            1, std::min<uint64_t>((uint64_t)A, (8192 + tiles - 1) / tiles));
+                           dim3(kMaxBlock), 0, s, (const RV *)volume, img, parts);
         const bool views = plan->d_bpv != nullptr;
         const bool fbp = weighting == CMI_BP2D_FBP;
-        auto kern = views ? (fbp ? k_bpT2d<SV, 2, true> : k_bpT2d<SV, 1, true>)
-                          : (fbp ? k_bpT2d<SV, 2, false> : k_bpT2d<SV, 1, false>);
-        hipLaunchKernelGGL(kern, dim3(tx, ty, (unsigned)(G * batch)), dim3(kMaxBlock), 0, s,
-                           accs, (const typename SV::raw *)volume,
+        auto kern = views ? (fbp ? k_bpT2d<SV, SS, 2, true> : k_bpT2d<SV, SS, 1, true>)
+                          : (fbp ? k_bpT2d<SV, SS, 2, false> : k_bpT2d<SV, SS, 1, false>);
+        // projections fastest: neighbouring workgroups gather overlapping wedges
+        hipLaunchKernelGGL(kern, dim3((unsigned)(A * nseg), (unsigned)batch), dim3(kMaxBlock),
+                           0, s, (typename SS::raw *)sinogram, (const RV *)scratch,
                            views ? plan->d_bpv : plan->d_meta, parts, (float)bound, nx, ny, D,
-                           A, G, (float)plan->g.sx, (float)plan->g.sy, (float)plan->g.ox,
+                           A, (float)plan->g.sx, (float)plan->g.sy, (float)plan->g.ox,
                            (float)plan->g.oy, (float)(1.0 / plan->g.du), (float)D / 2.0f);
-        hipLaunchKernelGGL((k_bpT2d_finish<SS>),
-                           dim3((unsigned)((rays + kMaxBlock - 1) / kMaxBlock), (unsigned)batch),
-                           dim3(kMaxBlock), 0, s, (typename SS::raw *)sinogram, accs, parts,
-                           (float)bound, rays);
         CMI_HIP_TRY(hipGetLastError());
         return CMI_OK;
     });

@@ -216,10 +216,11 @@ CMI_API cmi_status cmi_adjoint_2d(const cmi_plan2d *plan, void *volume, int vol_
  * the voxel-driven backprojection, with the backprojector's own FP32 detector coordinates and
  * weights (reference or FBP weighting; circular plans and plans from views). Replaces the
  * reference's approximate gradient of TorchBackprojection, the forward projection
- * (python/standalone/src/curadon/diffable.py:35-38). Sums are 64-bit fixed point in units of
- * max|volume| times the plan's weight bound over 2^30, so the result is deterministic.
+ * (python/standalone/src/curadon/diffable.py:35-38). Sums are fixed point in units of
+ * max|volume| times the plan's weight bound over 2^18, kept exactly (32-bit LDS cells moved
+ * into 64-bit registers before they could overflow), so the result is deterministic.
  * CMI_ERR_NOT_SUPPORTED when a source lies inside the image's bounding box. Scratch
- * (stream-ordered): 8 bytes per sinogram element plus 1 KB per image.
+ * (stream-ordered): two zero-bordered copies of each image plus 1 KB per image.
  */
 CMI_API cmi_status cmi_backward_2d_transpose(const cmi_plan2d *plan, void *sinogram,
                                              int sino_dtype, const void *volume, int vol_dtype,
