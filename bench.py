@@ -127,6 +127,7 @@ def cone_case(torch, cm, phantom, np, stream, reps):
         ("bp3d", lambda: cm.backward(sino, bvol, geom)),
         ("bp3d_matched", lambda: cm.backward(sino, bvol, geom, mode="matched")),
         ("adj3d", lambda: cm.adjoint(sino, bvol, geom)),
+        ("bp3d_transpose", lambda: cm.backward_transpose(vol, geom, sino)),
     ], reps)
     n3 = 256 ** 3
     fwd_b = info["total_samples"] * 8 * 4 + 360 * 384 * 384 * 4
@@ -138,6 +139,7 @@ def cone_case(torch, cm, phantom, np, stream, reps):
         "bp3d": {"ms": t["bp3d"], "tap_equiv_GB_per_s": bp_b / t["bp3d"] / 1e6},
         "bp3d_matched": {"ms": t["bp3d_matched"], "tap_equiv_GB_per_s": bp_b / t["bp3d_matched"] / 1e6},
         "adj3d": {"ms": t["adj3d"]},
+        "bp3d_transpose": {"ms": t["bp3d_transpose"]},
         "fwd_bwd_pairs_per_s": 1e3 / (t["fwd3d"] + t["bp3d"]),
     }
 
