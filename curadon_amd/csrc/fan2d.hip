@@ -1136,13 +1136,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_bpT2d(typename SS::raw *__restric
         rows = __builtin_amdgcn_readfirstlane(rows);
     }
     const int nK = rows ? ny : nx, nF = rows ? nx : ny;
-    // boundary lines as F = f0 + f1 K, valid where the line crosses the lines of K steeply
+    // Boundary lines as F = f0 + f1 K. The wedge's section of a line of K is the interval
+    // between the two crossings when both boundary rays cross the lines of K steeply and in the
+    // same direction. A line's coefficients are (uc Q - P) for u = P / Q with Q > 0 in front of
+    // the source, so rotating its normal by a right angle gives a direction whose orientation
+    // relative to the rays' forward direction is the same for every uc (the 2 x 2 determinant
+    // of P and Q does not depend on uc): equal signs of the normals' F components mean equal
+    // directions along K.
     float f0[2], f1[2];
     bool whole = false;
+    double sgn[2];
     for (int e = 0; e < 2; ++e) {
         double n[3];
         index_line(e ? (double)(c0 + kGS) : (double)(c0 - 1), n);
         const double nF_ = rows ? n[0] : n[1], nK_ = rows ? n[1] : n[0];
+        sgn[e] = nF_;
         if (!(fabs(nF_) >= 0.25 * sqrt(nF_ * nF_ + nK_ * nK_)) || nF_ == 0.0) {
             whole = true;
             f0[e] = f1[e] = 0.0f;
@@ -1151,6 +1159,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_bpT2d(typename SS::raw *__restric
             f1[e] = (float)(-nK_ / nF_);
         }
     }
+    if ((sgn[0] > 0.0) != (sgn[1] > 0.0)) whole = true;
     const int Wp = nx + 2 * kPad, Hp = ny + 2 * kPad;
     // pixel (f, k) along (F, K) lies at src[f + k strK] of the copy whose rows run along F
     const typename SV::raw *src = staged + b * 2 * (size_t)Wp * Hp +

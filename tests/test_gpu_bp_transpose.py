@@ -65,6 +65,18 @@ def test_views(oracle, gpu, mode, wpow):
           oracle.bp2d_T(record(geom), v, wpow=wpow), f"views B^T ({mode})")
 
 
+def test_wide_cells_and_wide_wedges(oracle, gpu):
+    """Detector cells so wide that a segment's boundary lines meet the image lines at grazing
+    angles (the workgroups walk whole lines), and coarse cells whose 64-cell segments cover most
+    of the image."""
+    for rec, what in ((fan(16, 8, 6, 40.0, 50.0, du=20.0), "20-unit cells"),
+                      (fan(48, 100, 30, 60.0, 90.0, du=3.0, ny=40), "3-unit cells")):
+        v = np.random.default_rng(46).random(tuple(int(x) for x in rec.vol_shape))
+        for mode, wpow in (("reference", 1), ("fbp", 2)):
+            got = host(cm.backward_transpose(dev(v, gpu), as_fan_geometry(rec), mode=mode))
+            check(got, oracle.bp2d_T(rec, v, wpow=wpow), f"B^T {what} ({mode})")
+
+
 @pytest.mark.parametrize("a0,a1", [(0, 4), (178, 182), (357, 361), (700, 704)])
 def test_c2_projection_blocks(oracle, gpu, a0, a1):
     """The 1024^2 workload, full GPU transpose, oracle on four projections per block. Max-abs
