@@ -19,8 +19,9 @@
  *   forward  overwrites the sinogram (forward_2d.hpp:58-61,78);
  *   backward accumulates into the volume, += (backward_2d.hpp:99, backward_3d.hpp:100-107).
  * Differences, all stated where the entry point is declared: no host synchronisation (work
- * is enqueued on the given stream), 3D missed rays write 0, and no per-call copies into
- * texture arrays.
+ * is enqueued on the given stream), 3D missed rays write 0, and no texture arrays: instead of
+ * the reference's per-call copy into a texture array, a call stages zero-bordered copies of
+ * its input in a stream-ordered scratch allocation (cmi_forward_2d_noalloc uses a static one).
  */
 #ifndef CURADON_MI_H
 #define CURADON_MI_H
