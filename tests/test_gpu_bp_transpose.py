@@ -65,6 +65,17 @@ def test_views(oracle, gpu, mode, wpow):
           oracle.bp2d_T(record(geom), v, wpow=wpow), f"views B^T ({mode})")
 
 
+def test_odd_shapes_partial_segments(oracle, gpu):
+    """7 cells (one partial segment), a 9 x 6 image, batch of 2."""
+    rec = fan(9, 7, 3, 30.0, 50.0, ny=6)
+    geom = as_fan_geometry(rec)
+    v = np.random.default_rng(47).random((2, 6, 9))
+    for mode, wpow in (("reference", 1), ("fbp", 2)):
+        got = host(cm.backward_transpose(dev(v, gpu), geom, mode=mode))
+        for i in range(2):
+            check(got[i], oracle.bp2d_T(rec, v[i], wpow=wpow), f"odd shapes B^T ({mode}) image {i}")
+
+
 def test_wide_cells_and_wide_wedges(oracle, gpu):
     """Detector cells so wide that a segment's boundary lines meet the image lines at grazing
     angles (the workgroups walk whole lines), and coarse cells whose 64-cell segments cover most

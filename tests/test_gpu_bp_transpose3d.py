@@ -76,6 +76,17 @@ def test_wide_pixels_walk_whole_slices(oracle, gpu):
         check(got, oracle.bp3d_T(rec, v, **kw), f"wide pixels B^T ({mode})")
 
 
+def test_odd_shapes_partial_tiles(oracle, gpu):
+    """A 7 x 5 panel (one partial tile) and a 9 x 8 x 7 volume, batch of 2."""
+    rec = cone(9, 7, 5, 3, 30.0, 50.0, nz=7, ny=8)
+    geom = as_cone_geometry(rec)
+    v = np.random.default_rng(57).random((2, 7, 8, 9))
+    for mode, kw in MODES:
+        got = host(cm.backward_transpose(dev(v, gpu), geom, mode=mode))
+        for i in range(2):
+            check(got[i], oracle.bp3d_T(rec, v[i], **kw), f"odd shapes B^T ({mode}) volume {i}")
+
+
 def test_views(oracle, gpu):
     rec = cone(28, 40, 34, 18, 60.0, 100.0, vol_spacing=(1.1, 0.9, 1.0),
                vol_offset=(0.5, -0.7, 0.3), nz=24, ny=26)
