@@ -233,7 +233,11 @@ __device__ __forceinline__ float sample3d(rsrc_t R, float f, float m, float l, i
 // of consecutive voxels.
 // sino[b][a][iv][iu] = sum_{i=0..nst} trilinear(vol, s + (tmin + i dt) d)  (forward_3d.hpp:87-93)
 constexpr int kSlab = 8;          // slices of lower taps per slab
-constexpr int kBoxCells = 6144;   // floats of one staged box (24 KiB)
+// floats of one staged box (16 KiB). A smaller box keeps more workgroups per CU and sends more
+// slabs to the global path: at 256^3 the staged-box capacity in floats gave 2560: 18.3 ms,
+// 3072: 16.7, 3584: 16.3, 4096: 16.3, 4608: 16.5, 5120: 17.5, 5632: 17.6, 6144: 19.1
+// (bit-identical: both paths add the same values in the same order).
+constexpr int kBoxCells = 4096;
 constexpr int kBoxQuads = 8;      // at most 8 x 4 cells per staged row
 constexpr unsigned kFwdGroup = 32; // projections per group in the workgroup order
 
