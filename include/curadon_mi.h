@@ -202,9 +202,10 @@ CMI_API cmi_status cmi_backward_2d_weighted(const cmi_plan2d *plan, void *volume
 
 /*
  * Matched adjoint of cmi_forward_2d, volume += A^T sinogram: the exact transpose of the
- * ray-march above (same sample positions and bilinear weights). Sums are formed in 64-bit
- * fixed point (units of max|y scale| / 2^30 per image), so the result is deterministic and a
- * batched call equals single calls bit for bit. Scratch (stream-ordered): 8 bytes per volume
+ * ray-march above (same sample positions and bilinear weights). Sums are fixed point in units
+ * of max|y scale| / 2^19 per image, kept exactly (32-bit LDS cells moved into 64-bit sums
+ * before they could overflow), so the result is deterministic and a batched call equals
+ * single calls bit for bit. Scratch (stream-ordered): 8 bytes per volume
  * element plus 1 KB per image. No counterpart in the reference, whose autograd pairs the
  * forward with the voxel-driven backprojector (python/standalone/src/curadon/diffable.py:22-24).
  */
@@ -317,9 +318,11 @@ CMI_API cmi_status cmi_backward_3d(const cmi_plan3d *plan, void *volume, int vol
 
 /* Matched adjoint of cmi_forward_3d, volume += A^T sinogram: every forward sample scatters
  * its trilinear weights times the ray's value, at the forward march's own positions. The sums
- * are formed in 64-bit fixed point (units of max|y| / 2^30 per volume), so the result is
- * deterministic and a batched call equals single calls bit for bit. Scratch (stream-ordered):
- * a 32-byte march table entry per ray plus 1 KB per volume. No counterpart in the reference. */
+ * are fixed point in units of max|y| / 2^19 per volume, kept exactly (32-bit LDS cells moved
+ * into 64-bit sums before they could overflow), so the result is deterministic and a batched
+ * call equals single calls bit for bit. Scratch (stream-ordered):
+ * a 32-byte march table entry per ray, 8 bytes per voxel (rounded up to 16^3 tiles) and 1 KB per
+ * volume. No counterpart in the reference. */
 CMI_API cmi_status cmi_adjoint_3d(const cmi_plan3d *plan, void *volume, int vol_dtype,
                                   const void *sinogram, int sino_dtype, uint64_t batch,
                                   void *stream);
