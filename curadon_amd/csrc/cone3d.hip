@@ -537,7 +537,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_absmax_parts(const typename S::ra
 // six (48.0 -> 43.3 ms at 256^3, bit-identical).
 // The tile is added to the volume once, in its storage type.
 constexpr int kAT = 16;
-constexpr int kPh = 4; // candidate rays of one wavefront are kPh pixels apart in u and v
+// candidate rays of one wavefront are kPh pixels apart in u and v (at 256^3, six wavefronts per
+// SIMD, tiles in XCD order: kPh 1: 47.1 ms, 2: 38.2; tiles in launch order: 2: 38.7, 3: 42.5,
+// 4: 43.7, 5: 45.5, 6: 46.1)
+constexpr int kPh = 2;
 // LDS tile: padded by one voxel before and after on each axis, so that every tap of a sample
 // whose lower corner lies in [-1, kAT) on each axis lands in the array (taps outside the tile
 // are added to the border and dropped): the eight taps are then immediate offsets from one LDS
@@ -587,8 +590,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj3d(typename SV::raw *__restric
     };
     int bound = 0; // terms a voxel may have received since the last drain
     const int ntz = (nz + kAT - 1) / kAT;
-    const int x0 = blockIdx.x * kAT, y0 = blockIdx.y * kAT, z0 = (blockIdx.z % ntz) * kAT;
-    const size_t b = blockIdx.z / ntz;
+    // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, and logical index li gives
+    // each XCD a contiguous run of tiles (x fastest), whose candidate rays share L2
+    int x0, y0, z0;
+    size_t b;
+    {
+        const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+        const unsigned nwg = gx * gy * gz;
+        const unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+        const unsigned n8 = nwg & ~7u;
+        const unsigned li = hw < n8 ? (hw & 7u) * (n8 >> 3) + (hw >> 3) : hw;
+        const unsigned per = gx * gy * (unsigned)ntz;
+        b = li / per;
+        const unsigned l = li % per;
+        x0 = (int)(l % gx) * kAT;
+        y0 = (int)((l / gx) % gy) * kAT;
+        z0 = (int)(l / (gx * gy)) * kAT;
+    }
     // fixed-point scale: max|y| of this volume's projections maps to 2^30
     float ymax = 0.0f;
     for (int i = tid & 63; i < kMaxParts; i += 64) ymax = fmaxf(ymax, parts[b * kMaxParts + i]);

@@ -803,6 +803,11 @@ __device__ __forceinline__ float adj2d_ymax(const float *__restrict__ parts) {
 // share a tile), and k_adj2d_finish adds that image to the volume.
 constexpr int kT2 = 32;       // texel tile edge
 constexpr int kAdjAngles = 8; // angles whose candidate rays are staged together
+// Candidate order within an angle: phase p holds the rays p, p + kAdjPh, p + 2 kAdjPh, ..., so
+// the lanes of a wavefront take rays kAdjPh cells apart and add to different LDS cells more
+// often (padding the phases to equal size leaves holes, which are skipped). At 1024^2: kAdjPh
+// 1 (adjacent rays): 1.23 ms, 2: 1.15, 3: 1.21, 4: 1.27; bit-identical.
+constexpr int kAdjPh = 2;
 static_assert(kAdjAngles <= 16, "lanes 4 s .. 4 s + 3 of a wavefront project angle s");
 constexpr int kT2Pad = kT2 + 2;   // padded rows and columns
 constexpr int kT2Pitch = kT2 + 3; // odd pitch in words
@@ -846,9 +851,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
         __syncthreads();
     };
     int run = 0; // candidates added since the last drain
-    const int g = blockIdx.z % G;
-    const size_t b = blockIdx.z / G;
-    const int tx0 = blockIdx.x * kT2, ty0 = blockIdx.y * kT2;
+    // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, and logical index li gives
+    // each XCD a contiguous run of tiles of one angle group (x fastest), whose candidate rays
+    // share L2 (1.15 -> 1.08 ms at 1024^2; angle groups fastest: 1.17 ms)
+    int g, tx0, ty0;
+    size_t b;
+    {
+        const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+        const unsigned nwg = gx * gy * gz;
+        const unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+        const unsigned n8 = nwg & ~7u;
+        const unsigned li = hw < n8 ? (hw & 7u) * (n8 >> 3) + (hw >> 3) : hw;
+        const unsigned per = gx * gy * (unsigned)G;
+        b = li / per;
+        const unsigned l = li % per;
+        tx0 = (int)(l % gx) * kT2;
+        ty0 = (int)((l / gx) % gy) * kT2;
+        g = (int)(l / (gx * gy));
+    }
     const int a0 = (int)((int64_t)g * A / G), a1 = (int)((int64_t)(g + 1) * A / G);
     for (int i = tid; i < kT2Cells; i += kMaxBlock) (&tile[0][0])[i] = 0u;
     __syncthreads(); // the tiles are clear before any wave adds to them
@@ -892,7 +912,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
             k0s[q] = k0;
             offs[q] = total;
             cnts[q] = cnt;
-            total += cnt;
+            total += kAdjPh * ((cnt + kAdjPh - 1) / kAdjPh);
         }
         // one candidate ray per thread, its parameters in registers; `total` and the pass
         // sizes are the same for the whole workgroup
@@ -908,8 +928,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_adj2d_tile(unsigned long long *__
             int q = 0;
 #pragma unroll
             for (int u = 0; u < kAdjAngles; ++u)
-                if (ri >= offs[u] && ri < offs[u] + cnts[u]) q = u;
-            const size_t r = (size_t)(ag + q) * D + k0s[q] + (ri - offs[q]);
+                if (ri >= offs[u]) q = u;
+            const int pm = (cnts[q] + kAdjPh - 1) / kAdjPh;
+            const int pp = ri - offs[q];
+            const int rl = (pp % pm) * kAdjPh + pp / pm;
+            if (pm == 0 || rl >= cnts[q]) continue;
+            const size_t r = (size_t)(ag + q) * D + k0s[q] + rl;
             const float4 qn = ns[r];
             const int n = __float_as_int(qn.y);
             if (n <= 0) continue;
