@@ -811,9 +811,28 @@ __global__ __launch_bounds__(kMaxBlock) void k_bp3d_y(typename SV::raw *__restri
                                                       int nz, int U, int V, int A, float cx,
                                                       float cy, float cz) {
     const int nzt = (nz + kTileZ - 1) / kTileZ;
-    const int ix0 = blockIdx.x * kTileX, iy0 = blockIdx.y * kYB;
-    const int iz0 = (blockIdx.z % nzt) * kTileZ;
-    const size_t b = blockIdx.z / nzt;
+    // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, and logical index li gives
+    // each XCD a contiguous run of tiles (x fastest), whose panel footprints share L2
+    // (2.51 -> 2.37 ms at 256^3)
+    int ix0, iy0, iz0;
+    size_t b;
+    {
+        const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+        const unsigned nwg = gx * gy * gz;
+        const unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+        const unsigned n8 = nwg & ~7u;
+        const unsigned li = hw < n8 ? (hw & 7u) * (n8 >> 3) + (hw >> 3) : hw;
+        const unsigned per = gx * gy * (unsigned)nzt;
+        b = li / per;
+        const unsigned l = li % per;
+        unsigned tx, ty, tz;
+        tx = l % gx;
+        ty = (l / gx) % gy;
+        tz = l / (gx * gy);
+        ix0 = (int)tx * kTileX;
+        iy0 = (int)ty * kYB;
+        iz0 = (int)tz * kTileZ;
+    }
     const int ix = ix0 + (int)(threadIdx.x % kTileX), iz = iz0 + (int)(threadIdx.x / kTileX);
     const uint32_t stack = (uint32_t)(U + 2 * kPad) * (V + 2 * kPad) * A;
     const rsrc_t R = make_rsrc(ppad + b * (size_t)stack, bytes_of<SS>(stack));
