@@ -503,10 +503,17 @@ __global__ __launch_bounds__(kFwdRays) void k_fwd2d(const typename SI::raw *__re
     float4 *ext = sh.ext;
     int *red = sh.red, *flag = sh.flag;
     const int nthreads = blockDim.x;
-    const int k = blockIdx.x * nthreads + threadIdx.x;
-    const int a = blockIdx.y;
+    // The launch's linear workgroup index taken with angles fastest: consecutive workgroups
+    // (dealt to the 8 XCDs in turn) hold the same cells at successive angles, so every XCD
+    // gets every eighth angle of one cell block at a time, and workgroups resident together
+    // march rays of similar length (0.593 -> 0.556 ms at 1024^2; cell blocks fastest, each
+    // XCD a contiguous run: 0.635 ms).
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int a = (int)(hw % gy);
+    const int k = (int)((hw / gy) % gx) * nthreads + threadIdx.x;
+    const size_t b = hw / (gx * gy);
     const bool valid = k < D;
-    const size_t b = blockIdx.z;
     const size_t r = (size_t)a * D + k;
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float4 e = valid ? ev[r] : zero;
