@@ -202,8 +202,11 @@ constexpr int kBand = 31;
 constexpr int kBandRows = kBand + 1;   // staged rows: the band and the row below it
 // LDS row pitch in floats: a multiple of 4 (16-byte row stores) and below 255, so that one
 // address reaches a tap pair and the pair below it with two ds_read2_b32 immediate offsets.
-// 128 rays at up to 1.1 texels per cell plus a band's slope and the tap margins fit.
-constexpr int kPitch = 192;
+// 152 (19.5 KB per band) keeps eight workgroups per CU: 128 rays at up to about 0.9 texels per
+// cell plus a band's slope and the tap margins fit; wider bands take the global path. At
+// 1024^2 (0.67 texels per cell) a pitch of 152 gave 0.522 ms against 0.556 with 192 (six
+// workgroups per CU), 176: 0.534, 144: 0.556, 136: 0.590.
+constexpr int kPitch = 152;
 constexpr int kFwdRays = 128;          // rays (threads) per workgroup of the forward
 constexpr int kMaxWaves = kFwdRays / 64;
 
@@ -485,7 +488,7 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
 }
 
 // One workgroup = nthreads <= kFwdRays adjacent detector cells of one angle, one thread per
-// ray; LDS holds one band of kBandRows x kPitch floats (24.6 KB).
+// ray; LDS holds one band of kBandRows x kPitch floats (19.5 KB).
 // sino[b][a][k] = scale * sum_{j<N} bilinear(img, e + j v)   (forward_2d.hpp:63-78)
 template <class SI, class SS>
 __global__ __launch_bounds__(kFwdRays) void k_fwd2d(const typename SI::raw *__restrict__ scratch,
