@@ -950,7 +950,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_bpT3d(typename SS::raw *__restric
                      q2 = (double)m[6] - vc * (double)m[10], q3 = (double)m[7] - vc * (double)m[11];
         const double cr0 = a1 * q2 - a2 * q1, cr1 = a2 * q0 - a0 * q2, cr2 = a0 * q1 - a1 * q0;
         // components along K, F and G
-        auto pick = [](double x, double y, double z, int i) { return i == 0 ? x : (i == 1 ? y : z); };
+        auto pick = [](double x, double y, double z, int i) {
+            return i == 0 ? x : (i == 1 ? y : z);
+        };
         const double pF = pick(a0, a1, a2, F), pG = pick(a0, a1, a2, G), pK = pick(a0, a1, a2, K);
         const double qF = pick(q0, q1, q2, F), qG = pick(q0, q1, q2, G), qK = pick(q0, q1, q2, K);
         const double det = pF * qG - pG * qF; // +-cr[K]
