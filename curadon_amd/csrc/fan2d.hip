@@ -440,10 +440,10 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
                 }
             } else {
                 const int zc = P.c_lo + kPitch - 2; // the zero cells
-                for (; t + 1 <= P.b_hi; t += 2, jf += 2.0f * sf, rowa += 2 * kPitch) {
-                    float c[2], x[2][4];
+                for (; t + 3 <= P.b_hi; t += 4, jf += 4.0f * sf, rowa += 4 * kPitch) {
+                    float c[4], x[4][4];
 #pragma unroll
-                    for (int u = 0; u < 2; ++u) {
+                    for (int u = 0; u < 4; ++u) {
                         c[u] = fmaf(u == 0 ? jf : jf + (float)u * sf, vc, mc);
                         const int i = floor_i32(c[u]);
                         const bool in = (unsigned)(t + u) - ta_m <= span_m;
@@ -454,11 +454,11 @@ __device__ __forceinline__ float march_band(rsrc_t R, float4 e, float4 q, bool m
                         x[u][3] = p[kPitch + 1];
                     }
 #pragma unroll
-                    for (int u = 0; u < 2; ++u)
+                    for (int u = 0; u < 4; ++u)
                         sums.add(x[u][0], x[u][1], x[u][2], x[u][3],
                                  __builtin_amdgcn_fractf(c[u]));
                 }
-                if (t <= P.b_hi) {
+                for (; t <= P.b_hi; ++t, jf += sf, rowa += kPitch) {
                     const float c = fmaf(jf, vc, mc);
                     const int i = floor_i32(c);
                     const bool in = (unsigned)t - ta_m <= span_m;
