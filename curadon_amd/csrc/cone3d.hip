@@ -434,28 +434,55 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
             auto sample_slab = [&](auto km) {
                 constexpr bool KM = decltype(km)::value;
                 const int mstep = KM ? S : WPp, lstep = KM ? WPp : S;
+                auto in_box = [&](int kk, int ip, int iq) {
+                    return staged && (unsigned)kk < (unsigned)kSlab &&
+                           (unsigned)ip <= (unsigned)(WP - 2) && (unsigned)iq <= (unsigned)(WQ - 2);
+                };
+                auto box_sample = [&](float pK, float pf, float pq, int kk, int ip, int iq) {
+                    const float *c = &box[__mul24(kk, S) + __mul24(iq, WPp) + ip];
+                    const float ff = __builtin_amdgcn_fractf(pf);
+                    const float fm = __builtin_amdgcn_fractf(KM ? pK : pq);
+                    const float fl = __builtin_amdgcn_fractf(KM ? pq : pK);
+                    const float a00 = fmaf(ff, c[1] - c[0], c[0]);
+                    const float a10 = fmaf(ff, c[mstep + 1] - c[mstep], c[mstep]);
+                    const float a01 = fmaf(ff, c[lstep + 1] - c[lstep], c[lstep]);
+                    const float a11 =
+                        fmaf(ff, c[mstep + lstep + 1] - c[mstep + lstep], c[mstep + lstep]);
+                    const float b0 = fmaf(fm, a10 - a00, a00);
+                    const float b1 = fmaf(fm, a11 - a01, a01);
+                    return fmaf(fl, b1 - b0, b0);
+                };
+                // Along a ray every box coordinate is monotonic in jf (jf takes exact integer
+                // values), so a lane whose first and last samples of the slab lie in the box has
+                // all of them there. When every lane of the wavefront is such a lane, it runs the
+                // loop without the per-sample test: the same samples from the same cells, added
+                // in the same order.
+                bool inside = true;
+                if (j < je) {
+                    const float jl = jf + (float)(je - 1 - j);
+                    inside = in_box(floor_i32(fmaf(jf, tK, mK)) + kofs,
+                                    floor_i32(fmaf(jf, tP, mP)) + pofs,
+                                    floor_i32(fmaf(jf, tQ, mQ)) + qofs) &&
+                             in_box(floor_i32(fmaf(jl, tK, mK)) + kofs,
+                                    floor_i32(fmaf(jl, tP, mP)) + pofs,
+                                    floor_i32(fmaf(jl, tQ, mQ)) + qofs);
+                }
                 float part = 0.0f;
-                for (; j < je; ++j, jf += 1.0f) {
-                    const float pK = fmaf(jf, tK, mK);
-                    const float pf = fmaf(jf, tP, mP), pq = fmaf(jf, tQ, mQ);
-                    const int kk = floor_i32(pK) + kofs;
-                    const int ip = floor_i32(pf) + pofs, iq = floor_i32(pq) + qofs;
-                    if (staged && (unsigned)kk < (unsigned)kSlab &&
-                        (unsigned)ip <= (unsigned)(WP - 2) && (unsigned)iq <= (unsigned)(WQ - 2)) {
-                        const float *c = &box[__mul24(kk, S) + __mul24(iq, WPp) + ip];
-                        const float ff = __builtin_amdgcn_fractf(pf);
-                        const float fm = __builtin_amdgcn_fractf(KM ? pK : pq);
-                        const float fl = __builtin_amdgcn_fractf(KM ? pq : pK);
-                        const float a00 = fmaf(ff, c[1] - c[0], c[0]);
-                        const float a10 = fmaf(ff, c[mstep + 1] - c[mstep], c[mstep]);
-                        const float a01 = fmaf(ff, c[lstep + 1] - c[lstep], c[lstep]);
-                        const float a11 = fmaf(ff, c[mstep + lstep + 1] - c[mstep + lstep],
-                                               c[mstep + lstep]);
-                        const float b0 = fmaf(fm, a10 - a00, a00);
-                        const float b1 = fmaf(fm, a11 - a01, a01);
-                        part += fmaf(fl, b1 - b0, b0);
-                    } else {
-                        part += global_sample();
+                if (__builtin_amdgcn_ballot_w64(!inside) == 0) {
+                    for (; j < je; ++j, jf += 1.0f) {
+                        const float pK = fmaf(jf, tK, mK);
+                        const float pf = fmaf(jf, tP, mP), pq = fmaf(jf, tQ, mQ);
+                        part += box_sample(pK, pf, pq, floor_i32(pK) + kofs,
+                                           floor_i32(pf) + pofs, floor_i32(pq) + qofs);
+                    }
+                } else {
+                    for (; j < je; ++j, jf += 1.0f) {
+                        const float pK = fmaf(jf, tK, mK);
+                        const float pf = fmaf(jf, tP, mP), pq = fmaf(jf, tQ, mQ);
+                        const int kk = floor_i32(pK) + kofs;
+                        const int ip = floor_i32(pf) + pofs, iq = floor_i32(pq) + qofs;
+                        if (in_box(kk, ip, iq)) part += box_sample(pK, pf, pq, kk, ip, iq);
+                        else part += global_sample();
                     }
                 }
                 acc += part;
