@@ -438,16 +438,23 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
                     return staged && (unsigned)kk < (unsigned)kSlab &&
                            (unsigned)ip <= (unsigned)(WP - 2) && (unsigned)iq <= (unsigned)(WQ - 2);
                 };
+                // strides in bytes, so that a sample's address is two 24-bit products and one
+                // shifted add
+                const int S4 = 4 * S, W4 = 4 * WPp, m4 = 4 * mstep, l4 = 4 * lstep;
                 auto box_sample = [&](float pK, float pf, float pq, int kk, int ip, int iq) {
-                    const float *c = &box[__mul24(kk, S) + __mul24(iq, WPp) + ip];
+                    const char *cb = reinterpret_cast<const char *>(box) + __mul24(kk, S4) +
+                                     __mul24(iq, W4) + 4 * ip;
+                    const float *c = reinterpret_cast<const float *>(cb);
+                    const float *cm = reinterpret_cast<const float *>(cb + m4);
+                    const float *cl = reinterpret_cast<const float *>(cb + l4);
+                    const float *cml = reinterpret_cast<const float *>(cb + m4 + l4);
                     const float ff = __builtin_amdgcn_fractf(pf);
                     const float fm = __builtin_amdgcn_fractf(KM ? pK : pq);
                     const float fl = __builtin_amdgcn_fractf(KM ? pq : pK);
                     const float a00 = fmaf(ff, c[1] - c[0], c[0]);
-                    const float a10 = fmaf(ff, c[mstep + 1] - c[mstep], c[mstep]);
-                    const float a01 = fmaf(ff, c[lstep + 1] - c[lstep], c[lstep]);
-                    const float a11 =
-                        fmaf(ff, c[mstep + lstep + 1] - c[mstep + lstep], c[mstep + lstep]);
+                    const float a10 = fmaf(ff, cm[1] - cm[0], cm[0]);
+                    const float a01 = fmaf(ff, cl[1] - cl[0], cl[0]);
+                    const float a11 = fmaf(ff, cml[1] - cml[0], cml[0]);
                     const float b0 = fmaf(fm, a10 - a00, a00);
                     const float b1 = fmaf(fm, a11 - a01, a01);
                     return fmaf(fl, b1 - b0, b0);
@@ -469,12 +476,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
                 }
                 float part = 0.0f;
                 if (__builtin_amdgcn_ballot_w64(!inside) == 0) {
-                    for (; j < je; ++j, jf += 1.0f) {
+                    // counted by jf alone (exact integers)
+                    const float jfe = jf + (float)max(je - j, 0);
+                    for (; jf < jfe; jf += 1.0f) {
                         const float pK = fmaf(jf, tK, mK);
                         const float pf = fmaf(jf, tP, mP), pq = fmaf(jf, tQ, mQ);
                         part += box_sample(pK, pf, pq, floor_i32(pK) + kofs,
                                            floor_i32(pf) + pofs, floor_i32(pq) + qofs);
                     }
+                    j = max(j, je);
                 } else {
                     for (; j < je; ++j, jf += 1.0f) {
                         const float pK = fmaf(jf, tK, mK);
