@@ -441,9 +441,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
                 // strides in bytes, so that a sample's address is two 24-bit products and one
                 // shifted add
                 const int S4 = 4 * S, W4 = 4 * WPp, m4 = 4 * mstep, l4 = 4 * lstep;
-                auto box_sample = [&](float pK, float pf, float pq, int kk, int ip, int iq) {
-                    const char *cb = reinterpret_cast<const char *>(box) + __mul24(kk, S4) +
-                                     __mul24(iq, W4) + 4 * ip;
+                // the sample at byte offset `ofs` of the box (its lower tap pair)
+                auto box_at = [&](float pK, float pf, float pq, int ofs) {
+                    const char *cb = reinterpret_cast<const char *>(box) + ofs;
                     const float *c = reinterpret_cast<const float *>(cb);
                     const float *cm = reinterpret_cast<const float *>(cb + m4);
                     const float *cl = reinterpret_cast<const float *>(cb + l4);
@@ -458,6 +458,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
                     const float b0 = fmaf(fm, a10 - a00, a00);
                     const float b1 = fmaf(fm, a11 - a01, a01);
                     return fmaf(fl, b1 - b0, b0);
+                };
+                auto box_sample = [&](float pK, float pf, float pq, int kk, int ip, int iq) {
+                    return box_at(pK, pf, pq, __mul24(kk, S4) + __mul24(iq, W4) + 4 * ip);
                 };
                 // Along a ray every box coordinate is monotonic in jf (jf takes exact integer
                 // values), so a lane whose first and last samples of the slab lie in the box has
@@ -476,13 +479,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_fwd3d(const typename SI::raw *__r
                 }
                 float part = 0.0f;
                 if (__builtin_amdgcn_ballot_w64(!inside) == 0) {
-                    // counted by jf alone (exact integers)
+                    // counted by jf alone (exact integers); the box offsets of the copy's
+                    // coordinates folded into one wave-uniform byte offset
                     const float jfe = jf + (float)max(je - j, 0);
+                    const int ofs0 = kofs * S4 + qofs * W4 + 4 * pofs;
                     for (; jf < jfe; jf += 1.0f) {
                         const float pK = fmaf(jf, tK, mK);
                         const float pf = fmaf(jf, tP, mP), pq = fmaf(jf, tQ, mQ);
-                        part += box_sample(pK, pf, pq, floor_i32(pK) + kofs,
-                                           floor_i32(pf) + pofs, floor_i32(pq) + qofs);
+                        part += box_at(pK, pf, pq,
+                                       __mul24(floor_i32(pK), S4) + __mul24(floor_i32(pq), W4) +
+                                           4 * floor_i32(pf) + ofs0);
                     }
                     j = max(j, je);
                 } else {
